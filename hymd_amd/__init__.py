@@ -1,0 +1,15 @@
+"""MI355X-native HyMD particle-field hot path.
+
+Drop-in modules over ``libhymd_amd.so`` (hand-written gfx950 HIP kernels
+behind the C-ABI in include/hymd_amd.h):
+
+* ``hymd_amd.field``          field.py: initialize_pm, update_field,
+                               compute_field_force, update_field_force_q,
+                               compute_self_energy_q, compute_field_and_kinetic_energy
+* ``hymd_amd.hamiltonian``    hamiltonian.py: DefaultNoChi, DefaultWithChi, SquaredPhi
+* ``hymd_amd.force``          force.py compiled kernels: compute_*_forces__fortran(+__double),
+                               dipole_forces_redistribution; ``force_kernels`` = cbf/caf/cdf(+_d)
+* ``hymd_amd.md``             the rRESPA loop of main.py on the device
+* ``hymd_amd.synth``          seeded synthetic systems
+"""
+__version__ = "0.1.0"

@@ -1,0 +1,678 @@
+// Bonded kernels (cbf / caf / cdf and their _d twins), dipole force
+// redistribution, and the persistent topology used by the rRESPA loop.
+//
+// Per-particle term lists (CSR) are built on the device: count members,
+// exclusive scan, fill with atomics, then sort every short list by
+// (term, role) so each particle sums its contributions in the reference's
+// term order.  One thread per particle then recomputes its terms in FP64.
+#include <cmath>
+#include <mutex>
+
+#include "bonded.h"
+
+namespace hymd {
+
+// ---------------------------------------------------------------------------
+// CSR build
+// ---------------------------------------------------------------------------
+struct TermIdx {
+    const int32_t *m[4];  // member arrays (roles 0..nroles-1)
+    int nroles;
+};
+
+__global__ void csr_count_kernel(TermIdx ti, int64_t m, int64_t n, int *__restrict__ counts, int *__restrict__ err) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < m; t += stride)
+        for (int r = 0; r < ti.nroles; ++r) {
+            const int32_t p = ti.m[r][t];
+            if (p < 0 || (int64_t)p >= n) {
+                atomicOr(err, 1);
+                continue;
+            }
+            atomicAdd(counts + p, 1);
+        }
+}
+
+__global__ void csr_fill_kernel(TermIdx ti, int64_t m, int64_t n, int *__restrict__ cursor,
+                                uint32_t *__restrict__ entries) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < m; t += stride)
+        for (int r = 0; r < ti.nroles; ++r) {
+            const int32_t p = ti.m[r][t];
+            if (p < 0 || (int64_t)p >= n) continue;
+            const int pos = atomicAdd(cursor + p, 1);
+            entries[pos] = ((uint32_t)t << 2) | (uint32_t)r;
+        }
+}
+
+__global__ void csr_sort_kernel(const int *__restrict__ off, int64_t n, uint32_t *__restrict__ entries) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        const int b = off[p], e = off[p + 1];
+        for (int i = b + 1; i < e; ++i) {
+            const uint32_t key = entries[i];
+            int j = i - 1;
+            while (j >= b && entries[j] > key) {
+                entries[j + 1] = entries[j];
+                --j;
+            }
+            entries[j + 1] = key;
+        }
+    }
+}
+
+// exclusive scan, three phases (block totals, scan of totals, local scans)
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kBlock * kScanItems;
+
+__device__ int block_exclusive_scan(int v, int *sh, int &total) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+        int add = (int)threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+        __syncthreads();
+        sh[threadIdx.x] += add;
+        __syncthreads();
+    }
+    total = sh[blockDim.x - 1];
+    const int incl = sh[threadIdx.x];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void scan_tiles_kernel(const int *__restrict__ in, int *__restrict__ out, int64_t n,
+                                  int *__restrict__ tile_sums) {
+    __shared__ int sh[kBlock];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int v[kScanItems], s = 0;
+    for (int j = 0; j < kScanItems; ++j) {
+        v[j] = (base + j < n) ? in[base + j] : 0;
+        s += v[j];
+    }
+    int total;
+    int run = block_exclusive_scan(s, sh, total);
+    for (int j = 0; j < kScanItems; ++j) {
+        if (base + j < n) out[base + j] = run;
+        run += v[j];
+    }
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+__global__ void scan_sums_kernel(int *__restrict__ sums, int ntiles, int *__restrict__ grand_total) {
+    __shared__ int sh[kBlock];
+    const int per = (ntiles + blockDim.x - 1) / blockDim.x;
+    const int b = threadIdx.x * per;
+    int s = 0;
+    for (int j = 0; j < per; ++j)
+        if (b + j < ntiles) s += sums[b + j];
+    int total;
+    int run = block_exclusive_scan(s, sh, total);
+    for (int j = 0; j < per; ++j)
+        if (b + j < ntiles) {
+            const int v = sums[b + j];
+            sums[b + j] = run;
+            run += v;
+        }
+    if (threadIdx.x == 0) *grand_total = total;
+}
+
+__global__ void scan_add_kernel(int *__restrict__ out, int64_t n, const int *__restrict__ sums,
+                                const int *__restrict__ grand_total) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += sums[i / kScanTile];
+    if (i == n) out[n] = *grand_total;
+}
+
+struct Csr {
+    int64_t n = 0, m = 0;
+    DevBuf counts;   // n ints (reused as cursor)
+    DevBuf offsets;  // n+1 ints
+    DevBuf entries;  // nroles*m uint32
+    DevBuf tiles;    // scan tile sums + grand total + error flag
+    const int *off() const { return (const int *)offsets.ptr; }
+    const uint32_t *ent() const { return (const uint32_t *)entries.ptr; }
+};
+
+static int build_csr(Csr &c, const TermIdx &ti, int64_t m, int64_t n, hipStream_t s) {
+    HYMD_REQUIRE(m < (int64_t(1) << 29), "too many terms (%lld)", (long long)m);
+    HYMD_REQUIRE(m * ti.nroles < (int64_t(1) << 31), "too many term members");
+    c.n = n;
+    c.m = m;
+    const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    HYMD_TRY(c.counts.reserve(sizeof(int) * std::max<int64_t>(n, 1)));
+    HYMD_TRY(c.offsets.reserve(sizeof(int) * (n + 1)));
+    HYMD_TRY(c.entries.reserve(sizeof(uint32_t) * std::max<int64_t>(m * ti.nroles, 1)));
+    HYMD_TRY(c.tiles.reserve(sizeof(int) * (ntiles + 2)));
+    int *counts = (int *)c.counts.ptr, *off = (int *)c.offsets.ptr, *tiles = (int *)c.tiles.ptr;
+    int *grand = tiles + ntiles, *err = tiles + ntiles + 1;
+    HYMD_HIP(hipMemsetAsync(counts, 0, sizeof(int) * std::max<int64_t>(n, 1), s));
+    HYMD_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
+    if (m > 0) csr_count_kernel<<<grid_for(m), kBlock, 0, s>>>(ti, m, n, counts, err);
+    if (n > 0) scan_tiles_kernel<<<(unsigned)ntiles, kBlock, 0, s>>>(counts, off, n, tiles);
+    scan_sums_kernel<<<1, kBlock, 0, s>>>(tiles, (int)ntiles, grand);
+    scan_add_kernel<<<grid_for(n + 1), kBlock, 0, s>>>(off, n, tiles, grand);
+    HYMD_HIP(hipGetLastError());
+    if (n > 0) HYMD_HIP(hipMemcpyAsync(counts, off, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+    if (m > 0) csr_fill_kernel<<<grid_for(m), kBlock, 0, s>>>(ti, m, n, counts, (uint32_t *)c.entries.ptr);
+    if (n > 0) csr_sort_kernel<<<grid_for(n), kBlock, 0, s>>>(off, n, (uint32_t *)c.entries.ptr);
+    HYMD_HIP(hipGetLastError());
+    int herr = 0;
+    HYMD_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+    HYMD_HIP(hipStreamSynchronize(s));
+    HYMD_REQUIRE(herr == 0, "a bonded-term particle index is outside [0, %lld)", (long long)n);
+    return HYMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// force kernels (gather)
+// ---------------------------------------------------------------------------
+struct BondArgs {
+    const int32_t *a, *b;
+    const double *r0, *k;
+};
+struct AngleArgs {
+    const int32_t *a, *b, *c;
+    const double *t0, *k;
+};
+
+template <typename PosT>
+__device__ __forceinline__ V3 bond_contrib(const PosT *__restrict__ r, const BondArgs &B, uint32_t e, const Box &box,
+                                           double *ev) {
+    const int64_t t = e >> 2;
+    const int role = e & 3;
+    const V3 rab = mic(load3(r, B.b[t]) - load3(r, B.a[t]), box);
+    double en;
+    const V3 fa = bond_term(rab, B.r0[t], B.k[t], en);
+    if (role == 0) {
+        ev[0] += en;
+        ev[1] += fa.x * rab.x;
+        ev[2] += fa.y * rab.y;
+        ev[3] += fa.z * rab.z;
+        return -fa;
+    }
+    return fa;
+}
+
+template <typename PosT>
+__device__ __forceinline__ V3 angle_contrib(const PosT *__restrict__ r, const AngleArgs &A, uint32_t e, const Box &box,
+                                            double *ev) {
+    const int64_t t = e >> 2;
+    const int role = e & 3;
+    const V3 rb = load3(r, A.b[t]);
+    const V3 ra = mic(load3(r, A.a[t]) - rb, box);
+    const V3 rc = mic(load3(r, A.c[t]) - rb, box);
+    V3 fa, fc;
+    double en;
+    if (!angle_term(ra, rc, A.t0[t], A.k[t], fa, fc, en)) return v3(0, 0, 0);
+    if (role == 0) {
+        ev[0] += en;
+        ev[1] -= fa.x * ra.x + fc.x * rc.x;
+        ev[2] -= fa.y * ra.y + fc.y * rc.y;
+        ev[3] -= fa.z * ra.z + fc.z * rc.z;
+        return -fa;
+    }
+    if (role == 2) return -fc;
+    return fa + fc;
+}
+
+template <typename PosT, typename OutT>
+__global__ void __launch_bounds__(kBlock) bond_gather_kernel(const PosT *__restrict__ r, OutT *__restrict__ f,
+                                                              int64_t n, const int *__restrict__ off,
+                                                              const uint32_t *__restrict__ ent, BondArgs B, Box box,
+                                                              double *__restrict__ partials) {
+    double ev[4] = {0, 0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        V3 acc = v3(0, 0, 0);
+        for (int j = off[p]; j < off[p + 1]; ++j) acc = acc + bond_contrib(r, B, ent[j], box, ev);
+        f[3 * p] = (OutT)acc.x;
+        f[3 * p + 1] = (OutT)acc.y;
+        f[3 * p + 2] = (OutT)acc.z;
+    }
+    if (partials) block_reduce_store<4>(ev, partials + 4 * blockIdx.x);
+}
+
+template <typename PosT, typename OutT>
+__global__ void __launch_bounds__(kBlock) angle_gather_kernel(const PosT *__restrict__ r, OutT *__restrict__ f,
+                                                               int64_t n, const int *__restrict__ off,
+                                                               const uint32_t *__restrict__ ent, AngleArgs A, Box box,
+                                                               double *__restrict__ partials) {
+    double ev[4] = {0, 0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        V3 acc = v3(0, 0, 0);
+        for (int j = off[p]; j < off[p + 1]; ++j) acc = acc + angle_contrib(r, A, ent[j], box, ev);
+        f[3 * p] = (OutT)acc.x;
+        f[3 * p + 1] = (OutT)acc.y;
+        f[3 * p + 2] = (OutT)acc.z;
+    }
+    if (partials) block_reduce_store<4>(ev, partials + 4 * blockIdx.x);
+}
+
+// bonds + angles of the persistent topology in one pass (float64 positions,
+// float32 total bonded force)
+__global__ void __launch_bounds__(kBlock) topo_gather_kernel(const double *__restrict__ r, float *__restrict__ f,
+                                                              int64_t n, const int *__restrict__ off2,
+                                                              const uint32_t *__restrict__ ent2, BondArgs B,
+                                                              const int *__restrict__ off3,
+                                                              const uint32_t *__restrict__ ent3, AngleArgs A, Box box,
+                                                              double *__restrict__ partials) {
+    double ev[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        V3 acc = v3(0, 0, 0);
+        if (off2)
+            for (int j = off2[p]; j < off2[p + 1]; ++j) acc = acc + bond_contrib(r, B, ent2[j], box, ev);
+        if (off3)
+            for (int j = off3[p]; j < off3[p + 1]; ++j) acc = acc + angle_contrib(r, A, ent3[j], box, ev + 4);
+        f[3 * p] = (float)acc.x;
+        f[3 * p + 1] = (float)acc.y;
+        f[3 * p + 2] = (float)acc.z;
+    }
+    if (partials) block_reduce_store<8>(ev, partials + 8 * blockIdx.x);
+}
+
+struct DihArgs {
+    const int32_t *a, *b, *c, *d, *dtype, *last;
+    const float *coeff;
+    int dipole_flag;
+};
+
+// One dihedral term (compute_dihedral_forces.f90:35-134) evaluated for the
+// particle in `role`; the role-0 thread also owns the term's energy, clamp
+// count and dipole / transfer-matrix outputs.
+template <typename PosT>
+__device__ V3 dihedral_contrib(const PosT *__restrict__ r, const DihArgs &D, uint32_t e, const Box &box, double *ev,
+                               PosT *__restrict__ dipoles, PosT *__restrict__ tmat) {
+    const int64_t t = e >> 2;
+    const int role = e & 3;
+    const V3 ra = load3(r, D.a[t]), rb = load3(r, D.b[t]), rc = load3(r, D.c[t]), rd = load3(r, D.d[t]);
+    const V3 f = mic(ra - rb, box), g = mic(rb - rc, box), h = mic(rd - rc, box);
+    const V3 v = cross(f, g), w = cross(h, g);
+    const double v_sq = dot(v, v), w_sq = dot(w, w), g_norm = norm(g);
+    const double cos_phi = dot(v, w);
+    const double sin_phi = dot(w, f) * g_norm;
+    const double phi = atan2(sin_phi, cos_phi);
+    const double f_dot_g = dot(f, g), h_dot_g = dot(h, g);
+    const float *cf = D.coeff + 30 * t;
+    const int dt = D.dtype[t];
+    double energy = 0.0, df = 0.0;
+    V3 out = v3(0, 0, 0);
+    if (dt == 0 || dt == 1) {
+        cosine_series(cf, cf + 5, phi, energy, df);
+        int zc = 0, zd = 0;
+        for (int i = 0; i < 5; ++i) {
+            zc += (cf[10 + i] == 0.f);
+            zd += (cf[15 + i] == 0.f);
+        }
+        if (zc != 5 && zd != 5) cosine_series(cf + 10, cf + 15, phi, energy, df);
+    }
+    const bool own = (role == 0);
+    if (dt == 1) {
+        Recon rec;
+        reconstruct(f, rb, -g, box, cf + 20, cf + 25, phi, D.dipole_flag, rec);
+        energy += rec.energy;
+        df += rec.df;
+        if (role == 0) out = out - rec.fa;
+        if (role == 1) out = out - rec.fb;
+        if (role == 2) out = out - rec.fc;
+        if (own) {
+            ev[1] += rec.clamped ? 1.0 : 0.0;
+            if (rec.has_dipole) {
+                for (int p = 0; p < 2; ++p) {
+                    dipoles[12 * t + 3 * p + 0] = (PosT)rec.dip[p].x;
+                    dipoles[12 * t + 3 * p + 1] = (PosT)rec.dip[p].y;
+                    dipoles[12 * t + 3 * p + 2] = (PosT)rec.dip[p].z;
+                }
+                for (int x = 0; x < 3; ++x)
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) tmat[54 * t + 9 * x + 3 * i + j] = (PosT)rec.tm[x][i][j];
+            }
+        }
+        if (D.last[t] == 1) {
+            Recon rec2;
+            reconstruct(g, rc, h, box, cf + 20, cf + 25, phi, D.dipole_flag, rec2);
+            energy += rec2.energy;
+            df += rec2.df;
+            if (role == 1) out = out - rec2.fa;
+            if (role == 2) out = out - rec2.fb;
+            if (role == 3) out = out - rec2.fc;
+            if (own) {
+                ev[1] += rec2.clamped ? 1.0 : 0.0;
+                if (rec2.has_dipole) {
+                    for (int p = 0; p < 2; ++p) {
+                        dipoles[12 * t + 6 + 3 * p + 0] = (PosT)rec2.dip[p].x;
+                        dipoles[12 * t + 6 + 3 * p + 1] = (PosT)rec2.dip[p].y;
+                        dipoles[12 * t + 6 + 3 * p + 2] = (PosT)rec2.dip[p].z;
+                    }
+                    for (int x = 0; x < 3; ++x)
+                        for (int i = 0; i < 3; ++i)
+                            for (int j = 0; j < 3; ++j)
+                                tmat[54 * t + 27 + 9 * x + 3 * i + j] = (PosT)rec2.tm[x][i][j];
+                }
+            }
+        }
+    }
+    if (dt == 2) {
+        const double eq = (double)cf[0], kf = (double)cf[1];
+        df = kf * (phi - eq);
+        energy += 0.5 * kf * ((phi - eq) * (phi - eq));
+    }
+    const V3 sc = (f_dot_g / (v_sq * g_norm)) * v - (h_dot_g / (w_sq * g_norm)) * w;
+    const V3 fa = (-df * g_norm / v_sq) * v;
+    const V3 fd = (df * g_norm / w_sq) * w;
+    if (role == 0) out = out + fa;
+    if (role == 1) out = out + (df * sc - fa);
+    if (role == 2) out = out + (-df * sc - fd);
+    if (role == 3) out = out + fd;
+    if (own) ev[0] += energy;
+    return out;
+}
+
+template <typename PosT>
+__global__ void __launch_bounds__(kBlock) dihedral_gather_kernel(const PosT *__restrict__ r, PosT *__restrict__ f,
+                                                                  int64_t n, const int *__restrict__ off,
+                                                                  const uint32_t *__restrict__ ent, DihArgs D, Box box,
+                                                                  PosT *__restrict__ dipoles, PosT *__restrict__ tmat,
+                                                                  double *__restrict__ partials) {
+    double ev[2] = {0, 0};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        V3 acc = v3(0, 0, 0);
+        for (int j = off[p]; j < off[p + 1]; ++j) acc = acc + dihedral_contrib(r, D, ent[j], box, ev, dipoles, tmat);
+        f[3 * p] = (PosT)acc.x;
+        f[3 * p + 1] = (PosT)acc.y;
+        f[3 * p + 2] = (PosT)acc.z;
+    }
+    if (partials) block_reduce_store<2>(ev, partials + 2 * blockIdx.x);
+}
+
+// force.py:855-877 as a gather over the dihedral lists
+template <typename PosT>
+__global__ void __launch_bounds__(kBlock) redistribution_kernel(PosT *__restrict__ out, int64_t n,
+                                                                 const int *__restrict__ off,
+                                                                 const uint32_t *__restrict__ ent,
+                                                                 const PosT *__restrict__ fdip,
+                                                                 const PosT *__restrict__ tmat,
+                                                                 const int32_t *__restrict__ dtype,
+                                                                 const int32_t *__restrict__ last) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
+        double acc[3] = {0, 0, 0};
+        for (int j = off[p]; j < off[p + 1]; ++j) {
+            const int64_t t = ent[j] >> 2;
+            const int role = ent[j] & 3;
+            if (dtype[t] != 1) continue;
+            const int npairs = last[t] == 1 ? 2 : 1;
+            for (int pair = 0; pair < npairs; ++pair) {
+                // pair 0 acts on (a,b,c) = roles (0,1,2); pair 1 on (b,c,d) = roles (1,2,3)
+                const int x = role - pair;
+                if (x < 0 || x > 2) continue;
+                const PosT *f0 = fdip + 12 * t + 6 * pair;
+                const PosT *f1 = f0 + 3;
+                const PosT *M = tmat + 54 * t + 27 * pair + 9 * x;
+                double diff[3], sum[3];
+                for (int q = 0; q < 3; ++q) {
+                    diff[q] = (double)f0[q] - (double)f1[q];
+                    sum[q] = (double)f0[q] + (double)f1[q];
+                }
+                for (int i = 0; i < 3; ++i) {
+                    double s = (double)M[3 * i] * diff[0] + (double)M[3 * i + 1] * diff[1] + (double)M[3 * i + 2] * diff[2];
+                    if (x > 0) s += 0.5 * sum[i];
+                    acc[i] += s;
+                }
+            }
+        }
+        out[3 * p] = (PosT)acc[0];
+        out[3 * p + 1] = (PosT)acc[1];
+        out[3 * p + 2] = (PosT)acc[2];
+    }
+}
+
+// Workspace of the stateless f2py-style entry points (rebuilt every call, as
+// the Fortran kernels recompute everything; kept for the process lifetime).
+struct Workspace {
+    std::mutex mu;
+    Csr csr;
+    DevBuf partials, result;
+    HostBuf host;
+};
+static Workspace &workspace() {
+    static Workspace *w = new Workspace();  // never freed: outlives the HIP runtime teardown
+    return *w;
+}
+
+static int finish_reduce(Workspace &w, int nblocks, int nv, double *out, hipStream_t s) {
+    HYMD_TRY(w.result.reserve(sizeof(double) * nv));
+    HYMD_TRY(w.host.reserve(sizeof(double) * nv));
+    HYMD_TRY(reduce_partials((const double *)w.partials.ptr, nblocks, nv, (double *)w.result.ptr, s));
+    HYMD_HIP(hipMemcpyAsync(w.host.ptr, w.result.ptr, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+    HYMD_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < nv; ++j) out[j] = ((double *)w.host.ptr)[j];
+    return HYMD_OK;
+}
+
+static Box make_box(const double box[3]) {
+    Box b;
+    for (int d = 0; d < 3; ++d) b.l[d] = box[d];
+    return b;
+}
+
+}  // namespace hymd
+
+using namespace hymd;
+
+extern "C" int hymd_cbf(void *f, const void *r, int64_t n, const double box[3], const int32_t *a, const int32_t *b,
+                        const double *r0, const double *k, int64_t m, int32_t precision, double *energy,
+                        double *bond_pr, void *stream) {
+    HYMD_REQUIRE(f && box && n >= 0 && m >= 0 && (r || n == 0), "bad argument");
+    HYMD_REQUIRE(m == 0 || (a && b && r0 && k), "null term array");
+    HYMD_REQUIRE(precision == HYMD_F32 || precision == HYMD_F64, "precision must be HYMD_F32 or HYMD_F64");
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &w = workspace();
+    std::lock_guard<std::mutex> lock(w.mu);
+    TermIdx ti{{a, b, nullptr, nullptr}, 2};
+    HYMD_TRY(build_csr(w.csr, ti, m, n, s));
+    const int nb = grid_for(n, kBlock, 2048);
+    const bool red = energy || bond_pr;
+    if (red) HYMD_TRY(w.partials.reserve(sizeof(double) * 4 * nb));
+    double *part = red ? (double *)w.partials.ptr : nullptr;
+    BondArgs B{a, b, r0, k};
+    Box bx = make_box(box);
+    if (precision == HYMD_F32)
+        bond_gather_kernel<float, float><<<nb, kBlock, 0, s>>>((const float *)r, (float *)f, n, w.csr.off(),
+                                                                w.csr.ent(), B, bx, part);
+    else
+        bond_gather_kernel<double, double><<<nb, kBlock, 0, s>>>((const double *)r, (double *)f, n, w.csr.off(),
+                                                                  w.csr.ent(), B, bx, part);
+    HYMD_HIP(hipGetLastError());
+    if (red) {
+        double out[4];
+        HYMD_TRY(finish_reduce(w, nb, 4, out, s));
+        if (energy) *energy = out[0];
+        if (bond_pr)
+            for (int d = 0; d < 3; ++d) bond_pr[d] = out[1 + d];
+    }
+    return HYMD_OK;
+}
+
+extern "C" int hymd_caf(void *f, const void *r, int64_t n, const double box[3], const int32_t *a, const int32_t *b,
+                        const int32_t *c, const double *t0, const double *k, int64_t m, int32_t precision,
+                        double *energy, double *angle_pr, void *stream) {
+    HYMD_REQUIRE(f && box && n >= 0 && m >= 0 && (r || n == 0), "bad argument");
+    HYMD_REQUIRE(m == 0 || (a && b && c && t0 && k), "null term array");
+    HYMD_REQUIRE(precision == HYMD_F32 || precision == HYMD_F64, "precision must be HYMD_F32 or HYMD_F64");
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &w = workspace();
+    std::lock_guard<std::mutex> lock(w.mu);
+    TermIdx ti{{a, b, c, nullptr}, 3};
+    HYMD_TRY(build_csr(w.csr, ti, m, n, s));
+    const int nb = grid_for(n, kBlock, 2048);
+    const bool red = energy || angle_pr;
+    if (red) HYMD_TRY(w.partials.reserve(sizeof(double) * 4 * nb));
+    double *part = red ? (double *)w.partials.ptr : nullptr;
+    AngleArgs A{a, b, c, t0, k};
+    Box bx = make_box(box);
+    if (precision == HYMD_F32)
+        angle_gather_kernel<float, float><<<nb, kBlock, 0, s>>>((const float *)r, (float *)f, n, w.csr.off(),
+                                                                 w.csr.ent(), A, bx, part);
+    else
+        angle_gather_kernel<double, double><<<nb, kBlock, 0, s>>>((const double *)r, (double *)f, n, w.csr.off(),
+                                                                   w.csr.ent(), A, bx, part);
+    HYMD_HIP(hipGetLastError());
+    if (red) {
+        double out[4];
+        HYMD_TRY(finish_reduce(w, nb, 4, out, s));
+        if (energy) *energy = out[0];
+        if (angle_pr)
+            for (int d = 0; d < 3; ++d) angle_pr[d] = out[1 + d];
+    }
+    return HYMD_OK;
+}
+
+extern "C" int hymd_cdf(void *f, const void *r, void *dipoles, void *transfer_matrix, int64_t n, const double box[3],
+                        const int32_t *a, const int32_t *b, const int32_t *c, const int32_t *d, const float *coeff,
+                        const int32_t *dtype, const int32_t *bb_index, int64_t m, int32_t dipole_flag,
+                        int32_t precision, double *energy, int64_t *n_clamped, void *stream) {
+    HYMD_REQUIRE(f && box && n >= 0 && m >= 0 && (r || n == 0), "bad argument");
+    HYMD_REQUIRE(m == 0 || (a && b && c && d && coeff && dtype && bb_index && dipoles && transfer_matrix),
+                 "null term array");
+    HYMD_REQUIRE(precision == HYMD_F32 || precision == HYMD_F64, "precision must be HYMD_F32 or HYMD_F64");
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &w = workspace();
+    std::lock_guard<std::mutex> lock(w.mu);
+    const size_t esz = precision == HYMD_F32 ? sizeof(float) : sizeof(double);
+    if (m > 0) {
+        HYMD_HIP(hipMemsetAsync(dipoles, 0, esz * 12 * m, s));
+        HYMD_HIP(hipMemsetAsync(transfer_matrix, 0, esz * 54 * m, s));
+    }
+    TermIdx ti{{a, b, c, d}, 4};
+    HYMD_TRY(build_csr(w.csr, ti, m, n, s));
+    const int nb = grid_for(n, kBlock, 2048);
+    const bool red = energy || n_clamped;
+    if (red) HYMD_TRY(w.partials.reserve(sizeof(double) * 2 * nb));
+    double *part = red ? (double *)w.partials.ptr : nullptr;
+    DihArgs D{a, b, c, d, dtype, bb_index, coeff, dipole_flag};
+    Box bx = make_box(box);
+    if (precision == HYMD_F32)
+        dihedral_gather_kernel<float><<<nb, kBlock, 0, s>>>((const float *)r, (float *)f, n, w.csr.off(), w.csr.ent(),
+                                                            D, bx, (float *)dipoles, (float *)transfer_matrix, part);
+    else
+        dihedral_gather_kernel<double><<<nb, kBlock, 0, s>>>((const double *)r, (double *)f, n, w.csr.off(),
+                                                             w.csr.ent(), D, bx, (double *)dipoles,
+                                                             (double *)transfer_matrix, part);
+    HYMD_HIP(hipGetLastError());
+    if (red) {
+        double out[2];
+        HYMD_TRY(finish_reduce(w, nb, 2, out, s));
+        if (energy) *energy = out[0];
+        if (n_clamped) *n_clamped = (int64_t)out[1];
+    }
+    return HYMD_OK;
+}
+
+extern "C" int hymd_dipole_redistribution(void *f_on_bead, int64_t n, const void *f_dipoles,
+                                          const void *transfer_matrix, const int32_t *a, const int32_t *b,
+                                          const int32_t *c, const int32_t *d, const int32_t *dtype,
+                                          const int32_t *last, int64_t m, int32_t precision, void *stream) {
+    HYMD_REQUIRE(f_on_bead && n >= 0 && m >= 0, "bad argument");
+    HYMD_REQUIRE(m == 0 || (a && b && c && d && dtype && last && f_dipoles && transfer_matrix), "null term array");
+    HYMD_REQUIRE(precision == HYMD_F32 || precision == HYMD_F64, "precision must be HYMD_F32 or HYMD_F64");
+    hipStream_t s = (hipStream_t)stream;
+    Workspace &w = workspace();
+    std::lock_guard<std::mutex> lock(w.mu);
+    TermIdx ti{{a, b, c, d}, 4};
+    HYMD_TRY(build_csr(w.csr, ti, m, n, s));
+    if (precision == HYMD_F32)
+        redistribution_kernel<float><<<grid_for(n), kBlock, 0, s>>>((float *)f_on_bead, n, w.csr.off(), w.csr.ent(),
+                                                                    (const float *)f_dipoles,
+                                                                    (const float *)transfer_matrix, dtype, last);
+    else
+        redistribution_kernel<double><<<grid_for(n), kBlock, 0, s>>>((double *)f_on_bead, n, w.csr.off(),
+                                                                     w.csr.ent(), (const double *)f_dipoles,
+                                                                     (const double *)transfer_matrix, dtype, last);
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// persistent topology
+// ---------------------------------------------------------------------------
+struct hymd_topology {
+    int64_t n = 0, m2 = 0, m3 = 0;
+    Csr c2, c3;
+    BondArgs B{};
+    AngleArgs A{};
+    DevBuf partials, result;
+    HostBuf host;
+    int nblocks = 1;
+};
+
+extern "C" int hymd_topology_create(hymd_topology **out, int64_t n, int64_t m2, const int32_t *a2, const int32_t *b2,
+                                    const double *r0, const double *k2, int64_t m3, const int32_t *a3,
+                                    const int32_t *b3, const int32_t *c3, const double *t0, const double *k3,
+                                    void *stream) {
+    HYMD_REQUIRE(out && n >= 0 && m2 >= 0 && m3 >= 0, "bad argument");
+    HYMD_REQUIRE(m2 == 0 || (a2 && b2 && r0 && k2), "null bond array");
+    HYMD_REQUIRE(m3 == 0 || (a3 && b3 && c3 && t0 && k3), "null angle array");
+    hipStream_t s = (hipStream_t)stream;
+    hymd_topology *t = new hymd_topology();
+    t->n = n;
+    t->m2 = m2;
+    t->m3 = m3;
+    t->B = BondArgs{a2, b2, r0, k2};
+    t->A = AngleArgs{a3, b3, c3, t0, k3};
+    int st = HYMD_OK;
+    if (m2 > 0) {
+        TermIdx ti{{a2, b2, nullptr, nullptr}, 2};
+        st = build_csr(t->c2, ti, m2, n, s);
+    }
+    if (st == HYMD_OK && m3 > 0) {
+        TermIdx ti{{a3, b3, c3, nullptr}, 3};
+        st = build_csr(t->c3, ti, m3, n, s);
+    }
+    t->nblocks = grid_for(n, kBlock, 2048);
+    if (st == HYMD_OK) st = t->partials.reserve(sizeof(double) * 8 * t->nblocks);
+    if (st == HYMD_OK) st = t->result.reserve(sizeof(double) * 8);
+    if (st == HYMD_OK) st = t->host.reserve(sizeof(double) * 8);
+    if (st != HYMD_OK) {
+        delete t;
+        return st;
+    }
+    *out = t;
+    return HYMD_OK;
+}
+
+extern "C" int hymd_topology_destroy(hymd_topology *top) {
+    delete top;
+    return HYMD_OK;
+}
+
+extern "C" int hymd_topology_forces(hymd_topology *top, const double *r, float *f, const double box[3],
+                                    double *energies, double *virials, void *stream) {
+    HYMD_REQUIRE(top && f && box && (r || top->n == 0), "bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    const bool red = energies || virials;
+    double *part = red ? (double *)top->partials.ptr : nullptr;
+    if (top->n > 0)
+        topo_gather_kernel<<<top->nblocks, kBlock, 0, s>>>(
+            r, f, top->n, top->m2 ? top->c2.off() : nullptr, top->m2 ? top->c2.ent() : nullptr, top->B,
+            top->m3 ? top->c3.off() : nullptr, top->m3 ? top->c3.ent() : nullptr, top->A, make_box(box), part);
+    HYMD_HIP(hipGetLastError());
+    if (red) {
+        HYMD_TRY(reduce_partials(part, top->nblocks, 8, (double *)top->result.ptr, s));
+        HYMD_HIP(hipMemcpyAsync(top->host.ptr, top->result.ptr, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+        HYMD_HIP(hipStreamSynchronize(s));
+        const double *h = (const double *)top->host.ptr;
+        if (energies) {
+            energies[0] = h[0];
+            energies[1] = h[4];
+        }
+        if (virials)
+            for (int d = 0; d < 3; ++d) {
+                virials[d] = h[1 + d];
+                virials[3 + d] = h[5 + d];
+            }
+    }
+    return HYMD_OK;
+}

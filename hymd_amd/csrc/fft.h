@@ -1,0 +1,177 @@
+// Batched mixed-radix (2,3,4,5,8) FFT for mesh edges 2^a 3^b 5^c, staged in LDS.
+//
+// Every 1-D transform is a self-sorting Stockham sequence: at a stage of radix
+// R after sub-transforms of size p, butterfly i (of N/R) with k = i mod p reads
+// x[i + j N/R], multiplies by w^(jk) (w = exp(-+2 pi i/(pR))), applies the
+// R-point DFT and writes y[(i/p) p R + k + j p].  A workgroup holds W lines
+// in two LDS buffers and ping-pongs between them, one barrier per stage.
+//
+// 3-D real transforms (unnormalised, numpy sign convention):
+//   r2c: z rows -> packed complex FFT of length Mz/2 + split (post twiddles),
+//        then in-place complex passes along y and x;
+//   c2r: the same in reverse.
+#pragma once
+#include "common.h"
+
+namespace hymd {
+
+constexpr int kMaxStages = 12;
+
+struct FftAxis {
+    int n;        // complex line length
+    int nstages;
+    int ld;       // LDS row stride (complex elements)
+    int radix[kMaxStages];
+    int twoff[kMaxStages];  // offset of the stage's twiddles in the table
+};
+
+struct FftPlan {
+    Mesh mesh;
+    FftAxis ax, ay, azh;         // x, y full axes; z half-length packed axis
+    int wx, wy, wz;              // lines per workgroup
+    DevBuf tw_x, tw_y, tw_zh;    // forward stage twiddles (n-1 entries)
+    DevBuf post;                 // exp(-2 pi i k / Mz), k = 0..Mz/2
+    size_t lds_x, lds_y, lds_z;  // dynamic LDS bytes per kernel
+};
+
+int fft_plan_init(FftPlan &plan, const Mesh &mesh);
+
+// r2c of `batch` int64 fixed-point grids: value = q * scale; clears the input
+// rows after reading when `clear` (so the paint grid is zero for the next call).
+int fft_r2c_fixed(const FftPlan &plan, int64_t *in, float2 *out, int batch, double scale, bool clear,
+                  hipStream_t s);
+int fft_r2c_real(const FftPlan &plan, const float *in, float2 *out, int batch, hipStream_t s);
+// forward complex passes only (y then x) / inverse passes (x then y), in place
+int fft_forward_xy(const FftPlan &plan, float2 *data, int batch, hipStream_t s);
+int fft_inverse_xy(const FftPlan &plan, float2 *data, int batch, hipStream_t s);
+// z c2r of `batch` half spectra (after fft_inverse_xy) into real grids, times `scale`
+int fft_c2r_z(const FftPlan &plan, const float2 *in, float *out, int batch, float scale, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// device building blocks
+// ---------------------------------------------------------------------------
+template <int R>
+__device__ __forceinline__ void small_dft(float2 (&u)[8], float sgn);
+
+template <>
+__device__ __forceinline__ void small_dft<2>(float2 (&u)[8], float) {
+    float2 a = u[0], b = u[1];
+    u[0] = cadd(a, b);
+    u[1] = csub(a, b);
+}
+
+template <>
+__device__ __forceinline__ void small_dft<3>(float2 (&u)[8], float sgn) {
+    const float h = 0.86602540378443864676f;  // sqrt(3)/2
+    float2 t = cadd(u[1], u[2]);
+    float2 m = make_float2(u[0].x - 0.5f * t.x, u[0].y - 0.5f * t.y);
+    float2 s = cmul_i(csub(u[1], u[2]), sgn * h);
+    u[0] = cadd(u[0], t);
+    u[1] = cadd(m, s);
+    u[2] = csub(m, s);
+}
+
+template <>
+__device__ __forceinline__ void small_dft<4>(float2 (&u)[8], float sgn) {
+    float2 t0 = cadd(u[0], u[2]), t1 = csub(u[0], u[2]);
+    float2 t2 = cadd(u[1], u[3]), t3 = cmul_i(csub(u[1], u[3]), sgn);
+    u[0] = cadd(t0, t2);
+    u[1] = cadd(t1, t3);
+    u[2] = csub(t0, t2);
+    u[3] = csub(t1, t3);
+}
+
+template <>
+__device__ __forceinline__ void small_dft<5>(float2 (&u)[8], float sgn) {
+    const float c1 = 0.30901699437494742410f;   // cos(2pi/5)
+    const float c2 = -0.80901699437494742410f;  // cos(4pi/5)
+    const float s1 = 0.95105651629515357212f;   // sin(2pi/5)
+    const float s2 = 0.58778525229247312917f;   // sin(4pi/5)
+    float2 a1 = cadd(u[1], u[4]), b1 = csub(u[1], u[4]);
+    float2 a2 = cadd(u[2], u[3]), b2 = csub(u[2], u[3]);
+    float2 m1 = make_float2(u[0].x + c1 * a1.x + c2 * a2.x, u[0].y + c1 * a1.y + c2 * a2.y);
+    float2 m2 = make_float2(u[0].x + c2 * a1.x + c1 * a2.x, u[0].y + c2 * a1.y + c1 * a2.y);
+    float2 n1 = cmul_i(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y), sgn);
+    float2 n2 = cmul_i(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y), sgn);
+    u[0] = cadd(u[0], cadd(a1, a2));
+    u[1] = cadd(m1, n1);
+    u[4] = csub(m1, n1);
+    u[2] = cadd(m2, n2);
+    u[3] = csub(m2, n2);
+}
+
+template <>
+__device__ __forceinline__ void small_dft<8>(float2 (&u)[8], float sgn) {
+    const float r = 0.70710678118654752440f;
+    float2 e[8] = {u[0], u[2], u[4], u[6], u[0], u[0], u[0], u[0]};
+    float2 o[8] = {u[1], u[3], u[5], u[7], u[0], u[0], u[0], u[0]};
+    small_dft<4>(e, sgn);
+    small_dft<4>(o, sgn);
+    // w^j o_j, w = exp(sgn 2 pi i / 8)
+    float2 w1 = cscale(cadd(o[1], cmul_i(o[1], sgn)), r);
+    float2 w2 = cmul_i(o[2], sgn);
+    float2 w3 = cscale(csub(cmul_i(o[3], sgn), o[3]), r);
+    u[0] = cadd(e[0], o[0]);
+    u[4] = csub(e[0], o[0]);
+    u[1] = cadd(e[1], w1);
+    u[5] = csub(e[1], w1);
+    u[2] = cadd(e[2], w2);
+    u[6] = csub(e[2], w2);
+    u[3] = cadd(e[3], w3);
+    u[7] = csub(e[3], w3);
+}
+
+template <int R>
+__device__ __forceinline__ void stockham_stage(const float2 *__restrict__ src, float2 *__restrict__ dst, int W,
+                                               int ld, int N, int p, const float2 *__restrict__ tw, float sgn) {
+    const int NR = N / R;
+    const int total = W * NR;
+    for (int b = threadIdx.x; b < total; b += blockDim.x) {
+        const int l = b / NR;
+        const int i = b - l * NR;
+        const int k = i % p;
+        const float2 *s = src + l * ld;
+        float2 u[8];
+#pragma unroll
+        for (int j = 0; j < R; ++j) u[j] = s[i + j * NR];
+        if (p > 1) {
+#pragma unroll
+            for (int j = 1; j < R; ++j) {
+                float2 w = tw[k * (R - 1) + j - 1];
+                if (sgn > 0.f) w.y = -w.y;
+                u[j] = cmul(u[j], w);
+            }
+        }
+        small_dft<R>(u, sgn);
+        float2 *d = dst + l * ld;
+        const int ob = (i / p) * p * R + k;
+#pragma unroll
+        for (int j = 0; j < R; ++j) d[ob + j * p] = u[j];
+    }
+}
+
+// Transforms W lines held in `a` (row stride ax.ld); returns the buffer that
+// holds the result (a or b).  Caller must barrier after filling `a`.
+__device__ __forceinline__ float2 *fft_lds(float2 *a, float2 *b, int W, const FftAxis &ax,
+                                           const float2 *__restrict__ tw, float sgn) {
+    int p = 1;
+    for (int s = 0; s < ax.nstages; ++s) {
+        const int R = ax.radix[s];
+        const float2 *tws = tw + ax.twoff[s];
+        switch (R) {
+            case 2: stockham_stage<2>(a, b, W, ax.ld, ax.n, p, tws, sgn); break;
+            case 3: stockham_stage<3>(a, b, W, ax.ld, ax.n, p, tws, sgn); break;
+            case 4: stockham_stage<4>(a, b, W, ax.ld, ax.n, p, tws, sgn); break;
+            case 5: stockham_stage<5>(a, b, W, ax.ld, ax.n, p, tws, sgn); break;
+            default: stockham_stage<8>(a, b, W, ax.ld, ax.n, p, tws, sgn); break;
+        }
+        __syncthreads();
+        float2 *t = a;
+        a = b;
+        b = t;
+        p *= R;
+    }
+    return a;
+}
+
+}  // namespace hymd

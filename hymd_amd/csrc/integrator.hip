@@ -1,0 +1,104 @@
+// rRESPA velocity-Verlet pieces in FP64 (integrator.py:9-73 composed as in
+// main.py:802-887,1143-1169).  Forces arrive in float32; positions and
+// velocities stay float64; a float32 copy of the positions feeds the paint
+// and readout kernels.
+#include <cmath>
+
+#include "common.h"
+
+namespace hymd {
+
+// np.mod(x, L) for L > 0: fmod, then shift negatives up by L (can return L
+// itself for tiny negative x, as numpy does).
+__device__ __forceinline__ double np_mod(double x, double L) {
+    double r = fmod(x, L);
+    if (r != 0.0 && r < 0.0) r += L;
+    return r;
+}
+
+__global__ void __launch_bounds__(kBlock) kick_drift_kernel(double *__restrict__ x, double *__restrict__ v,
+                                                             float *__restrict__ x32, int64_t n,
+                                                             const float *__restrict__ ff, double cf,
+                                                             const float *__restrict__ fs, double cs, double dt,
+                                                             double b0, double b1, double b2) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n3 = 3 * n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += stride) {
+        const int d = (int)(i % 3);
+        const double L = d == 0 ? b0 : (d == 1 ? b1 : b2);
+        double vi = v[i];
+        if (fs) vi = vi + cs * (double)fs[i];
+        vi = vi + cf * (double)ff[i];
+        const double xi = np_mod(x[i] + dt * vi, L);
+        v[i] = vi;
+        x[i] = xi;
+        if (x32) x32[i] = (float)xi;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) kick_kernel(double *__restrict__ v, int64_t n, const float *__restrict__ f1,
+                                                       double c1, const float *__restrict__ f2, double c2) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n3 = 3 * n;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += stride) {
+        double vi = v[i] + c1 * (double)f1[i];
+        if (f2) vi = vi + c2 * (double)f2[i];
+        v[i] = vi;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) ke_kernel(const T *__restrict__ v, int64_t n3,
+                                                     double *__restrict__ partials) {
+    double acc[1] = {0.0};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += stride)
+        acc[0] += (double)v[i] * (double)v[i];
+    block_reduce_store<1>(acc, partials + blockIdx.x);
+}
+
+}  // namespace hymd
+
+using namespace hymd;
+
+extern "C" int hymd_respa_kick_drift(double *x, double *v, float *x32, int64_t n, const float *f_fast, double c_fast,
+                                     const float *f_slow, double c_slow, double dt, const double box[3],
+                                     void *stream) {
+    HYMD_REQUIRE(n >= 0 && box && (n == 0 || (x && v && f_fast)), "bad argument");
+    for (int d = 0; d < 3; ++d) HYMD_REQUIRE(box[d] > 0.0, "box edges must be positive");
+    if (n > 0)
+        kick_drift_kernel<<<grid_for(3 * n), kBlock, 0, (hipStream_t)stream>>>(x, v, x32, n, f_fast, c_fast, f_slow,
+                                                                              c_slow, dt, box[0], box[1], box[2]);
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
+}
+
+extern "C" int hymd_respa_kick(double *v, int64_t n, const float *f1, double c1, const float *f2, double c2,
+                               void *stream) {
+    HYMD_REQUIRE(n >= 0 && (n == 0 || (v && f1)), "bad argument");
+    if (n > 0) kick_kernel<<<grid_for(3 * n), kBlock, 0, (hipStream_t)stream>>>(v, n, f1, c1, f2, c2);
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
+}
+
+extern "C" int hymd_kinetic_energy(const void *v, int64_t n, int32_t precision, double mass, double *out,
+                                   void *stream) {
+    HYMD_REQUIRE(out && n >= 0 && (v || n == 0), "bad argument");
+    HYMD_REQUIRE(precision == HYMD_F32 || precision == HYMD_F64, "precision must be HYMD_F32 or HYMD_F64");
+    hipStream_t s = (hipStream_t)stream;
+    const int nb = grid_for(3 * n, kBlock, 1024);
+    DevBuf part, res;
+    HYMD_TRY(part.reserve(sizeof(double) * nb));
+    HYMD_TRY(res.reserve(sizeof(double)));
+    if (precision == HYMD_F32)
+        ke_kernel<float><<<nb, kBlock, 0, s>>>((const float *)v, 3 * n, (double *)part.ptr);
+    else
+        ke_kernel<double><<<nb, kBlock, 0, s>>>((const double *)v, 3 * n, (double *)part.ptr);
+    HYMD_HIP(hipGetLastError());
+    HYMD_TRY(reduce_partials((const double *)part.ptr, nb, 1, (double *)res.ptr, s));
+    double sum = 0.0;
+    HYMD_HIP(hipMemcpyAsync(&sum, res.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
+    HYMD_HIP(hipStreamSynchronize(s));
+    *out = 0.5 * mass * sum;
+    return HYMD_OK;
+}

@@ -1,0 +1,111 @@
+"""rRESPA molecular dynamics on one GPU (main.py:764-1169 without IO/thermostat).
+
+Per outer step, in the reference's order (main.py:802-887, 975-1004,
+1143-1169):
+
+    v += 1/2 (n dt) F_slow/m                         outer half kick
+    repeat n times:  v += 1/2 dt F_fast/m ; x += dt v ; x = mod(x, L)
+                     F_fast = bonds(x) + angles(x)
+                     v += 1/2 dt F_fast/m
+    F_slow = readout(update_field(x))                slow (particle-field) forces
+    v += 1/2 (n dt) F_slow/m                         outer half kick
+
+Adjacent kicks share one kernel with the drift (each half kick is still its
+own FP64 addition, in the reference's order).  Positions and velocities are
+float64; forces float32; a float32 copy of x feeds paint and readout.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, darr, dptr
+from .field import ParticleMesh
+from .force import BondedTopology
+from .synth import hamiltonian_form
+
+
+class RespaMD:
+    def __init__(self, system, stream=None):
+        _lib.require_gpu()
+        s = system
+        self.system = s
+        self.n = int(s.n)
+        self.T = int(s.n_types)
+        self.box = np.asarray(s.box, dtype=np.float64)
+        self.mass = float(s.mass)
+        self.dt = float(s.time_step)
+        self.n_inner = int(s.respa_inner)
+        self.sigma = float(s.sigma)
+        self.A, self.b, self.c0 = hamiltonian_form(s)
+        self.type_mass = [1.0] * self.T
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.x = torch.as_tensor(np.ascontiguousarray(s.positions, dtype=np.float64), device=dev)
+        self.x = torch.remainder(self.x, torch.as_tensor(self.box, device=dev)).contiguous()
+        self.v = torch.as_tensor(np.ascontiguousarray(s.velocities, dtype=np.float64), device=dev)
+        self.x32 = self.x.to(torch.float32)
+        self.types = torch.as_tensor(np.ascontiguousarray(s.types, dtype=np.int32), device=dev)
+        self.pm = ParticleMesh(s.mesh, self.box, self.T, stream=stream)
+        self.topo = BondedTopology(self.n, bonds=s.bonds, angles=s.angles)
+        self.force_mesh = self.pm.create("real", batch=(self.T, 3))
+        self.f_field = torch.zeros((self.n, 3), dtype=torch.float32, device=dev)
+        self.f_fast = torch.zeros((self.n, 3), dtype=torch.float32, device=dev)
+        self._box_c = darr(self.box, 3)
+        self.energies = {}
+        # initial forces (main.py:331-371, 508-551)
+        self.slow_forces()
+        self.fast_forces()
+
+    # ------------------------------------------------------------------
+    def slow_forces(self, energy=False):
+        e = self.pm.field_update(self.x32, self.types, self.T, self.A, self.b, self.c0, self.type_mass,
+                                 self.sigma, self.force_mesh, want_energy=energy)
+        self.pm.readout_forces(self.x32, self.types, self.T, self.force_mesh, self.f_field)
+        if energy:
+            self.energies["field"] = e
+
+    def fast_forces(self, energy=False):
+        r = self.topo.forces(self.x, self.f_fast, self.box, energies=energy)
+        if energy:
+            (eb, ea), (pb, pa) = r
+            self.energies.update(bond=eb, angle=ea, bond_pr=pb, angle_pr=pa)
+
+    def kick_drift(self, f_fast, c_fast, f_slow=None, c_slow=0.0):
+        call("hymd_respa_kick_drift", dptr(self.x), dptr(self.v), dptr(self.x32), self.n, dptr(f_fast),
+             float(c_fast), dptr(f_slow), float(c_slow), self.dt, self._box_c, _lib.stream_ptr())
+
+    def kick(self, f1, c1, f2=None, c2=0.0):
+        call("hymd_respa_kick", dptr(self.v), self.n, dptr(f1), float(c1), dptr(f2), float(c2), _lib.stream_ptr())
+
+    def kinetic_energy(self):
+        out = ctypes.c_double(0.0)
+        call("hymd_kinetic_energy", dptr(self.v), self.n, _lib.HYMD_F64, self.mass, ctypes.byref(out),
+             _lib.stream_ptr())
+        return out.value
+
+    # ------------------------------------------------------------------
+    def step(self, energies=False, timer=None):
+        """One outer rRESPA step.  ``timer`` (optional) is called as
+        timer(name, fn) around the bonded-force evaluations."""
+        half_in = 0.5 * self.dt / self.mass
+        half_out = 0.5 * self.n_inner * self.dt / self.mass
+        for i in range(self.n_inner):
+            if i == 0:
+                self.kick_drift(self.f_fast, half_in, self.f_field, half_out)
+            else:
+                self.kick_drift(self.f_fast, half_in, self.f_fast, half_in)
+            last = energies and i == self.n_inner - 1
+            if timer is None:
+                self.fast_forces(energy=last)
+            else:
+                timer("bonded", lambda: self.fast_forces(energy=last))
+        self.slow_forces(energy=energies)
+        self.kick(self.f_fast, half_in, self.f_field, half_out)
+        if energies:
+            self.energies["kinetic"] = self.kinetic_energy()
+        return self.energies if energies else None
+
+    def close(self):
+        self.topo.close()
+        self.pm.close()

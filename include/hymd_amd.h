@@ -1,0 +1,175 @@
+/*
+ * hymd_amd.h -- C-ABI of the MI355X-native HyMD particle-field hot path.
+ *
+ * Plain pointers and sizes only.  "device" pointers are HIP device memory
+ * owned by the caller (e.g. torch tensors' data_ptr()); "host" pointers are
+ * ordinary host memory.  Every entry point returns HYMD_OK (0) or a negative
+ * status; hymd_last_error() returns a message for the calling thread.
+ * Work is enqueued on the stream given at creation (or per call) and is
+ * asynchronous unless an energy/virial is requested into host memory, in
+ * which case the call synchronises that stream before returning.  Nothing
+ * prints or aborts.  Calls on distinct handles may run from distinct threads;
+ * calls on one handle must not overlap.
+ *
+ * Reference interfaces replaced (Cascella-Group-UiO/HyMD v2.2.0):
+ *   field.py   initialize_pm :10-149, compute_field_force :152-200,
+ *              compute_self_energy_q :203-238, update_field_force_q :241-403,
+ *              update_field :428-616, compute_field_and_kinetic_energy :619-703
+ *   force.py   f2py module force_kernels cbf/caf/cdf(+_d) :8-23
+ *              (compute_bond_forces*.f90, compute_angle_forces*.f90,
+ *              compute_dihedral_forces*.f90), dipole_forces_redistribution :855-877
+ *   integrator.py integrate_velocity :9-41, integrate_position :44-73 as
+ *              composed by the rRESPA loop main.py:802-887,1143-1169
+ */
+#ifndef HYMD_AMD_H
+#define HYMD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HYMD_OK 0
+#define HYMD_ERR_INVALID (-1)     /* bad argument or shape */
+#define HYMD_ERR_HIP (-2)         /* a HIP runtime call failed */
+#define HYMD_ERR_UNSUPPORTED (-3) /* e.g. a mesh edge that is not 2^a 3^b 5^c */
+#define HYMD_ERR_NOMEM (-4)       /* device allocation failed */
+
+#define HYMD_MAX_TYPES 16
+/* fraction bits of the deterministic int64 fixed-point density grid */
+#define HYMD_FIXED_SHIFT 32
+
+/* precision selector of the bonded kernels (f2py cbf vs cbf_d) */
+#define HYMD_F32 0
+#define HYMD_F64 1
+
+const char *hymd_last_error(void);
+int hymd_abi_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Particle-mesh handle: mesh, box, FFT plans, internal scratch.            */
+/* Replaces pmesh.ParticleMesh + the grids of initialize_pm (field.py:45-86). */
+/* Mesh edges: even, of the form 2^a 3^b 5^c, 4 <= M <= 1024.               */
+/* ------------------------------------------------------------------------ */
+typedef struct hymd_pm hymd_pm;
+
+int hymd_pm_create(hymd_pm **pm, const int32_t mesh[3], const double box[3], int32_t max_types,
+                   void *stream);
+int hymd_pm_destroy(hymd_pm *pm);
+int hymd_pm_set_stream(hymd_pm *pm, void *stream);
+/* new box at fixed mesh (barostats re-create pmesh objects, barostat.py:163-164) */
+int hymd_pm_set_box(hymd_pm *pm, const double box[3]);
+
+/* update_field (field.py:428-616) for a quadratic Hamiltonian
+ *   w(phi) = 1/2 phi^T A phi + b.phi + c0   (hamiltonian.py:145,259,393-412)
+ * x          device float32 (n,3) row-major positions (any real value; wrapped periodically)
+ * types      device int32 (n) in [0, n_types); other values are skipped
+ * coupling   host double (n_types, n_types) A;  shift  host double (n_types) b
+ * mass       host double (n_types) paint weight per type (config.m, field.py:574)
+ * force_mesh device float32 (n_types,3,Mx,My,Mz) out: F_{t,d} = c2r(-i k_d H r2c(v_t))
+ * phi        device float32 (n_types,Mx,My,Mz) out or NULL: filtered densities (field.py:578)
+ * phi_fourier device complex64 (n_types,Mx,My,Mz/2+1) out or NULL: H*r2c(phi) (pmesh norm.)
+ * v_ext      device float32 (n_types,Mx,My,Mz) out or NULL: potentials (field.py:615-616)
+ * energy     host double out or NULL: E_field = dV sum w(phi~) (field.py:692-693)
+ */
+int hymd_update_field(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
+                      const double *coupling, const double *shift, double constant, const double *mass,
+                      double sigma, float *force_mesh, float *phi, void *phi_fourier, float *v_ext,
+                      double *energy);
+
+/* compute_field_force (field.py:152-200): force[i] = CIC readout of force_mesh[types[i]] */
+int hymd_compute_field_force(hymd_pm *pm, const float *x, const int32_t *types, int64_t n,
+                             int32_t n_types, const float *force_mesh, float *force);
+
+/* update_field_force_q (field.py:241-403): PME/Poisson forces on charges
+ * charges    device float32 (n); elec_factor = coulomb_constant / dielectric_const
+ * elec_forces device float32 (n,3) out
+ * phi_q, psi device float32 (Mx,My,Mz) out or NULL; elec_field float32 (3,Mx,My,Mz) out or NULL
+ * energy_q   host double out or NULL: dV sum(1/2 phi_q psi) - self_energy (field.py:697-699)
+ */
+int hymd_update_field_force_q(hymd_pm *pm, const float *x, const float *charges, int64_t n, double sigma,
+                              double elec_factor, double self_energy, float *elec_forces, float *phi_q,
+                              float *psi, float *elec_field, double *energy_q);
+
+/* compute_field_and_kinetic_energy (field.py:688-703), real-space sums in FP64:
+ * field energy  dV sum_cells w(phi~) from filtered densities phi (n_types,Mx,My,Mz);
+ * charge energy dV sum_cells (1/2 phi_q psi - self_energy/V). host double out. */
+int hymd_field_energy(hymd_pm *pm, const float *phi, int32_t n_types, const double *coupling, const double *shift,
+                      double constant, double *energy);
+int hymd_field_q_energy(hymd_pm *pm, const float *phi_q, const float *psi, double self_energy, double *energy);
+
+/* compute_self_energy_q (field.py:203-238): host double out */
+int hymd_self_energy_q(const float *charges, int64_t n, double sigma, double elec_factor, double *out,
+                       void *stream);
+
+/* Building blocks (used by the above; exported for parity tests and callers
+ * that assemble their own pipelines). */
+/* CIC paint into int64 fixed-point grids (value = round(w*m*2^32)); grid is
+ * (n_types or 1, Mx,My,Mz); if types==NULL all particles go to grid 0 with weight
+ * weights[i] (device float32) or mass[0]. accumulate=0 zeroes the grid first. */
+int hymd_paint_fixed(hymd_pm *pm, const float *x, const int32_t *types, const float *weights, int64_t n,
+                     int32_t n_types, const double *mass, int64_t *grid, int32_t accumulate);
+/* unnormalised batched 3-D r2c / c2r on (batch,Mx,My,Mz) real <-> (batch,Mx,My,Mz/2+1) complex64 */
+int hymd_fft_r2c(hymd_pm *pm, const float *in, void *out, int32_t batch);
+int hymd_fft_c2r(hymd_pm *pm, const void *in, float *out, int32_t batch);
+/* CIC readout of one real grid at n positions */
+int hymd_readout(hymd_pm *pm, const float *grid, const float *x, int64_t n, float *out);
+
+/* ------------------------------------------------------------------------ */
+/* Bonded kernels: the f2py calls of force.py:8-23 (f2py zeroes nothing but  */
+/* the Fortran zeroes its outputs on entry, as these do).  precision selects */
+/* cbf (HYMD_F32: f,r float32) or cbf_d (HYMD_F64: f,r float64).  r, f are   */
+/* device (n,3) row-major; index/parameter arrays are device arrays.          */
+/* energy / pr are host doubles (may be NULL: skips the reduction and sync).  */
+/* ------------------------------------------------------------------------ */
+int hymd_cbf(void *f, const void *r, int64_t n, const double box[3], const int32_t *a, const int32_t *b,
+             const double *r0, const double *k, int64_t m, int32_t precision, double *energy, double *bond_pr,
+             void *stream);
+int hymd_caf(void *f, const void *r, int64_t n, const double box[3], const int32_t *a, const int32_t *b,
+             const int32_t *c, const double *t0, const double *k, int64_t m, int32_t precision, double *energy,
+             double *angle_pr, void *stream);
+/* coeff device float32 (m,6,5); dipoles device (m,4,3), transfer_matrix device
+ * (m,6,3,3) in the precision of r (zeroed, then filled where dtype==1 and
+ * dipole_flag!=0); n_clamped host int64 out or NULL: count of sin(gamma) clamps */
+int hymd_cdf(void *f, const void *r, void *dipoles, void *transfer_matrix, int64_t n, const double box[3],
+             const int32_t *a, const int32_t *b, const int32_t *c, const int32_t *d, const float *coeff,
+             const int32_t *dtype, const int32_t *bb_index, int64_t m, int32_t dipole_flag, int32_t precision,
+             double *energy, int64_t *n_clamped, void *stream);
+/* dipole_forces_redistribution (force.py:855-877): f_on_bead (n,3) out */
+int hymd_dipole_redistribution(void *f_on_bead, int64_t n, const void *f_dipoles, const void *transfer_matrix,
+                               const int32_t *a, const int32_t *b, const int32_t *c, const int32_t *d,
+                               const int32_t *dtype, const int32_t *last, int64_t m, int32_t precision,
+                               void *stream);
+
+/* Persistent topology: bonds + angles with the per-particle term lists built
+ * once (the rRESPA loop calls the kernels respa_inner times per step on an
+ * unchanged topology, main.py:830-881).  The term arrays are referenced, not
+ * copied: they must stay allocated and unchanged while the handle is used. */
+typedef struct hymd_topology hymd_topology;
+int hymd_topology_create(hymd_topology **top, int64_t n, int64_t m2, const int32_t *a2, const int32_t *b2,
+                         const double *r0, const double *k2, int64_t m3, const int32_t *a3, const int32_t *b3,
+                         const int32_t *c3, const double *t0, const double *k3, void *stream);
+int hymd_topology_destroy(hymd_topology *top);
+/* total bonded force (bonds + angles) from float64 positions into float32 f;
+ * energies (host double[2]: bond, angle) and virials (host double[6]) or NULL */
+int hymd_topology_forces(hymd_topology *top, const double *r, float *f, const double box[3], double *energies,
+                         double *virials, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* rRESPA integrator (FP64 x, v): main.py:802-887,1143-1169.                 */
+/* ------------------------------------------------------------------------ */
+/* v += c_fast*f_fast + c_slow*f_slow (f_slow may be NULL); x += dt*v;
+ * x = mod(x, box) (np.mod semantics, main.py:837); x32 = float(x) if non-NULL */
+int hymd_respa_kick_drift(double *x, double *v, float *x32, int64_t n, const float *f_fast, double c_fast,
+                          const float *f_slow, double c_slow, double dt, const double box[3], void *stream);
+/* v += c1*f1 + c2*f2 (f2 may be NULL) */
+int hymd_respa_kick(double *v, int64_t n, const float *f1, double c1, const float *f2, double c2, void *stream);
+/* 1/2 m sum v^2 (field.py:695) into host double; v (n,3) float32 or float64 per precision */
+int hymd_kinetic_energy(const void *v, int64_t n, int32_t precision, double mass, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HYMD_AMD_H */

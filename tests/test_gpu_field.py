@@ -1,0 +1,214 @@
+"""HIP particle-field path against the FP64 oracle (and closed forms).
+
+Tolerances: the int64 fixed-point density grid is compared bit for bit; FP32
+FFT / forces against the FP64 oracle within the north-star bound of 1e-4
+relative RMS (tighter where the data allow, stated per test).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import field as F
+from tests.conftest import rel_err, rms_rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _pm(mesh, box, T):
+    from hymd_amd.field import ParticleMesh
+    return ParticleMesh(mesh, box, T)
+
+
+def _dev(a, dtype):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(dtype).cuda().contiguous()
+
+
+@pytest.mark.parametrize("mesh,box", [((32, 48, 40), (7.0, 9.5, 8.25)), ((16, 16, 16), (4.0, 4.0, 4.0))])
+def test_paint_fixed_bit_exact(hip, mesh, box):
+    rng = np.random.default_rng(11)
+    n, T = 20000, 3
+    x = rng.uniform(-0.5, 1.5, (n, 3)) * np.array(box)           # includes out-of-box positions
+    x[:5] = np.array(box)                                         # exactly on the upper face
+    x32 = x.astype(np.float32)
+    types = rng.integers(0, T, n).astype(np.int32)
+    mass = [1.0, 0.75, 2.5]
+    pm = _pm(mesh, box, T)
+    grid = pm.paint_fixed(_dev(x32, torch.float32), _dev(types, torch.int32), T, mass=mass).cpu().numpy()
+    for t in range(T):
+        sel = types == t
+        ref = F.paint_fixed(x32[sel].astype(np.float64), mesh, box, mass=mass[t])
+        np.testing.assert_array_equal(grid[t], ref)
+        assert int(grid[t].sum()) == int(np.rint(mass[t] * 2 ** 32)) * int(sel.sum())
+        np.testing.assert_allclose(grid[t] / 2.0 ** 32, F.paint(x32[sel], mesh, box, mass=mass[t]),
+                                   atol=8 * 2.0 ** -33 * mass[t] * 8)
+
+
+def test_paint_empty(hip):
+    pm = _pm((8, 8, 8), (2.0, 2.0, 2.0), 1)
+    x = torch.zeros((0, 3), dtype=torch.float32, device="cuda")
+    t = torch.zeros(0, dtype=torch.int32, device="cuda")
+    assert int(pm.paint_fixed(x, t, 1).abs().sum()) == 0
+
+
+@pytest.mark.parametrize("mesh", [(8, 8, 8), (32, 32, 32), (48, 64, 96), (128, 128, 128), (160, 160, 160),
+                                  (20, 30, 36)])
+def test_fft_vs_numpy(hip, mesh):
+    rng = np.random.default_rng(sum(mesh))
+    pm = _pm(mesh, (1.0, 1.0, 1.0), 1)
+    B = 2
+    x = rng.normal(size=(B,) + mesh).astype(np.float32)
+    X = pm.r2c(_dev(x, torch.float32)).cpu().numpy()
+    ref = np.fft.rfftn(x.astype(np.float64), axes=(1, 2, 3))
+    # FP32 transform error grows ~ sqrt(log G); bound relative to the spectrum's RMS
+    assert np.sqrt(np.mean(np.abs(X - ref) ** 2)) / np.sqrt(np.mean(np.abs(ref) ** 2)) < 2e-6
+    y = pm.c2r(_dev(ref.astype(np.complex64), torch.complex64)).cpu().numpy()
+    G = np.prod(mesh)
+    np.testing.assert_allclose(y / G, x, atol=2e-5 * np.abs(x).max())
+
+
+def _diblock(n_chains=600, box=8.0):
+    from hymd_amd import synth
+    return synth.diblock_melt(n_chains=n_chains, box=box, mesh=32, seed=1000)
+
+
+def test_update_field_matches_oracle(hip):
+    from hymd_amd.synth import hamiltonian_form
+    s = _diblock()
+    A, b, c0 = hamiltonian_form(s)
+    ham = F.Quadratic(A, b, c0)
+    x32 = s.positions.astype(np.float32)
+    ref = F.update_field(x32.astype(np.float64), s.types, 2, s.mesh, s.box, s.sigma, ham, compute_potential=True)
+    f_ref = F.compute_field_force(x32.astype(np.float64), s.types, ref["force_mesh"], s.mesh, s.box)
+    e_ref = F.field_energy(ref["phi"], ham, s.mesh, s.box)
+
+    pm = _pm(s.mesh, s.box, 2)
+    xd, td = _dev(x32, torch.float32), _dev(s.types, torch.int32)
+    fm = pm.create("real", batch=(2, 3))
+    phi = pm.create("real", batch=2)
+    phif = pm.create("complex", batch=2)
+    vext = pm.create("real", batch=2)
+    e = pm.field_update(xd, td, 2, A, b, c0, [1.0, 1.0], s.sigma, fm, phi=phi, phi_fourier=phif, v_ext=vext,
+                        want_energy=True)
+    f = torch.empty((s.n, 3), dtype=torch.float32, device="cuda")
+    pm.readout_forces(xd, td, 2, fm, f)
+    f = f.cpu().numpy()
+    fmg = fm.cpu().numpy()
+    for t in range(2):
+        for d in range(3):
+            assert rms_rel(fmg[t, d], ref["force_mesh"][t][d]) < 1e-5
+        assert rms_rel(phi[t].cpu().numpy(), ref["phi"][t]) < 1e-6
+        assert rms_rel(phif[t].cpu().numpy(), ref["phi_fourier"][t]) < 1e-6
+        assert rms_rel(vext[t].cpu().numpy(), ref["v_ext"][t]) < 1e-6
+    assert rms_rel(f, f_ref) < 1e-5
+    assert rel_err(f, f_ref) < 1e-4
+    assert e == pytest.approx(e_ref, rel=1e-6)
+    # real-space energy reduction of the stored filtered densities
+    assert pm.field_energy(phi, 2, A, b, c0) == pytest.approx(e_ref, rel=1e-6)
+
+
+def test_forces_on_nodes_match_pair_forces(hip):
+    """Closed form (continuum Gaussian-core pair forces) on the GPU path."""
+    M, L, sigma = 64, 10.0, 0.5
+    rng = np.random.default_rng(3)
+    idx = rng.choice(M ** 3, 12, replace=False)
+    x = np.stack(np.unravel_index(idx, (M, M, M)), axis=1) * (L / M)
+    types = rng.integers(0, 2, 12).astype(np.int32)
+    rho0 = 16.0
+    ham = F.default_with_chi(2, 0.05, rho0, rho0, np.array([[0.0, 20.0], [20.0, 0.0]]))
+    _, f_ref = F.gaussian_core_pairs(x, types, ham.A, sigma, np.full(3, L))
+    pm = _pm((M,) * 3, (L,) * 3, 2)
+    xd, td = _dev(x.astype(np.float32), torch.float32), _dev(types, torch.int32)
+    fm = pm.create("real", batch=(2, 3))
+    pm.field_update(xd, td, 2, ham.A, ham.b, ham.c0, [1.0, 1.0], sigma, fm)
+    f = torch.empty((12, 3), dtype=torch.float32, device="cuda")
+    pm.readout_forces(xd, td, 2, fm, f)
+    np.testing.assert_allclose(f.cpu().numpy(), f_ref, atol=2e-5 * np.abs(f_ref).max())
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_gaussian_core_energy_160(hip, goldens, case):
+    """160^3 mesh (radix 5), closed-form energy of test/test_hamiltonian.py:172-227."""
+    g = goldens["gaussian_core_no_chi"]
+    kappa, sigma = g["params"][case]
+    r = np.array(g["positions"])
+    box = np.array(g["box"])
+    N = len(r)
+    rho0 = N / np.prod(box)
+    ham = F.default_no_chi(1, kappa, rho0, rho0)
+    c = 16.0 * np.pi ** 1.5 * kappa * sigma ** 3 * rho0
+    E = N / c - 0.5 * N / kappa + sum(2 * np.exp(-np.sum((r[i] - r[j]) ** 2) / (4 * sigma ** 2)) / c
+                                      for i in range(N) for j in range(i + 1, N))
+    pm = _pm((160,) * 3, box, 1)
+    xd = _dev(r.astype(np.float32), torch.float32)
+    td = torch.zeros(N, dtype=torch.int32, device="cuda")
+    fm = pm.create("real", batch=(1, 3))
+    phi = pm.create("real", batch=1)
+    e = pm.field_update(xd, td, 1, ham.A, ham.b, ham.c0, [1.0], sigma, fm, phi=phi, want_energy=True)
+    assert e == pytest.approx(E, rel=2e-6)
+    assert pm.field_energy(phi, 1, ham.A, ham.b, ham.c0) == pytest.approx(E, rel=2e-6)
+
+
+def test_pme_matches_oracle(hip):
+    rng = np.random.default_rng(7)
+    n, M, L = 4000, 40, 10.0
+    x = rng.uniform(0, L, (n, 3)).astype(np.float32)
+    q = np.zeros(n, dtype=np.float32)
+    q[: n // 10] = 1.0
+    q[n // 10: n // 5] = -1.0
+    ke, eps, sigma = 138.935458, 45.0, 0.5
+    ref = F.update_field_force_q(x.astype(np.float64), q, M, np.full(3, L), sigma, ke, eps)
+    e_self = F.self_energy_q(q, sigma, ke, eps)
+    e_ref = F.field_q_energy(ref["phi_q"], ref["psi"], e_self, M, np.full(3, L))
+    pm = _pm((M,) * 3, (L,) * 3, 1)
+    xd, qd = _dev(x, torch.float32), _dev(q, torch.float32)
+    forces = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    phi_q = pm.create("real")
+    psi = pm.create("real")
+    ef = pm.create("real", batch=3)
+    e = pm.pme(xd, qd, sigma, ke / eps, e_self, forces, phi_q=phi_q, psi=psi, elec_field=ef, want_energy=True)
+    f = forces.cpu().numpy()
+    charged = q != 0
+    assert rms_rel(f[charged], ref["forces"][charged]) < 1e-5
+    assert np.abs(f[~charged]).max() == 0.0
+    assert rms_rel(phi_q.cpu().numpy(), ref["phi_q"]) < 1e-6
+    assert rms_rel(psi.cpu().numpy(), ref["psi"]) < 1e-5
+    for d in range(3):
+        assert rms_rel(ef[d].cpu().numpy(), ref["elec_field"][d]) < 1e-5
+    assert e == pytest.approx(e_ref, rel=1e-5)
+    assert pm.field_q_energy(phi_q, psi, e_self) == pytest.approx(e_ref, rel=1e-5)
+
+
+def test_drop_in_field_surface(hip):
+    """initialize_pm / update_field / compute_field_force /
+    compute_field_and_kinetic_energy with the reference's call signatures."""
+    from hymd_amd import field as HF
+    from hymd_amd.config import Chi, Config
+    from hymd_amd.hamiltonian import DefaultWithChi
+    s = _diblock(n_chains=300, box=6.0)
+    cfg = Config(n_steps=1, time_step=0.03, mesh_size=32, sigma=0.5, kappa=0.05, box_size=s.box,
+                 n_particles=s.n, n_types=2, unique_names=["A", "B"], type_to_name_map={0: "A", 1: "B"},
+                 chi=[Chi("A", "B", 15.0)])
+    ham = DefaultWithChi(cfg, cfg.unique_names, cfg.type_to_name_map)
+    pm, field_list, _, _ = HF.initialize_pm(None, cfg)
+    phi, phi_fourier, force_on_grid, v_ext_fourier, v_ext, phi_transfer, phi_laplacian = field_list
+    x = _dev(s.positions.astype(np.float32), torch.float32)
+    types = _dev(s.types, torch.int32)
+    HF.update_field(phi, phi_laplacian, phi_transfer, None, force_on_grid, ham, pm, x, types, cfg, v_ext,
+                    phi_fourier, v_ext_fourier, cfg.m, compute_potential=True)
+    forces = torch.zeros((s.n, 3), dtype=torch.float32, device="cuda")
+    HF.compute_field_force(None, x, force_on_grid, forces, types, 2)
+    v = torch.as_tensor(s.velocities).cuda()
+    e_field, e_kin, e_q = HF.compute_field_and_kinetic_energy(phi, None, None, v, ham, x, types, v_ext, cfg, None)
+
+    A, b, c0 = ham.quadratic()
+    oham = F.Quadratic(A, b, c0)
+    xr = s.positions.astype(np.float32).astype(np.float64)
+    ref = F.update_field(xr, s.types, 2, s.mesh, s.box, 0.5, oham)
+    f_ref = F.compute_field_force(xr, s.types, ref["force_mesh"], s.mesh, s.box)
+    assert rms_rel(forces.cpu().numpy(), f_ref) < 1e-5
+    assert e_field == pytest.approx(F.field_energy(ref["phi"], oham, s.mesh, s.box), rel=1e-6)
+    assert pm.last_field_energy == pytest.approx(e_field, rel=1e-6)
+    assert e_kin == pytest.approx(F.kinetic_energy(s.velocities, cfg.mass), rel=1e-12)
+    assert e_q == 0.0
+    for t in range(2):
+        assert rms_rel(phi[t].cpu().numpy(), ref["phi"][t]) < 1e-6

@@ -1,0 +1,104 @@
+"""rRESPA kernels and the device MD loop.
+
+* kick / drift kernels against the closed-form recurrences of
+  test/test_integrator.py:60-298 (FP64, 1e-13 absolute as the reference);
+* np.mod wrapping semantics of main.py:837;
+* a few outer steps of RespaMD against the FP64 oracle loop.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from tests.conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _kick_drift(x, v, f_fast, c_fast, f_slow, c_slow, dt, box):
+    from hymd_amd import _lib
+    from hymd_amd._lib import call, darr, dptr
+    call("hymd_respa_kick_drift", dptr(x), dptr(v), None, x.shape[0], dptr(f_fast), c_fast, dptr(f_slow), c_slow,
+         dt, darr(box, 3), _lib.stream_ptr())
+
+
+def _kick(v, f1, c1, f2=None, c2=0.0):
+    from hymd_amd import _lib
+    from hymd_amd._lib import call, dptr
+    call("hymd_respa_kick", dptr(v), v.shape[0], dptr(f1), c1, dptr(f2), c2, _lib.stream_ptr())
+
+
+@pytest.mark.parametrize("respa", [1, 2, 3, 4, 5, 10, 20])
+def test_respa_constant_force_recurrence(hip, goldens, respa):
+    """test/test_integrator.py:225-298 with m = 1 (the kernels take F/m
+    coefficients) and float32-representable accelerations."""
+    g = goldens["three_atoms"]
+    Dt = g["dt"]
+    a_l = np.array(goldens["integrator"]["acceleration_long"]).astype(np.float32)
+    a_s = np.array(goldens["integrator"]["acceleration_short"]).astype(np.float32)
+    x0 = np.array(g["positions"])
+    v0 = np.array(g["velocities"])
+    box = np.array([1e6, 1e6, 1e6])  # no wrapping
+    x = torch.as_tensor(x0.copy()).cuda()
+    v = torch.as_tensor(v0.copy()).cuda()
+    fl = torch.as_tensor(a_l).cuda()
+    fs = torch.as_tensor(a_s).cuda()
+    M = respa
+    dt = Dt / M
+    al, as_ = a_l.astype(np.float64), a_s.astype(np.float64)
+    for step in range(1, 10):
+        v_o = v.cpu().numpy()
+        p_o = x.cpu().numpy()
+        for s in range(M):
+            if s == 0:
+                _kick_drift(x, v, fs, 0.5 * dt, fl, 0.5 * Dt, dt, box)
+            else:
+                _kick_drift(x, v, fs, 0.5 * dt, fs, 0.5 * dt, dt, box)
+        _kick(v, fs, 0.5 * dt, fl, 0.5 * Dt)
+        v_exp = v0 + step * (Dt * al + M * dt * as_)
+        p_exp = p_o + (dt * M * v_o + dt * M / 2 * Dt * al + dt * M ** 2 / 2 * dt * as_)
+        np.testing.assert_allclose(v.cpu().numpy(), v_exp, atol=1e-13, rtol=0)
+        np.testing.assert_allclose(x.cpu().numpy(), p_exp, atol=1e-13, rtol=0)
+
+
+def test_wrap_matches_np_mod(hip):
+    box = np.array([3.0, 4.0, 5.0])
+    rng = np.random.default_rng(0)
+    x0 = rng.uniform(-10, 10, (1000, 3))
+    x0[0] = [-1e-20, -1e-17, 0.0]
+    v0 = rng.normal(size=(1000, 3))
+    x = torch.as_tensor(x0.copy()).cuda()
+    v = torch.as_tensor(v0.copy()).cuda()
+    zero = torch.zeros((1000, 3), dtype=torch.float32, device="cuda")
+    _kick_drift(x, v, zero, 0.0, None, 0.0, 0.01, box)
+    np.testing.assert_array_equal(x.cpu().numpy(), np.mod(x0 + 0.01 * v0, box[None, :]))
+
+
+def test_respa_md_matches_oracle_loop(hip, oracle_built):
+    """Two outer steps (inner = 5) of the diblock melt: device loop (FP32 forces,
+    FP64 x/v) against the FP64 oracle loop of main.py."""
+    from hymd_amd import synth
+    from hymd_amd.md import RespaMD
+    from oracle import field as F
+    from oracle import md as OM
+    s = synth.diblock_melt(n_chains=400, box=7.0, mesh=32, seed=1000)
+    A, b, c0 = synth.hamiltonian_form(s)
+    ref = OM.State(s, F.Quadratic(A, b, c0))
+    md = RespaMD(s)
+    assert rel_err(md.f_field.cpu().numpy(), ref.f_field) < 1e-4
+    assert rel_err(md.f_fast.cpu().numpy(), ref.f_fast) < 1e-5
+    for _ in range(2):
+        ref.outer_step()
+        md.step()
+    x = md.x.cpu().numpy()
+    dx = x - ref.x
+    dx -= s.box * np.round(dx / s.box)
+    assert np.abs(dx).max() < 1e-5
+    assert rel_err(md.v.cpu().numpy(), ref.v) < 1e-4
+    e = md.step(energies=True)
+    ref.outer_step()
+    assert e["kinetic"] == pytest.approx(ref.kinetic_energy(), rel=1e-6)
+    assert e["field"] == pytest.approx(ref.field_energy(), rel=1e-5)
+    assert e["bond"] == pytest.approx(ref.e_fast["bond"], rel=1e-6)
+    assert e["angle"] == pytest.approx(ref.e_fast["angle"], rel=1e-6)
