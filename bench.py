@@ -5,10 +5,13 @@ rRESPA inner = 10 (bonds + angles every inner step, particle-field forces
 every outer step), synthetic seeded inputs already resident in HBM.
 A step is one outer rRESPA step; value = N * steps / wall seconds.
 
-Prints one JSON line.  ``roofline`` is for one kernel (``--roofline-kernel``),
-timed with HIP events around its launches on the launch stream inside the
-timed region; ``cpu_baseline`` times the FP64 oracle (same algorithm, numpy +
-scipy.fft + the C restatement of the bonded kernels) on a bounded sample.
+Prints one JSON line on stdout (progress goes to stderr).  The timed region
+replays one captured outer step (hipGraph) K times.  ``roofline`` is for one
+stage (``--roofline-kernel``): the same outer steps are then launched eagerly
+by the library, which records HIP events on the launch stream around every
+launch of that stage.  ``cpu_baseline`` times the FP64 oracle (same algorithm:
+numpy + scipy.fft + the C restatement of the bonded kernels) on a bounded
+sample of the same workload.
 """
 import argparse
 import json
@@ -24,6 +27,8 @@ if ROOT not in sys.path:
 
 METRIC = "particle-steps/s, 1M beads/128³ mesh (fp32 vs fp64 ref) + batched χ-grads/s"
 HBM_PEAK_GBS = 8000.0
+KERNEL_NAMES = {"bonded": "topo_gather_kernel", "kick_drift": "kick_drift_kernel",
+                "field": "update_field (paint + FFT + k-space kernels)", "readout": "readout_kernel"}
 
 
 def algorithmic_bytes(kernel, n, n_bonds, n_angles, T, mesh):
@@ -33,9 +38,11 @@ def algorithmic_bytes(kernel, n, n_bonds, n_angles, T, mesh):
         return 36 * n + 24 * n_bonds + 28 * n_angles
     if kernel == "kick_drift":  # v,x f64 read+write, F f32 in, x f32 out
         return 120 * n
-    if kernel == "field":       # paint + r2c + k-pass + c2r + readout of one update
+    if kernel == "field":       # paint + r2c + k-pass + c2r of one update
         Gc = mesh[0] * mesh[1] * (mesh[2] // 2 + 1)
-        return (16 * n + 8 * T * G) + T * (4 * G + 8 * Gc) + 16 * T * Gc + 3 * T * (8 * Gc + 4 * G) + (28 * n + 12 * T * G)
+        return (16 * n + 8 * T * G) + T * (4 * G + 8 * Gc) + 16 * T * Gc + 3 * T * (8 * Gc + 4 * G)
+    if kernel == "readout":     # x f32 + type in, 3 grids per type once, F f32 out
+        return 28 * n + 12 * T * G
     raise ValueError(kernel)
 
 
@@ -61,14 +68,19 @@ def cpu_baseline(system, steps):
                           f"single-threaded paint/readout and bonded C loops"}
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--roofline-kernel", default="bonded", choices=["bonded", "kick_drift", "field"])
-    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle outer steps for cpu_baseline (0 = skip)")
+    ap.add_argument("--roofline-kernel", default="bonded", choices=["bonded", "kick_drift", "field", "readout"])
+    ap.add_argument("--cpu-steps", type=int, default=4, help="oracle outer steps for cpu_baseline (0 = skip)")
     ap.add_argument("--small", action="store_true", help="reduced system for quick checks (not a bench line)")
+    ap.add_argument("--no-graph", action="store_true", help="launch each step's kernels eagerly (no hipGraph)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
@@ -86,49 +98,47 @@ def main():
     A, b, c0 = synth.hamiltonian_form(s)
     s.extra.update(A=A, b=b, c0=c0)
     torch.cuda.set_device(0)
+    log(f"system ready: {s.n} beads, mesh {s.mesh}")
     md = RespaMD(s)
+    log("device state and initial forces ready")
     f_field0 = md.f_field.cpu().numpy()
     f_fast0 = md.f_fast.cpu().numpy()
-    for _ in range(args.warmup):
-        md.step()
+    md.run(args.warmup, use_graph=not args.no_graph)
     torch.cuda.synchronize()
 
-    stream = torch.cuda.current_stream()
-    events = []
-
-    def timer(name, fn):
-        if name != args.roofline_kernel:
-            return fn()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        out = fn()
-        e1.record(stream)
-        events.append((e0, e1))
-        return out
-
-    torch.cuda.synchronize()
+    # timed region: K outer steps (one captured step replayed K times)
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        md.step(timer=timer)
+    md.run(args.steps, use_graph=not args.no_graph)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = s.n * args.steps / elapsed
+    log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step, {value:.4g} particle-steps/s")
 
-    roofline = None
-    if events:
-        dur_ms = float(np.mean([a.elapsed_time(b) for a, b in events]))
-        nbytes = algorithmic_bytes(args.roofline_kernel, s.n, len(s.bonds[0]), len(s.angles[0]), s.n_types, s.mesh)
-        achieved = nbytes / (dur_ms * 1e-3) / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": args.roofline_kernel, "avg_launch_us": round(dur_ms * 1e3, 2),
-                     "algorithmic_bytes": int(nbytes)}
+    # roofline: the same outer steps launched eagerly by the library with HIP
+    # events recorded on the launch stream around every launch of one stage
+    md.time_stage(args.roofline_kernel)
+    rsteps = max(1, min(args.steps, 20))
+    t0 = time.perf_counter()
+    md.run(rsteps, use_graph=False)
+    torch.cuda.synchronize()
+    eager_ms = 1e3 * (time.perf_counter() - t0) / rsteps
+    dur_ms, launches = md.stage_time()
+    md.time_stage(None)
+    nbytes = algorithmic_bytes(args.roofline_kernel, s.n, len(s.bonds[0]), len(s.angles[0]), s.n_types, s.mesh)
+    achieved = nbytes / (dur_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": KERNEL_NAMES[args.roofline_kernel], "avg_launch_us": round(dur_ms * 1e3, 2),
+                "launches_timed": launches, "algorithmic_bytes": int(nbytes),
+                "eager_ms_per_step": round(eager_ms, 4)}
 
+    log(f"roofline stage {args.roofline_kernel}: {dur_ms * 1e3:.2f} us/launch over {launches} launches")
     cpu = accuracy = None
     if args.cpu_steps > 0:
+        log(f"cpu baseline: {args.cpu_steps} oracle outer steps")
         f0, cpu = cpu_baseline(s, args.cpu_steps)
+        log(f"cpu baseline: {cpu['value']:.4g} particle-steps/s")
         fo_field, fo_fast = f0
         # FP32 device forces vs the FP64 oracle at the same initial configuration
         accuracy = {

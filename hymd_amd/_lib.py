@@ -55,6 +55,12 @@ SIGNATURES = {
     "hymd_respa_kick_drift": [_p, _p, _p, _i64, _p, _d, _p, _d, _d, _pd, _p],
     "hymd_respa_kick": [_p, _i64, _p, _d, _p, _d, _p],
     "hymd_kinetic_energy": [_p, _i64, _i32, _d, _pd, _p],
+    "hymd_md_create": [ctypes.POINTER(_p), _p, _p, _i64, _i32, _p, _p, _p, _p, _p, _p, _p, _pd, _pd, _d, _pd, _d,
+                       _d, _d, _i32, _pd],
+    "hymd_md_step": [_p, _i32, _i32, _pd, _p],
+    "hymd_md_destroy": [_p],
+    "hymd_md_time_stage": [_p, _i32, _i32],
+    "hymd_md_stage_time": [_p, _pd, _pi32],
 }
 _RESTYPES = {"hymd_last_error": ctypes.c_char_p}
 
@@ -99,6 +105,12 @@ def require_gpu():
 # ---------------------------------------------------------------------------
 # argument helpers
 # ---------------------------------------------------------------------------
+class DevPtr(ctypes.c_void_p):
+    """A device pointer that keeps its tensor alive: a temporary tensor passed
+    as dptr(f(x)) must not return to torch's allocator (and be handed to the
+    next argument) before the library call that uses it."""
+
+
 def dptr(t, dtype=None, name="tensor"):
     """Device pointer of a contiguous CUDA/HIP tensor (None -> NULL)."""
     if t is None:
@@ -109,7 +121,9 @@ def dptr(t, dtype=None, name="tensor"):
         raise TypeError(f"{name} must have dtype {dtype}, got {t.dtype}")
     if not t.is_contiguous():
         raise ValueError(f"{name} must be contiguous")
-    return _p(t.data_ptr())
+    p = DevPtr(t.data_ptr())
+    p.tensor = t
+    return p
 
 
 def darr(values, n=None):

@@ -74,7 +74,7 @@ int fft_plan_init(FftPlan &plan, const Mesh &mesh) {
     plan.lds_x = sizeof(float2) * (size_t)(plan.ax.n + 2 * plan.wx * plan.ax.ld);
     plan.lds_y = sizeof(float2) * (size_t)(plan.ay.n + 2 * plan.wy * plan.ay.ld);
     plan.lds_z = sizeof(float2) * (size_t)(plan.azh.n + mesh.zh + 2 * plan.wz * plan.azh.ld);
-    const size_t lds_max = 160 * 1024;
+    const size_t lds_max = 64 * 1024;  // default dynamic-LDS limit without an opt-in attribute
     if (plan.lds_x > lds_max || plan.lds_y > lds_max || plan.lds_z > lds_max) {
         set_error("mesh too large for the LDS-staged FFT");
         return HYMD_ERR_UNSUPPORTED;

@@ -1,0 +1,211 @@
+// rRESPA outer step enqueued from C++ (main.py:802-887, 975-1004, 1143-1169),
+// optionally captured once into a hipGraph and replayed.
+//
+// Per outer step:  kick_drift (outer half kick + inner half kick + drift + wrap)
+//                  -> bonded forces, repeated respa_inner times (later inner
+//                  steps kick with the previous and the current half of F_fast),
+//                  -> update_field -> readout -> final kick (inner + outer halves).
+// The graph holds kernel nodes only: the paint grid is cleared by the z pass
+// of the forward FFT, so nothing in the step is a memset.
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+
+using namespace hymd;
+
+struct hymd_md {
+    hymd_pm *pm = nullptr;
+    hymd_topology *topo = nullptr;
+    hipStream_t work = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int64_t n = 0;
+    int T = 0;
+    double *x = nullptr, *v = nullptr;
+    float *x32 = nullptr, *f_field = nullptr, *f_fast = nullptr, *force_mesh = nullptr;
+    const int32_t *types = nullptr;
+    double A[HYMD_MAX_TYPES * HYMD_MAX_TYPES], b[HYMD_MAX_TYPES], c0 = 0.0, mass_t[HYMD_MAX_TYPES];
+    double sigma = 0.0, mass = 1.0, dt = 0.0, box[3] = {0, 0, 0};
+    int respa_inner = 1;
+    // optional HIP-event timing of one stage in eager steps
+    int timed = 0;  // 0 none, 1 bonded forces, 2 kick+drift, 3 update_field, 4 readout
+    std::vector<hipEvent_t> ev;
+    size_t ev_used = 0;
+};
+
+static void mark(hymd_md *m, int stage) {
+    if (m->timed == stage && m->ev_used < m->ev.size()) (void)hipEventRecord(m->ev[m->ev_used++], m->work);
+}
+
+// energies (host, 5): field, bond, angle, kinetic, (unused = 0)
+static int enqueue_step(hymd_md *m, double *energies) {
+    const double half_in = 0.5 * m->dt / m->mass;
+    const double half_out = 0.5 * m->respa_inner * m->dt / m->mass;
+    HYMD_TRY(hymd_pm_set_stream(m->pm, m->work));
+    for (int i = 0; i < m->respa_inner; ++i) {
+        const bool first = (i == 0);
+        mark(m, 2);
+        HYMD_TRY(hymd_respa_kick_drift(m->x, m->v, m->x32, m->n, m->f_fast, half_in, first ? m->f_field : m->f_fast,
+                                       first ? half_out : half_in, m->dt, m->box, m->work));
+        mark(m, 2);
+        const bool last_e = energies && (i == m->respa_inner - 1);
+        double eb[2];
+        mark(m, 1);
+        HYMD_TRY(hymd_topology_forces(m->topo, m->x, m->f_fast, m->box, last_e ? eb : nullptr, nullptr, m->work));
+        mark(m, 1);
+        if (last_e) {
+            energies[1] = eb[0];
+            energies[2] = eb[1];
+        }
+    }
+    mark(m, 3);
+    HYMD_TRY(hymd_update_field(m->pm, m->x32, m->types, m->n, m->T, m->A, m->b, m->c0, m->mass_t, m->sigma,
+                               m->force_mesh, nullptr, nullptr, nullptr, energies ? &energies[0] : nullptr));
+    mark(m, 3);
+    mark(m, 4);
+    HYMD_TRY(hymd_compute_field_force(m->pm, m->x32, m->types, m->n, m->T, m->force_mesh, m->f_field));
+    mark(m, 4);
+    HYMD_TRY(hymd_respa_kick(m->v, m->n, m->f_fast, half_in, m->f_field, half_out, m->work));
+    if (energies) HYMD_TRY(hymd_kinetic_energy(m->v, m->n, HYMD_F64, m->mass, &energies[3], m->work));
+    return HYMD_OK;
+}
+
+extern "C" int hymd_md_create(hymd_md **out, hymd_pm *pm, hymd_topology *topo, int64_t n, int32_t n_types,
+                              double *x, double *v, float *x32, const int32_t *types, float *f_field, float *f_fast,
+                              float *force_mesh, const double *coupling, const double *shift, double constant,
+                              const double *type_mass, double sigma, double mass, double dt, int32_t respa_inner,
+                              const double box[3]) {
+    HYMD_REQUIRE(out && pm && topo && box && coupling && n >= 0, "bad argument");
+    HYMD_REQUIRE(n == 0 || (x && v && x32 && types && f_field && f_fast && force_mesh), "null array");
+    HYMD_REQUIRE(n_types >= 1 && n_types <= HYMD_MAX_TYPES, "n_types out of range");
+    HYMD_REQUIRE(respa_inner >= 1 && mass > 0.0 && dt > 0.0, "respa_inner, mass and dt must be positive");
+    hymd_md *m = new hymd_md();
+    m->pm = pm;
+    m->topo = topo;
+    m->n = n;
+    m->T = n_types;
+    m->x = x;
+    m->v = v;
+    m->x32 = x32;
+    m->types = types;
+    m->f_field = f_field;
+    m->f_fast = f_fast;
+    m->force_mesh = force_mesh;
+    for (int i = 0; i < n_types * n_types; ++i) m->A[i] = coupling[i];
+    for (int t = 0; t < n_types; ++t) {
+        m->b[t] = shift ? shift[t] : 0.0;
+        m->mass_t[t] = type_mass ? type_mass[t] : 1.0;
+    }
+    m->c0 = constant;
+    m->sigma = sigma;
+    m->mass = mass;
+    m->dt = dt;
+    m->respa_inner = respa_inner;
+    for (int d = 0; d < 3; ++d) m->box[d] = box[d];
+    hipError_t e = hipStreamCreateWithFlags(&m->work, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_in, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_out, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        set_error("stream/event creation failed: %s", hipGetErrorString(e));
+        if (m->work) (void)hipStreamDestroy(m->work);
+        delete m;
+        return HYMD_ERR_HIP;
+    }
+    *out = m;
+    return HYMD_OK;
+}
+
+extern "C" int hymd_md_destroy(hymd_md *m) {
+    if (!m) return HYMD_OK;
+    if (m->work) (void)hipStreamSynchronize(m->work);
+    for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
+    if (m->exec) (void)hipGraphExecDestroy(m->exec);
+    if (m->graph) (void)hipGraphDestroy(m->graph);
+    if (m->ev_in) (void)hipEventDestroy(m->ev_in);
+    if (m->ev_out) (void)hipEventDestroy(m->ev_out);
+    if (m->work) (void)hipStreamDestroy(m->work);
+    delete m;
+    return HYMD_OK;
+}
+
+static int capture(hymd_md *m) {
+    const int timed = m->timed;
+    m->timed = 0;  // no event records inside the graph
+    HYMD_HIP(hipStreamBeginCapture(m->work, hipStreamCaptureModeThreadLocal));
+    int st = enqueue_step(m, nullptr);
+    m->timed = timed;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(m->work, &g);
+    if (st != HYMD_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return st;
+    }
+    if (e != hipSuccess) {
+        set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
+        return HYMD_ERR_HIP;
+    }
+    m->graph = g;
+    HYMD_HIP(hipGraphInstantiate(&m->exec, g, nullptr, nullptr, 0));
+    return HYMD_OK;
+}
+
+// Runs n_steps outer steps ordered after the work already queued on `stream`,
+// and makes `stream` wait for them.  use_graph != 0 replays a captured graph of
+// one step.  energies (host double[5]) or NULL: energies after the last step
+// (that step runs eagerly).
+extern "C" int hymd_md_step(hymd_md *m, int32_t n_steps, int32_t use_graph, double *energies, void *stream) {
+    HYMD_REQUIRE(m && n_steps >= 0, "bad argument");
+    hipStream_t caller = (hipStream_t)stream;
+    HYMD_HIP(hipEventRecord(m->ev_in, caller));
+    HYMD_HIP(hipStreamWaitEvent(m->work, m->ev_in, 0));
+    const int plain = energies ? n_steps - 1 : n_steps;
+    if (use_graph && plain > 0 && !m->exec) HYMD_TRY(capture(m));
+    for (int s = 0; s < plain; ++s) {
+        if (use_graph)
+            HYMD_HIP(hipGraphLaunch(m->exec, m->work));
+        else
+            HYMD_TRY(enqueue_step(m, nullptr));
+    }
+    if (energies && n_steps > 0) {
+        std::memset(energies, 0, 5 * sizeof(double));
+        HYMD_TRY(enqueue_step(m, energies));
+    }
+    HYMD_HIP(hipEventRecord(m->ev_out, m->work));
+    HYMD_HIP(hipStreamWaitEvent(caller, m->ev_out, 0));
+    return HYMD_OK;
+}
+
+// Time one stage of the eager steps that follow with HIP events recorded on
+// the launch stream around every launch of that stage (graph replays are not
+// timed).  stage: 0 off, 1 bonded forces, 2 kick+drift, 3 update_field (all its
+// kernels), 4 readout.  Up to max_launches launches are recorded.
+extern "C" int hymd_md_time_stage(hymd_md *m, int32_t stage, int32_t max_launches) {
+    HYMD_REQUIRE(m && stage >= 0 && stage <= 4 && max_launches >= 0, "bad argument");
+    HYMD_HIP(hipStreamSynchronize(m->work));
+    for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
+    m->ev.clear();
+    m->ev_used = 0;
+    m->timed = stage;
+    if (stage == 0) return HYMD_OK;
+    m->ev.resize(2 * (size_t)max_launches, nullptr);
+    for (auto &e : m->ev) HYMD_HIP(hipEventCreate(&e));
+    return HYMD_OK;
+}
+
+// Total milliseconds and number of timed launches so far (synchronises).
+extern "C" int hymd_md_stage_time(hymd_md *m, double *total_ms, int32_t *launches) {
+    HYMD_REQUIRE(m && total_ms && launches, "bad argument");
+    HYMD_HIP(hipStreamSynchronize(m->work));
+    double tot = 0.0;
+    const size_t pairs = m->ev_used / 2;
+    for (size_t i = 0; i < pairs; ++i) {
+        float ms = 0.f;
+        HYMD_HIP(hipEventElapsedTime(&ms, m->ev[2 * i], m->ev[2 * i + 1]));
+        tot += ms;
+    }
+    *total_ms = tot;
+    *launches = (int32_t)pairs;
+    return HYMD_OK;
+}

@@ -217,21 +217,23 @@ def _packed(grids, dtype):
 
 
 class _Out:
-    """A packed output: writes straight into the caller's grids when they are
-    packed, otherwise into a temporary copied back afterwards."""
+    """A packed output of ``n`` grids of ``elem_shape``: the caller's grids
+    when they are consecutive slices of one buffer, else a temporary copied
+    back by finish()."""
 
-    def __init__(self, grids, shape, dtype, device):
+    def __init__(self, grids, elem_shape, dtype, device):
         self.grids = grids
-        self.flat = _packed(grids, dtype) if grids is not None else None
+        self.tensor = None
+        self.flat = None
         if grids is None:
-            self.tensor = None
-        elif self.flat is not None:
-            n = len(self.flat)
-            self.tensor = torch.as_strided(self.flat[0], (n,) + tuple(shape),
-                                           (self.flat[0].numel(),) + tuple(self.flat[0].stride()))
+            return
+        self.flat = _packed(grids, dtype)
+        n = len(_flatten(grids))
+        if self.flat is not None:
+            f0 = self.flat[0]
+            self.tensor = torch.as_strided(f0, (n,) + tuple(elem_shape), (f0.numel(),) + tuple(f0.stride()))
         else:
-            n = len(_flatten(grids))
-            self.tensor = torch.empty((n,) + tuple(shape), dtype=dtype, device=device)
+            self.tensor = torch.empty((n,) + tuple(elem_shape), dtype=dtype, device=device)
 
     def finish(self):
         if self.grids is not None and self.flat is None:
@@ -293,12 +295,7 @@ def update_field(phi, phi_laplacian, phi_transfer, layouts, force_mesh, hamilton
     ty = _device_tensor(types, torch.int32, "types")
     mass = list(m) if m is not None else [1.0] * T
     dev = pm.device
-    fm = _Out(force_mesh, (3,) + pm.real_shape, torch.float32, dev)
-    if fm.flat is not None:
-        fm.tensor = torch.as_strided(fm.flat[0], (T, 3) + pm.real_shape,
-                                     (3 * fm.flat[0].numel(), fm.flat[0].numel()) + tuple(fm.flat[0].stride()))
-    else:
-        fm.tensor = torch.empty((T, 3) + pm.real_shape, dtype=torch.float32, device=dev)
+    fm = _Out(force_mesh, pm.real_shape, torch.float32, dev)
     ph = _Out(phi, pm.real_shape, torch.float32, dev)
     pf = _Out(phi_fourier, pm.complex_shape, torch.complex64, dev)
     ve = _Out(v_ext if compute_potential else None, pm.real_shape, torch.float32, dev)

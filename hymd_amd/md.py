@@ -56,6 +56,43 @@ class RespaMD:
         # initial forces (main.py:331-371, 508-551)
         self.slow_forces()
         self.fast_forces()
+        self._native = None
+
+    # ------------------------------------------------------------------
+    def _native_handle(self):
+        if self._native is None:
+            h = ctypes.c_void_p()
+            call("hymd_md_create", ctypes.byref(h), self.pm._h, self.topo._h, self.n, self.T, dptr(self.x),
+                 dptr(self.v), dptr(self.x32), dptr(self.types), dptr(self.f_field), dptr(self.f_fast),
+                 dptr(self.force_mesh), darr(np.ravel(self.A)), darr(self.b), float(self.c0),
+                 darr(self.type_mass), self.sigma, self.mass, self.dt, self.n_inner, self._box_c)
+            self._native = h
+        return self._native
+
+    def run(self, n_steps, use_graph=True, energies=False):
+        """``n_steps`` outer steps enqueued by the library (one ctypes call;
+        with ``use_graph`` one captured step is replayed).  With ``energies``
+        the last step reports field, bond, angle and kinetic energies."""
+        e = (ctypes.c_double * 5)() if energies else None
+        call("hymd_md_step", self._native_handle(), int(n_steps), 1 if use_graph else 0, e, _lib.stream_ptr())
+        if energies:
+            self.energies.update(field=e[0], bond=e[1], angle=e[2], kinetic=e[3])
+            return dict(self.energies)
+        return None
+
+    STAGES = {"bonded": 1, "kick_drift": 2, "field": 3, "readout": 4}
+
+    def time_stage(self, name, max_launches=10000):
+        """Record HIP events around every launch of one stage in the eager
+        (``use_graph=False``) steps that follow."""
+        call("hymd_md_time_stage", self._native_handle(), self.STAGES[name] if name else 0, int(max_launches))
+
+    def stage_time(self):
+        """(mean milliseconds per timed launch, launches)."""
+        tot = ctypes.c_double(0.0)
+        cnt = ctypes.c_int32(0)
+        call("hymd_md_stage_time", self._native_handle(), ctypes.byref(tot), ctypes.byref(cnt))
+        return (tot.value / cnt.value if cnt.value else float("nan")), cnt.value
 
     # ------------------------------------------------------------------
     def slow_forces(self, energy=False):
@@ -107,5 +144,8 @@ class RespaMD:
         return self.energies if energies else None
 
     def close(self):
+        if self._native is not None:
+            _lib.lib().hymd_md_destroy(self._native)
+            self._native = None
         self.topo.close()
         self.pm.close()

@@ -169,6 +169,29 @@ int hymd_respa_kick(double *v, int64_t n, const float *f1, double c1, const floa
 /* 1/2 m sum v^2 (field.py:695) into host double; v (n,3) float32 or float64 per precision */
 int hymd_kinetic_energy(const void *v, int64_t n, int32_t precision, double mass, double *out, void *stream);
 
+/* ------------------------------------------------------------------------ */
+/* Whole rRESPA outer steps enqueued from C (the MD loop of main.py:764-1169 */
+/* restricted to the hot path: kicks, drift + wrap, bonds + angles every     */
+/* inner step, update_field + compute_field_force every outer step).  The    */
+/* handle references the caller's arrays, the pm handle and the topology.    */
+/* ------------------------------------------------------------------------ */
+typedef struct hymd_md hymd_md;
+int hymd_md_create(hymd_md **md, hymd_pm *pm, hymd_topology *topo, int64_t n, int32_t n_types, double *x, double *v,
+                   float *x32, const int32_t *types, float *f_field, float *f_fast, float *force_mesh,
+                   const double *coupling, const double *shift, double constant, const double *type_mass,
+                   double sigma, double mass, double dt, int32_t respa_inner, const double box[3]);
+/* n_steps outer steps ordered after the work queued on `stream`, which then
+ * waits for them; use_graph replays one captured step as a hipGraph;
+ * energies host double[5] (field, bond, angle, kinetic, 0) after the last step, or NULL */
+int hymd_md_step(hymd_md *md, int32_t n_steps, int32_t use_graph, double *energies, void *stream);
+int hymd_md_destroy(hymd_md *md);
+/* HIP-event timing of one stage (1 bonded forces, 2 kick+drift, 3 update_field,
+ * 4 readout; 0 off) in the eager steps that follow, recorded on the launch
+ * stream around each launch; at most max_launches are recorded */
+int hymd_md_time_stage(hymd_md *md, int32_t stage, int32_t max_launches);
+/* total milliseconds and count of the timed launches so far (synchronises) */
+int hymd_md_stage_time(hymd_md *md, double *total_ms, int32_t *launches);
+
 #ifdef __cplusplus
 }
 #endif

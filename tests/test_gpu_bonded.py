@@ -78,12 +78,13 @@ def test_bonds_angles_vs_oracle(hip, oracle_built, single):
     f_ref, e_ref, pr_ref = oracle_built.cbf(r, box, a, b, r0, k)
     assert rel_err(f.cpu().numpy(), f_ref) < tol
     assert e == pytest.approx(e_ref, rel=1e-12)
-    np.testing.assert_allclose(pr, pr_ref, rtol=1e-10)
+    pr_tol = 1e-6 if single else 1e-10  # cbf returns bond_pr as real(4), cbf_d as real(8)
+    np.testing.assert_allclose(pr, pr_ref, rtol=pr_tol)
     e, pr = (K.caf if single else K.caf_d)(f, rd, box, a, b, c, t0, k)
     f_ref, e_ref, pr_ref = oracle_built.caf(r, box, a, b, c, t0, k)
     assert rel_err(f.cpu().numpy(), f_ref) < tol
     assert e == pytest.approx(e_ref, rel=1e-12)
-    np.testing.assert_allclose(pr, pr_ref, rtol=1e-9)
+    np.testing.assert_allclose(pr, pr_ref, rtol=max(pr_tol, 1e-9))
 
 
 @pytest.mark.parametrize("single", [False, True])
@@ -145,6 +146,8 @@ def test_topology_forces(hip, oracle_built):
     from hymd_amd.force import BondedTopology
     from hymd_amd import synth
     s = synth.lipid_water(n_chains=2000, n_water=5000, box=8.0, mesh=32, seed=5)
+    # random-walk bonds sit exactly at r0; move the beads off equilibrium
+    s.positions = np.mod(s.positions + np.random.default_rng(6).normal(scale=0.05, size=s.positions.shape), s.box)
     topo = BondedTopology(s.n, bonds=s.bonds, angles=s.angles)
     x = torch.as_tensor(s.positions).cuda()
     f = torch.zeros((s.n, 3), dtype=torch.float32, device="cuda")
