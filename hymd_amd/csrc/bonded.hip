@@ -643,6 +643,25 @@ extern "C" int hymd_topology_create(hymd_topology **out, int64_t n, int64_t m2, 
     return HYMD_OK;
 }
 
+namespace hymd {
+TopoTerms topology_terms(const hymd_topology *t) {
+    TopoTerms v;
+    v.n = t->n;
+    v.m2 = t->m2;
+    v.m3 = t->m3;
+    v.a2 = t->B.a;
+    v.b2 = t->B.b;
+    v.r0 = t->B.r0;
+    v.k2 = t->B.k;
+    v.a3 = t->A.a;
+    v.b3 = t->A.b;
+    v.c3 = t->A.c;
+    v.t0 = t->A.t0;
+    v.k3 = t->A.k;
+    return v;
+}
+}  // namespace hymd
+
 extern "C" int hymd_topology_destroy(hymd_topology *top) {
     delete top;
     return HYMD_OK;

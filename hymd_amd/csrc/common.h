@@ -88,19 +88,26 @@ __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y
 // multiply by +i (s>0) or -i (s<0)
 __device__ __forceinline__ float2 cmul_i(float2 a, float s) { return make_float2(-s * a.y, s * a.x); }
 
-// Deterministic block reduction of doubles (fixed tree order).
+// Deterministic block reduction of doubles (fixed order: a shuffle tree
+// inside each 64-wide wavefront, then the wavefront sums in order).  All
+// kBlock threads of the block must call it.
 template <int NV>
 __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *out_block) {
-    __shared__ double red[NV][kBlock];
-    for (int j = 0; j < NV; ++j) red[j][threadIdx.x] = v[j];
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s)
-            for (int j = 0; j < NV; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + s];
-        __syncthreads();
+    constexpr int kWaves = kBlock / 64;
+    __shared__ double red[NV][kWaves];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int j = 0; j < NV; ++j) {
+        double s = v[j];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        if (lane == 0) red[j][wid] = s;
     }
+    __syncthreads();
     if (threadIdx.x == 0)
-        for (int j = 0; j < NV; ++j) out_block[j] = red[j][0];
+        for (int j = 0; j < NV; ++j) {
+            double t = 0.0;
+            for (int w = 0; w < kWaves; ++w) t += red[j][w];
+            out_block[j] = t;
+        }
 }
 
 // Reduce `nblocks` partial rows of NV doubles (row-major) in fixed order into
@@ -115,6 +122,14 @@ struct DevBuf {
     void release();
     ~DevBuf() { release(); }
 };
+
+// Term arrays held by a persistent topology (device pointers).
+struct TopoTerms {
+    int64_t n, m2, m3;
+    const int32_t *a2, *b2, *a3, *b3, *c3;
+    const double *r0, *k2, *t0, *k3;
+};
+TopoTerms topology_terms(const hymd_topology *t);
 
 // Pinned host staging for small results.
 struct HostBuf {

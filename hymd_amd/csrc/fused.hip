@@ -1,0 +1,475 @@
+// Molecule-blocked fused rRESPA inner loop (main.py:830-887).
+//
+// Bonded terms couple particles of one molecule only, and the particle-field
+// force is constant during the inner steps.  When every molecule is a
+// contiguous index range, whole molecules are packed into blocks of at most
+// kBlock * PPT particles and one workgroup advances its block through all
+// respa_inner sub-steps: positions live in LDS, velocities and forces in
+// registers, and each sub-step evaluates every bond / angle term once into an
+// LDS slot, after which each particle sums the slots of its terms in term
+// order (bonds, then angles), as the per-particle gather kernel does.  HBM is
+// touched once per outer step: x, v, F_fast, F_slow in; x, x32, v, F_fast out.
+//
+// Kick order per outer step, identical to the unfused kernels:
+//   [pending outer half kick of the previous step] + outer half kick (F_slow)
+//   inner i: first half kick (F_fast), drift, wrap, F_fast(x), second half kick
+// where the second half of sub-step i and the first half of i+1 are separate
+// FP64 additions of the same force.
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+#include "bonded.h"
+
+namespace hymd {
+
+constexpr int kMaxTermsPerThread = 4;  // bonds and angles per thread and block
+
+struct FusedArgs {
+    double *x, *v;
+    float *x32, *f_fast;
+    const float *f_slow;
+    const int *blk_start;        // nb+1 global particle index
+    const int *blk_bond_off;     // nb+1 into bond_ab / bond_par
+    const uint32_t *bond_ab;     // a | b << 16 (block-local)
+    const double2 *bond_par;     // r0, k
+    const int *blk_ang_off;      // nb+1
+    const uint64_t *ang_abc;     // a | b << 16 | c << 32 (block-local)
+    const double2 *ang_par;      // theta0, k
+    const int *csr_off;          // N+1 (global)
+    const uint32_t *csr_ent;     // (term << 3) | (angle << 2) | role, term block-local
+    double h_in, h_out, dt;
+    double box[3];
+    int n_inner;
+    int pending;                 // apply the previous step's outer half kick first
+    int max_bonds, max_angles, max_entries, max_particles;  // LDS sizing
+    double *partials;            // 8 per block (E_bond, pr_bond[3], E_angle, pr_angle[3]) or null
+};
+
+__device__ __forceinline__ double wrap_np_mod(double x, double L) {
+    double r = fmod(x, L);
+    if (r != 0.0 && r < 0.0) r += L;
+    return r;
+}
+
+template <int PPT>
+__global__ void __launch_bounds__(kBlock) fused_inner_kernel(FusedArgs A) {
+    extern __shared__ double smem_d[];
+    const int blk = blockIdx.x;
+    const int p0 = A.blk_start[blk];
+    const int np = A.blk_start[blk + 1] - p0;
+    const int b0 = A.blk_bond_off[blk], nb = A.blk_bond_off[blk + 1] - b0;
+    const int a0 = A.blk_ang_off[blk], na = A.blk_ang_off[blk + 1] - a0;
+    const int e0 = A.csr_off[p0], ne = A.csr_off[p0 + np] - e0;
+    // LDS: positions, bond slots (fa), angle slots (fa, fc), CSR
+    double *xs = smem_d;                              // 3 * max_particles
+    double *sb = xs + 3 * A.max_particles;            // 3 * max_bonds
+    double *sa = sb + 3 * A.max_bonds;                // 6 * max_angles
+    int *coff = (int *)(sa + 6 * A.max_angles);       // max_particles + 1
+    uint32_t *ent = (uint32_t *)(coff + A.max_particles + 1);  // max_entries
+    for (int j = threadIdx.x; j <= np; j += blockDim.x) coff[j] = A.csr_off[p0 + j] - e0;
+    for (int j = threadIdx.x; j < ne; j += blockDim.x) ent[j] = A.csr_ent[e0 + j];
+    const Box box{{A.box[0], A.box[1], A.box[2]}};
+
+    double x[PPT][3], v[PPT][3];
+    float ff[PPT][3], fs[PPT][3];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int j = threadIdx.x + k * kBlock;
+        if (j < np) {
+            const int64_t g = 3 * (int64_t)(p0 + j);
+            for (int d = 0; d < 3; ++d) {
+                x[k][d] = A.x[g + d];
+                v[k][d] = A.v[g + d];
+                ff[k][d] = A.f_fast[g + d];
+                fs[k][d] = A.f_slow[g + d];
+            }
+        }
+    }
+    // own terms in registers
+    uint32_t bab[kMaxTermsPerThread];
+    double2 bpar[kMaxTermsPerThread];
+    uint64_t aabc[kMaxTermsPerThread];
+    double2 apar[kMaxTermsPerThread];
+#pragma unroll
+    for (int k = 0; k < kMaxTermsPerThread; ++k) {
+        const int tb = threadIdx.x + k * kBlock;
+        if (tb < nb) {
+            bab[k] = A.bond_ab[b0 + tb];
+            bpar[k] = A.bond_par[b0 + tb];
+        }
+        if (tb < na) {
+            aabc[k] = A.ang_abc[a0 + tb];
+            apar[k] = A.ang_par[a0 + tb];
+        }
+    }
+    double ev[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+
+    for (int it = 0; it < A.n_inner; ++it) {
+        // kicks + drift + wrap of own particles
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const int j = threadIdx.x + k * kBlock;
+            if (j >= np) continue;
+            for (int d = 0; d < 3; ++d) {
+                double vi = v[k][d];
+                if (it == 0) {
+                    if (A.pending) vi = vi + A.h_out * (double)fs[k][d];
+                    vi = vi + A.h_out * (double)fs[k][d];
+                } else {
+                    vi = vi + A.h_in * (double)ff[k][d];
+                }
+                vi = vi + A.h_in * (double)ff[k][d];
+                v[k][d] = vi;
+                x[k][d] = wrap_np_mod(x[k][d] + A.dt * vi, A.box[d]);
+                xs[3 * j + d] = x[k][d];
+            }
+        }
+        __syncthreads();
+        const bool last = (A.partials != nullptr) && (it == A.n_inner - 1);
+        // every term once into its slot
+#pragma unroll
+        for (int k = 0; k < kMaxTermsPerThread; ++k) {
+            const int tb = threadIdx.x + k * kBlock;
+            if (tb < nb) {
+                const int ia = bab[k] & 0xffff, ib = bab[k] >> 16;
+                const V3 rab = mic(v3(xs[3 * ib] - xs[3 * ia], xs[3 * ib + 1] - xs[3 * ia + 1],
+                                      xs[3 * ib + 2] - xs[3 * ia + 2]),
+                                   box);
+                double en;
+                const V3 fa = bond_term(rab, bpar[k].x, bpar[k].y, en);
+                sb[3 * tb] = fa.x;
+                sb[3 * tb + 1] = fa.y;
+                sb[3 * tb + 2] = fa.z;
+                if (last) {
+                    ev[0] += en;
+                    ev[1] += fa.x * rab.x;
+                    ev[2] += fa.y * rab.y;
+                    ev[3] += fa.z * rab.z;
+                }
+            }
+            if (tb < na) {
+                const int ia = (int)(aabc[k] & 0xffff), ib = (int)((aabc[k] >> 16) & 0xffff),
+                          ic = (int)(aabc[k] >> 32);
+                const V3 rb = v3(xs[3 * ib], xs[3 * ib + 1], xs[3 * ib + 2]);
+                const V3 ra = mic(v3(xs[3 * ia], xs[3 * ia + 1], xs[3 * ia + 2]) - rb, box);
+                const V3 rc = mic(v3(xs[3 * ic], xs[3 * ic + 1], xs[3 * ic + 2]) - rb, box);
+                V3 fa, fc;
+                double en;
+                angle_term(ra, rc, apar[k].x, apar[k].y, fa, fc, en);
+                sa[6 * tb] = fa.x;
+                sa[6 * tb + 1] = fa.y;
+                sa[6 * tb + 2] = fa.z;
+                sa[6 * tb + 3] = fc.x;
+                sa[6 * tb + 4] = fc.y;
+                sa[6 * tb + 5] = fc.z;
+                if (last) {
+                    ev[4] += en;
+                    ev[5] -= fa.x * ra.x + fc.x * rc.x;
+                    ev[6] -= fa.y * ra.y + fc.y * rc.y;
+                    ev[7] -= fa.z * ra.z + fc.z * rc.z;
+                }
+            }
+        }
+        __syncthreads();
+        // gather own particles' contributions in term order
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const int j = threadIdx.x + k * kBlock;
+            if (j >= np) continue;
+            double acc[3] = {0.0, 0.0, 0.0};
+            for (int q = coff[j]; q < coff[j + 1]; ++q) {
+                const uint32_t e = ent[q];
+                const int role = e & 3, kind = (e >> 2) & 1, t = (int)(e >> 3);
+                if (kind == 0) {
+                    const double s = role == 0 ? -1.0 : 1.0;
+                    for (int d = 0; d < 3; ++d) acc[d] += s * sb[3 * t + d];
+                } else {
+                    for (int d = 0; d < 3; ++d) {
+                        const double fa = sa[6 * t + d], fc = sa[6 * t + 3 + d];
+                        acc[d] += role == 0 ? -fa : (role == 2 ? -fc : fa + fc);
+                    }
+                }
+            }
+            for (int d = 0; d < 3; ++d) ff[k][d] = (float)acc[d];
+        }
+        // the barrier at the top of the next sub-step orders slot reuse
+    }
+    // second half kick of the last sub-step; write back
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int j = threadIdx.x + k * kBlock;
+        if (j >= np) continue;
+        const int64_t g = 3 * (int64_t)(p0 + j);
+        for (int d = 0; d < 3; ++d) {
+            const double vi = v[k][d] + A.h_in * (double)ff[k][d];
+            A.v[g + d] = vi;
+            A.x[g + d] = x[k][d];
+            A.x32[g + d] = (float)x[k][d];
+            A.f_fast[g + d] = ff[k][d];
+        }
+    }
+    if (A.partials) block_reduce_store<8>(ev, A.partials + 8 * blk);
+}
+
+}  // namespace hymd
+
+using namespace hymd;
+
+struct hymd_fused {
+    int64_t n = 0;
+    int nblocks = 0, ppt = 1;
+    int max_bonds = 0, max_angles = 0, max_entries = 0, max_particles = 0;
+    size_t lds = 0;
+    DevBuf blk_start, blk_bond_off, bond_ab, bond_par, blk_ang_off, ang_abc, ang_par, csr_off, csr_ent;
+    DevBuf partials, result;
+    HostBuf host;
+};
+
+namespace {
+
+struct UF {
+    std::vector<int> p;
+    explicit UF(int64_t n) : p(n) { std::iota(p.begin(), p.end(), 0); }
+    int find(int a) {
+        while (p[a] != a) a = p[a] = p[p[a]];
+        return a;
+    }
+    void join(int a, int b) {
+        a = find(a);
+        b = find(b);
+        if (a != b) p[std::max(a, b)] = std::min(a, b);
+    }
+};
+
+template <typename T>
+int upload(DevBuf &buf, const std::vector<T> &h) {
+    HYMD_TRY(buf.reserve(sizeof(T) * std::max<size_t>(h.size(), 1)));
+    if (!h.empty()) HYMD_HIP(hipMemcpy(buf.ptr, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    return HYMD_OK;
+}
+
+template <typename T>
+int download(std::vector<T> &h, const void *d, int64_t n) {
+    h.resize(n);
+    if (n > 0) HYMD_HIP(hipMemcpy(h.data(), d, sizeof(T) * n, hipMemcpyDeviceToHost));
+    return HYMD_OK;
+}
+
+}  // namespace
+
+// Builds the blocks from device term arrays.  Returns HYMD_ERR_UNSUPPORTED
+// (with a message) when a molecule is not a contiguous index range or does
+// not fit one workgroup; the caller then keeps the unfused kernels.
+extern "C" int hymd_fused_create(hymd_fused **out, int64_t n, int64_t m2, const int32_t *a2, const int32_t *b2,
+                                 const double *r0, const double *k2, int64_t m3, const int32_t *a3,
+                                 const int32_t *b3, const int32_t *c3, const double *t0, const double *k3) {
+    HYMD_REQUIRE(out && n >= 0 && m2 >= 0 && m3 >= 0 && n < (int64_t(1) << 31), "bad argument");
+    std::vector<int32_t> A2, B2, A3, B3, C3;
+    std::vector<double> R0, K2, T0, K3;
+    HYMD_TRY(download(A2, a2, m2));
+    HYMD_TRY(download(B2, b2, m2));
+    HYMD_TRY(download(R0, r0, m2));
+    HYMD_TRY(download(K2, k2, m2));
+    HYMD_TRY(download(A3, a3, m3));
+    HYMD_TRY(download(B3, b3, m3));
+    HYMD_TRY(download(C3, c3, m3));
+    HYMD_TRY(download(T0, t0, m3));
+    HYMD_TRY(download(K3, k3, m3));
+    auto bad = [n](int32_t i) { return i < 0 || (int64_t)i >= n; };
+    UF uf(n);
+    for (int64_t t = 0; t < m2; ++t) {
+        HYMD_REQUIRE(!bad(A2[t]) && !bad(B2[t]), "bond %lld has an index outside [0, %lld)", (long long)t, (long long)n);
+        uf.join(A2[t], B2[t]);
+    }
+    for (int64_t t = 0; t < m3; ++t) {
+        HYMD_REQUIRE(!bad(A3[t]) && !bad(B3[t]) && !bad(C3[t]), "angle %lld has an index outside [0, %lld)",
+                     (long long)t, (long long)n);
+        uf.join(A3[t], B3[t]);
+        uf.join(B3[t], C3[t]);
+    }
+    // components must be contiguous: root = smallest member, last member = root + size - 1
+    std::vector<int> size(n, 0), last(n, -1);
+    for (int64_t i = 0; i < n; ++i) {
+        const int r = uf.find((int)i);
+        size[r]++;
+        last[r] = (int)i;
+    }
+    const int ppt = 2;
+    const int pmax = kBlock * ppt;
+    std::vector<int> blk_start{0};
+    int cur = 0;
+    for (int64_t i = 0; i < n;) {
+        const int r = uf.find((int)i);
+        if (r != (int)i || last[r] - r + 1 != size[r]) {
+            set_error("molecule containing particle %lld is not a contiguous index range", (long long)i);
+            return HYMD_ERR_UNSUPPORTED;
+        }
+        if (size[r] > pmax) {
+            set_error("a molecule of %d beads exceeds the fused block size %d", size[r], pmax);
+            return HYMD_ERR_UNSUPPORTED;
+        }
+        if (cur + size[r] > pmax) {
+            blk_start.push_back((int)i);
+            cur = 0;
+        }
+        cur += size[r];
+        i += size[r];
+    }
+    if (blk_start.back() != (int)n) blk_start.push_back((int)n);
+    const int nblk = (int)blk_start.size() - 1;
+    std::vector<int> blk_of(n);
+    for (int b = 0; b < nblk; ++b)
+        for (int i = blk_start[b]; i < blk_start[b + 1]; ++i) blk_of[i] = b;
+    // terms per block, in global term order
+    std::vector<int> bcount(nblk + 1, 0), acount(nblk + 1, 0);
+    for (int64_t t = 0; t < m2; ++t) bcount[blk_of[A2[t]] + 1]++;
+    for (int64_t t = 0; t < m3; ++t) acount[blk_of[A3[t]] + 1]++;
+    std::partial_sum(bcount.begin(), bcount.end(), bcount.begin());
+    std::partial_sum(acount.begin(), acount.end(), acount.begin());
+    std::vector<uint32_t> bab(m2);
+    std::vector<double2> bpar(m2);
+    std::vector<uint64_t> aabc(m3);
+    std::vector<double2> apar(m3);
+    std::vector<int> bfill(bcount.begin(), bcount.end() - 1), afill(acount.begin(), acount.end() - 1);
+    std::vector<int> bond_local(m2), ang_local(m3);
+    for (int64_t t = 0; t < m2; ++t) {
+        const int b = blk_of[A2[t]], s = blk_start[b], slot = bfill[b]++;
+        bab[slot] = (uint32_t)(A2[t] - s) | ((uint32_t)(B2[t] - s) << 16);
+        bpar[slot] = make_double2(R0[t], K2[t]);
+        bond_local[t] = slot - bcount[b];
+    }
+    for (int64_t t = 0; t < m3; ++t) {
+        const int b = blk_of[A3[t]], s = blk_start[b], slot = afill[b]++;
+        aabc[slot] = (uint64_t)(A3[t] - s) | ((uint64_t)(B3[t] - s) << 16) | ((uint64_t)(C3[t] - s) << 32);
+        apar[slot] = make_double2(T0[t], K3[t]);
+        ang_local[t] = slot - acount[b];
+    }
+    int max_b = 0, max_a = 0, max_p = 0;
+    for (int b = 0; b < nblk; ++b) {
+        max_b = std::max(max_b, bcount[b + 1] - bcount[b]);
+        max_a = std::max(max_a, acount[b + 1] - acount[b]);
+        max_p = std::max(max_p, blk_start[b + 1] - blk_start[b]);
+    }
+    if (max_b > kBlock * kMaxTermsPerThread || max_a > kBlock * kMaxTermsPerThread) {
+        set_error("too many bonded terms per block for the fused kernel");
+        return HYMD_ERR_UNSUPPORTED;
+    }
+    // per-particle entries: bonds in term order, then angles in term order
+    std::vector<int> off(n + 1, 0);
+    for (int64_t t = 0; t < m2; ++t) {
+        off[A2[t] + 1]++;
+        off[B2[t] + 1]++;
+    }
+    for (int64_t t = 0; t < m3; ++t) {
+        off[A3[t] + 1]++;
+        off[B3[t] + 1]++;
+        off[C3[t] + 1]++;
+    }
+    std::partial_sum(off.begin(), off.end(), off.begin());
+    std::vector<uint32_t> ent(off[n]);
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (int64_t t = 0; t < m2; ++t) {
+        const uint32_t base = (uint32_t)bond_local[t] << 3;
+        ent[fill[A2[t]]++] = base | 0u;
+        ent[fill[B2[t]]++] = base | 1u;
+    }
+    for (int64_t t = 0; t < m3; ++t) {
+        const uint32_t base = ((uint32_t)ang_local[t] << 3) | 4u;
+        ent[fill[A3[t]]++] = base | 0u;
+        ent[fill[B3[t]]++] = base | 1u;
+        ent[fill[C3[t]]++] = base | 2u;
+    }
+    int max_e = 0;
+    for (int b = 0; b < nblk; ++b) max_e = std::max(max_e, off[blk_start[b + 1]] - off[blk_start[b]]);
+    hymd_fused *f = new hymd_fused();
+    f->n = n;
+    f->nblocks = nblk;
+    f->ppt = ppt;
+    f->max_bonds = std::max(max_b, 1);
+    f->max_angles = std::max(max_a, 1);
+    f->max_entries = std::max(max_e, 1);
+    f->max_particles = std::max(max_p, 1);
+    f->lds = sizeof(double) * (3 * f->max_particles + 3 * f->max_bonds + 6 * f->max_angles) +
+             sizeof(int) * (f->max_particles + 1) + sizeof(uint32_t) * f->max_entries;
+    int st = HYMD_OK;
+    if (f->lds > 64 * 1024) {
+        set_error("fused block needs %zu bytes of LDS", f->lds);
+        st = HYMD_ERR_UNSUPPORTED;
+    }
+    if (st == HYMD_OK) st = upload(f->blk_start, blk_start);
+    if (st == HYMD_OK) st = upload(f->blk_bond_off, bcount);
+    if (st == HYMD_OK) st = upload(f->bond_ab, bab);
+    if (st == HYMD_OK) st = upload(f->bond_par, bpar);
+    if (st == HYMD_OK) st = upload(f->blk_ang_off, acount);
+    if (st == HYMD_OK) st = upload(f->ang_abc, aabc);
+    if (st == HYMD_OK) st = upload(f->ang_par, apar);
+    if (st == HYMD_OK) st = upload(f->csr_off, off);
+    if (st == HYMD_OK) st = upload(f->csr_ent, ent);
+    if (st == HYMD_OK) st = f->partials.reserve(sizeof(double) * 8 * std::max(nblk, 1));
+    if (st == HYMD_OK) st = f->result.reserve(sizeof(double) * 8);
+    if (st == HYMD_OK) st = f->host.reserve(sizeof(double) * 8);
+    if (st != HYMD_OK) {
+        delete f;
+        return st;
+    }
+    *out = f;
+    return HYMD_OK;
+}
+
+extern "C" int hymd_fused_destroy(hymd_fused *f) {
+    delete f;
+    return HYMD_OK;
+}
+
+extern "C" int hymd_fused_inner(hymd_fused *f, double *x, double *v, float *x32, float *f_fast, const float *f_slow,
+                                double h_in, double h_out, double dt, int32_t n_inner, int32_t pending,
+                                const double box[3], double *energies, double *virials, void *stream) {
+    HYMD_REQUIRE(f && box && n_inner >= 1 && (f->n == 0 || (x && v && x32 && f_fast && f_slow)), "bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    FusedArgs A;
+    A.x = x;
+    A.v = v;
+    A.x32 = x32;
+    A.f_fast = f_fast;
+    A.f_slow = f_slow;
+    A.blk_start = (const int *)f->blk_start.ptr;
+    A.blk_bond_off = (const int *)f->blk_bond_off.ptr;
+    A.bond_ab = (const uint32_t *)f->bond_ab.ptr;
+    A.bond_par = (const double2 *)f->bond_par.ptr;
+    A.blk_ang_off = (const int *)f->blk_ang_off.ptr;
+    A.ang_abc = (const uint64_t *)f->ang_abc.ptr;
+    A.ang_par = (const double2 *)f->ang_par.ptr;
+    A.csr_off = (const int *)f->csr_off.ptr;
+    A.csr_ent = (const uint32_t *)f->csr_ent.ptr;
+    A.h_in = h_in;
+    A.h_out = h_out;
+    A.dt = dt;
+    for (int d = 0; d < 3; ++d) A.box[d] = box[d];
+    A.n_inner = n_inner;
+    A.pending = pending ? 1 : 0;
+    A.max_bonds = f->max_bonds;
+    A.max_angles = f->max_angles;
+    A.max_entries = f->max_entries;
+    A.max_particles = f->max_particles;
+    const bool red = energies || virials;
+    A.partials = red ? (double *)f->partials.ptr : nullptr;
+    if (f->nblocks > 0) fused_inner_kernel<2><<<f->nblocks, kBlock, f->lds, s>>>(A);
+    HYMD_HIP(hipGetLastError());
+    if (red) {
+        HYMD_TRY(reduce_partials((const double *)f->partials.ptr, f->nblocks, 8, (double *)f->result.ptr, s));
+        HYMD_HIP(hipMemcpyAsync(f->host.ptr, f->result.ptr, sizeof(double) * 8, hipMemcpyDeviceToHost, s));
+        HYMD_HIP(hipStreamSynchronize(s));
+        const double *h = (const double *)f->host.ptr;
+        if (energies) {
+            energies[0] = h[0];
+            energies[1] = h[4];
+        }
+        if (virials)
+            for (int d = 0; d < 3; ++d) {
+                virials[d] = h[1 + d];
+                virials[3 + d] = h[5 + d];
+            }
+    }
+    return HYMD_OK;
+}

@@ -29,6 +29,8 @@ struct hymd_md {
     double A[HYMD_MAX_TYPES * HYMD_MAX_TYPES], b[HYMD_MAX_TYPES], c0 = 0.0, mass_t[HYMD_MAX_TYPES];
     double sigma = 0.0, mass = 1.0, dt = 0.0, box[3] = {0, 0, 0};
     int respa_inner = 1;
+    hymd_fused *fused = nullptr;  // fused inner loop (null: per-kernel path)
+    int pending = 0;              // fused: outer half kick of the last step not yet applied
     // optional HIP-event timing of one stage in eager steps
     int timed = 0;  // 0 none, 1 bonded forces, 2 kick+drift, 3 update_field, 4 readout
     std::vector<hipEvent_t> ev;
@@ -39,11 +41,34 @@ static void mark(hymd_md *m, int stage) {
     if (m->timed == stage && m->ev_used < m->ev.size()) (void)hipEventRecord(m->ev[m->ev_used++], m->work);
 }
 
+extern "C" int hymd_md_set_fused(hymd_md *m, int32_t enable);
+
 // energies (host, 5): field, bond, angle, kinetic, (unused = 0)
-static int enqueue_step(hymd_md *m, double *energies) {
+static int enqueue_step(hymd_md *m, double *energies, int pending) {
     const double half_in = 0.5 * m->dt / m->mass;
     const double half_out = 0.5 * m->respa_inner * m->dt / m->mass;
     HYMD_TRY(hymd_pm_set_stream(m->pm, m->work));
+    if (m->fused) {
+        // fused: the step ends after the slow forces; its outer half kick is
+        // applied by the next step's kernel (or by finish_pending)
+        double eb[2];
+        mark(m, 1);
+        HYMD_TRY(hymd_fused_inner(m->fused, m->x, m->v, m->x32, m->f_fast, m->f_field, half_in, half_out, m->dt,
+                                  m->respa_inner, pending, m->box, energies ? eb : nullptr, nullptr, m->work));
+        mark(m, 1);
+        if (energies) {
+            energies[1] = eb[0];
+            energies[2] = eb[1];
+        }
+        mark(m, 3);
+        HYMD_TRY(hymd_update_field(m->pm, m->x32, m->types, m->n, m->T, m->A, m->b, m->c0, m->mass_t, m->sigma,
+                                   m->force_mesh, nullptr, nullptr, nullptr, energies ? &energies[0] : nullptr));
+        mark(m, 3);
+        mark(m, 4);
+        HYMD_TRY(hymd_compute_field_force(m->pm, m->x32, m->types, m->n, m->T, m->force_mesh, m->f_field));
+        mark(m, 4);
+        return HYMD_OK;
+    }
     for (int i = 0; i < m->respa_inner; ++i) {
         const bool first = (i == 0);
         mark(m, 2);
@@ -68,7 +93,6 @@ static int enqueue_step(hymd_md *m, double *energies) {
     HYMD_TRY(hymd_compute_field_force(m->pm, m->x32, m->types, m->n, m->T, m->force_mesh, m->f_field));
     mark(m, 4);
     HYMD_TRY(hymd_respa_kick(m->v, m->n, m->f_fast, half_in, m->f_field, half_out, m->work));
-    if (energies) HYMD_TRY(hymd_kinetic_energy(m->v, m->n, HYMD_F64, m->mass, &energies[3], m->work));
     return HYMD_OK;
 }
 
@@ -113,6 +137,13 @@ extern "C" int hymd_md_create(hymd_md **out, hymd_pm *pm, hymd_topology *topo, i
         delete m;
         return HYMD_ERR_HIP;
     }
+    // fuse the inner loop when the topology allows it; otherwise keep the
+    // per-kernel path (an unsupported topology is not an error)
+    const int st = hymd_md_set_fused(m, 1);
+    if (st == HYMD_ERR_HIP || st == HYMD_ERR_NOMEM) {
+        (void)hymd_md_destroy(m);
+        return st;
+    }
     *out = m;
     return HYMD_OK;
 }
@@ -121,6 +152,7 @@ extern "C" int hymd_md_destroy(hymd_md *m) {
     if (!m) return HYMD_OK;
     if (m->work) (void)hipStreamSynchronize(m->work);
     for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
+    if (m->fused) (void)hymd_fused_destroy(m->fused);
     if (m->exec) (void)hipGraphExecDestroy(m->exec);
     if (m->graph) (void)hipGraphDestroy(m->graph);
     if (m->ev_in) (void)hipEventDestroy(m->ev_in);
@@ -134,7 +166,7 @@ static int capture(hymd_md *m) {
     const int timed = m->timed;
     m->timed = 0;  // no event records inside the graph
     HYMD_HIP(hipStreamBeginCapture(m->work, hipStreamCaptureModeThreadLocal));
-    int st = enqueue_step(m, nullptr);
+    int st = enqueue_step(m, nullptr, 1);
     m->timed = timed;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(m->work, &g);
@@ -155,27 +187,73 @@ static int capture(hymd_md *m) {
 // and makes `stream` wait for them.  use_graph != 0 replays a captured graph of
 // one step.  energies (host double[5]) or NULL: energies after the last step
 // (that step runs eagerly).
+// fused path: apply the outer half kick left pending by the last step
+static int finish_pending(hymd_md *m) {
+    if (!m->fused || !m->pending) return HYMD_OK;
+    const double half_out = 0.5 * m->respa_inner * m->dt / m->mass;
+    HYMD_TRY(hymd_respa_kick(m->v, m->n, m->f_field, half_out, nullptr, 0.0, m->work));
+    m->pending = 0;
+    return HYMD_OK;
+}
+
+// Runs n_steps outer steps ordered after the work already queued on `stream`,
+// and makes `stream` wait for them.  use_graph != 0 replays a captured graph of
+// one step.  energies (host double[5]) or NULL: energies after the last step
+// (that step runs eagerly).  Velocities are complete when the call returns.
 extern "C" int hymd_md_step(hymd_md *m, int32_t n_steps, int32_t use_graph, double *energies, void *stream) {
     HYMD_REQUIRE(m && n_steps >= 0, "bad argument");
     hipStream_t caller = (hipStream_t)stream;
     HYMD_HIP(hipEventRecord(m->ev_in, caller));
     HYMD_HIP(hipStreamWaitEvent(m->work, m->ev_in, 0));
     const int plain = energies ? n_steps - 1 : n_steps;
-    if (use_graph && plain > 0 && !m->exec) HYMD_TRY(capture(m));
     for (int s = 0; s < plain; ++s) {
-        if (use_graph)
+        // the captured step assumes a pending outer kick (fused) and is the
+        // plain per-kernel step otherwise
+        const bool graphable = use_graph && (!m->fused || m->pending);
+        if (graphable) {
+            if (!m->exec) HYMD_TRY(capture(m));
             HYMD_HIP(hipGraphLaunch(m->exec, m->work));
-        else
-            HYMD_TRY(enqueue_step(m, nullptr));
+        } else {
+            HYMD_TRY(enqueue_step(m, nullptr, m->pending));
+        }
+        if (m->fused) m->pending = 1;
     }
     if (energies && n_steps > 0) {
         std::memset(energies, 0, 5 * sizeof(double));
-        HYMD_TRY(enqueue_step(m, energies));
+        HYMD_TRY(enqueue_step(m, energies, m->pending));
+        if (m->fused) m->pending = 1;
+        HYMD_TRY(finish_pending(m));
+        HYMD_TRY(hymd_kinetic_energy(m->v, m->n, HYMD_F64, m->mass, &energies[3], m->work));
     }
+    HYMD_TRY(finish_pending(m));
     HYMD_HIP(hipEventRecord(m->ev_out, m->work));
     HYMD_HIP(hipStreamWaitEvent(caller, m->ev_out, 0));
     return HYMD_OK;
 }
+
+extern "C" int hymd_md_set_fused(hymd_md *m, int32_t enable) {
+    HYMD_REQUIRE(m, "null handle");
+    HYMD_HIP(hipStreamSynchronize(m->work));
+    if (m->exec) {
+        (void)hipGraphExecDestroy(m->exec);
+        m->exec = nullptr;
+    }
+    if (m->graph) {
+        (void)hipGraphDestroy(m->graph);
+        m->graph = nullptr;
+    }
+    if (!enable) {
+        if (m->fused) (void)hymd_fused_destroy(m->fused);
+        m->fused = nullptr;
+        return HYMD_OK;
+    }
+    if (m->fused) return HYMD_OK;
+    const TopoTerms t = topology_terms(m->topo);
+    HYMD_REQUIRE(t.n == m->n, "topology and system sizes differ");
+    return hymd_fused_create(&m->fused, t.n, t.m2, t.a2, t.b2, t.r0, t.k2, t.m3, t.a3, t.b3, t.c3, t.t0, t.k3);
+}
+
+extern "C" int hymd_md_is_fused(hymd_md *m) { return (m && m->fused) ? 1 : 0; }
 
 // Time one stage of the eager steps that follow with HIP events recorded on
 // the launch stream around every launch of that stage (graph replays are not

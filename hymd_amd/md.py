@@ -69,6 +69,16 @@ class RespaMD:
             self._native = h
         return self._native
 
+    @property
+    def fused(self):
+        """True when the library runs the molecule-blocked fused inner loop."""
+        return bool(_lib.lib().hymd_md_is_fused(self._native_handle()))
+
+    def set_fused(self, enable):
+        """Select the fused (True) or per-kernel (False) inner loop; raises
+        HymdError if the topology cannot be fused."""
+        call("hymd_md_set_fused", self._native_handle(), 1 if enable else 0)
+
     def run(self, n_steps, use_graph=True, energies=False):
         """``n_steps`` outer steps enqueued by the library (one ctypes call;
         with ``use_graph`` one captured step is replayed).  With ``energies``

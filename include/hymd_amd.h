@@ -157,6 +157,25 @@ int hymd_topology_destroy(hymd_topology *top);
 int hymd_topology_forces(hymd_topology *top, const double *r, float *f, const double box[3], double *energies,
                          double *virials, void *stream);
 
+/* Molecule-blocked fused inner loop: all respa_inner sub-steps (kicks, drift,
+ * wrap, bonds + angles) of main.py:830-887 in one launch, one workgroup per
+ * block of whole molecules.  Creation fails with HYMD_ERR_UNSUPPORTED when a
+ * molecule is not a contiguous index range or does not fit a workgroup.
+ * The term arrays are copied at creation. */
+typedef struct hymd_fused hymd_fused;
+int hymd_fused_create(hymd_fused **fused, int64_t n, int64_t m2, const int32_t *a2, const int32_t *b2,
+                      const double *r0, const double *k2, int64_t m3, const int32_t *a3, const int32_t *b3,
+                      const int32_t *c3, const double *t0, const double *k3);
+int hymd_fused_destroy(hymd_fused *fused);
+/* v += [pending ? h_out f_slow : 0] + h_out f_slow; then n_inner times
+ * { v += h_in f_fast; x += dt v; wrap; f_fast = bonds + angles; v += h_in f_fast }
+ * (the second half kick of one sub-step and the first of the next are separate
+ * additions); x32 = float(x).  energies host double[2], virials host double[6]
+ * of the last sub-step, or NULL. */
+int hymd_fused_inner(hymd_fused *fused, double *x, double *v, float *x32, float *f_fast, const float *f_slow,
+                     double h_in, double h_out, double dt, int32_t n_inner, int32_t pending, const double box[3],
+                     double *energies, double *virials, void *stream);
+
 /* ------------------------------------------------------------------------ */
 /* rRESPA integrator (FP64 x, v): main.py:802-887,1143-1169.                 */
 /* ------------------------------------------------------------------------ */
@@ -185,6 +204,11 @@ int hymd_md_create(hymd_md **md, hymd_pm *pm, hymd_topology *topo, int64_t n, in
  * energies host double[5] (field, bond, angle, kinetic, 0) after the last step, or NULL */
 int hymd_md_step(hymd_md *md, int32_t n_steps, int32_t use_graph, double *energies, void *stream);
 int hymd_md_destroy(hymd_md *md);
+/* use the fused inner loop (1) or the per-kernel inner loop (0); returns
+ * HYMD_ERR_UNSUPPORTED if the topology does not allow fusing */
+int hymd_md_set_fused(hymd_md *md, int32_t enable);
+/* 1 if the fused inner loop is active */
+int hymd_md_is_fused(hymd_md *md);
 /* HIP-event timing of one stage (1 bonded forces, 2 kick+drift, 3 update_field,
  * 4 readout; 0 off) in the eager steps that follow, recorded on the launch
  * stream around each launch; at most max_launches are recorded */
