@@ -218,6 +218,58 @@ __global__ void __launch_bounds__(kBlock) field_kspace_kernel(float2 *__restrict
     if (partials) block_reduce_store<1>(e_acc, partials + blockIdx.x);
 }
 
+// Spectral moments of the filtered densities phi~_t = H rho_t / G (pmesh
+// normalisation), summed over the full spectrum through the half spectrum
+// weights: P_ts = sum_k w Re(phi~_t^* phi~_s), Q_ts = sum_k w |k|^2 Re(...),
+// t <= s packed row-major, then phi~_t(0).  Every quadratic Hamiltonian's
+// energy and its derivatives in chi, kappa and sigma follow from them.
+template <int MAXT>
+__global__ void __launch_bounds__(kBlock) field_moments_kernel(const float2 *__restrict__ spec, KGeom g, int T,
+                                                                double sigma2, double inv_g,
+                                                                double *__restrict__ partials) {
+    constexpr int NP = MAXT * (MAXT + 1) / 2;
+    double acc[2 * NP + MAXT];
+#pragma unroll
+    for (int j = 0; j < 2 * NP + MAXT; ++j) acc[j] = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; kk < g.gc; kk += stride) {
+        double k[3], kd[3], wgt;
+        kvec(g, kk, k, kd, wgt);
+        const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        const double hn = exp(-0.5 * sigma2 * k2) * inv_g;
+        double re[MAXT], im[MAXT];
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t) {
+            const float2 v = (t < T) ? spec[(int64_t)t * g.gc + kk] : make_float2(0.f, 0.f);
+            re[t] = hn * (double)v.x;
+            im[t] = hn * (double)v.y;
+        }
+        int q = 0;
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t)
+#pragma unroll
+            for (int s = t; s < MAXT; ++s, ++q) {
+                const double p = wgt * (re[t] * re[s] + im[t] * im[s]);
+                acc[q] += p;
+                acc[NP + q] += k2 * p;
+            }
+        if (kk == 0)
+#pragma unroll
+            for (int t = 0; t < MAXT; ++t) acc[2 * NP + t] += re[t];
+    }
+    // compact to the caller's T and reduce
+    double out[2 * NP + MAXT];
+    int o = 0;
+    for (int t = 0; t < T; ++t)
+        for (int s = t; s < T; ++s) out[o++] = acc[t * MAXT - t * (t - 1) / 2 + (s - t)];
+    for (int t = 0; t < T; ++t)
+        for (int s = t; s < T; ++s) out[o++] = acc[NP + t * MAXT - t * (t - 1) / 2 + (s - t)];
+    for (int t = 0; t < T; ++t) out[o++] = acc[2 * NP + t];
+    for (int j = o; j < 2 * NP + MAXT; ++j) out[j] = 0.0;
+    // one row of 2 NP + MAXT doubles per block (the first 2 T(T+1)/2 + T are used)
+    block_reduce_store<2 * NP + MAXT>(out, partials + (int64_t)blockIdx.x * (2 * NP + MAXT));
+}
+
 struct PmeK {
     double sigma2;
     double inv_g;
@@ -544,6 +596,56 @@ extern "C" int hymd_update_field(hymd_pm *pm, const float *x, const int32_t *typ
         HYMD_TRY(read_result(pm, kb, 1, &e));
         *energy = e * (m.box[0] * m.box[1] * m.box[2]);
     }
+    return HYMD_OK;
+}
+
+// paint + r2c + moments (and, if force_mesh != NULL, the force grids of the
+// quadratic Hamiltonian (coupling, shift) as in hymd_update_field)
+extern "C" int hymd_field_moments(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
+                                  const double *mass, double sigma, double *moments, const double *coupling,
+                                  const double *shift, float *force_mesh) {
+    HYMD_REQUIRE(pm && moments && (x || n == 0) && (types || n == 0) && n >= 0, "bad argument");
+    HYMD_REQUIRE(n_types >= 1 && n_types <= 8 && n_types <= pm->max_types, "moments support n_types <= 8");
+    HYMD_REQUIRE(!force_mesh || coupling, "force grids need the coupling matrix");
+    const Mesh &m = pm->mesh;
+    const int T = n_types;
+    hipStream_t s = pm->stream;
+    unsigned long long *grid = (unsigned long long *)pm->paint.ptr;
+    float2 *spec = (float2 *)pm->spec.ptr;
+    HYMD_TRY(launch_paint(pm, x, types, nullptr, n, T, mass, grid));
+    const double scale = std::ldexp(1.0, -HYMD_FIXED_SHIFT) / cell_volume(pm);
+    HYMD_TRY(fft_r2c_fixed(pm->fft, (int64_t *)grid, spec, T, scale, true, s));
+    KGeom g = kgeom(pm);
+    const int np = T * (T + 1) / 2;
+    const int nv = 2 * np + T;
+    const int row = (T <= 4) ? 2 * 10 + 4 : 2 * 36 + 8;  // 2 NP + MAXT of the instance
+    const int nb = std::min(pm->kblocks, 1024);
+    DevBuf part;
+    HYMD_TRY(part.reserve(sizeof(double) * (size_t)row * nb));
+    HYMD_TRY(pm->result.reserve(sizeof(double) * row));
+    HYMD_TRY(pm->host.reserve(sizeof(double) * row));
+    if (T <= 4)
+        field_moments_kernel<4><<<nb, kBlock, 0, s>>>(spec, g, T, sigma * sigma, 1.0 / (double)m.g,
+                                                      (double *)part.ptr);
+    else
+        field_moments_kernel<8><<<nb, kBlock, 0, s>>>(spec, g, T, sigma * sigma, 1.0 / (double)m.g,
+                                                      (double *)part.ptr);
+    HYMD_HIP(hipGetLastError());
+    HYMD_TRY(reduce_partials((const double *)part.ptr, nb, row, (double *)pm->result.ptr, s));
+    HYMD_HIP(hipMemcpyAsync(pm->host.ptr, pm->result.ptr, sizeof(double) * row, hipMemcpyDeviceToHost, s));
+    if (force_mesh) {
+        FieldK p = field_params(T, coupling, shift, 0.0, sigma, m);
+        float2 *grad = (float2 *)pm->grad.ptr;
+        if (T <= 4)
+            field_kspace_kernel<4><<<pm->kblocks, kBlock, 0, s>>>(spec, grad, nullptr, 0, g, p, nullptr);
+        else
+            field_kspace_kernel<8><<<pm->kblocks, kBlock, 0, s>>>(spec, grad, nullptr, 0, g, p, nullptr);
+        HYMD_HIP(hipGetLastError());
+        HYMD_TRY(fft_inverse_xy(pm->fft, grad, 3 * T, s));
+        HYMD_TRY(fft_c2r_z(pm->fft, grad, force_mesh, 3 * T, 1.f, s));
+    }
+    HYMD_HIP(hipStreamSynchronize(s));
+    for (int j = 0; j < nv; ++j) moments[j] = ((double *)pm->host.ptr)[j];
     return HYMD_OK;
 }
 

@@ -14,7 +14,21 @@ mass motion removed.  numpy PCG64 streams, seed = 1000 + workload index.
                      r0 = 0.47, k = 1250; angles k = 25 with theta0 = 120 deg at
                      an L vertex and 180 deg otherwise; chi 4x4 symmetric
                      U[0,30]; kappa = 0.03, sigma = 0.5, dt = 0.03, inner 10.
-Both accept smaller chain / solvent counts for fast parity cases.
+* ``peptide_solvent`` 2,097,152 beads, 50 nm box, 160^3 mesh (shared by the
+                     density and PME), 4 types BB/SC1/SC2/W: 8,738 chains of
+                     40 residues x 3 beads (backbone + 2 side beads) = 50 %
+                     peptide, the rest free solvent.  Choices the workload
+                     leaves open, fixed here: backbone bond angles drawn in
+                     [100, 130] deg; bonds BB-BB 0.38 nm, BB-SC1 and SC1-SC2
+                     0.30 nm (k = 1250); angles BB-BB-SC1 100 deg and
+                     BB-SC1-SC2 180 deg (k = 25); every backbone dihedral is a
+                     combined bending-torsion term (type 1) with cosine
+                     coefficients n = 1..4 ~ U[-5,5] kJ/mol, K(phi) = 60 kJ/mol
+                     and the chain's last dihedral flagged; 10 % of the beads
+                     (side beads and solvent) carry +-1 with zero net charge;
+                     dielectric 45; chi 4x4 symmetric U[0,30]; kappa 0.03;
+                     sigma 0.5; dt 0.03; respa_inner 1.
+All accept smaller chain / solvent counts for fast parity cases.
 """
 from dataclasses import dataclass, field
 from typing import Optional
@@ -115,6 +129,77 @@ def lipid_water(n_chains=52428, n_water=419440, box=40.0, mesh=128, seed=1001):
     v = maxwell_velocities(rng, len(types), 72.0)
     return System("lipid_water", box, (mesh,) * 3, 4, ["H", "L", "T", "W"], types, x, v, bonds, angles, chi, 0.03,
                   0.5, 0.03, 10, 1000)
+
+
+def _bent_backbone(rng, n_chains, n_res, box, step=0.38, angle_range=(100.0, 130.0)):
+    """Backbone random walks whose bond angles lie in ``angle_range`` (deg)."""
+    x = np.zeros((n_chains, n_res, 3))
+    x[:, 0] = rng.uniform(0.0, 1.0, (n_chains, 3)) * box
+    d = rng.normal(size=(n_chains, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    x[:, 1] = x[:, 0] + step * d
+    for i in range(2, n_res):
+        gamma = np.radians(rng.uniform(*angle_range, n_chains))
+        alpha = np.pi - gamma                       # turn away from the previous direction
+        u = rng.normal(size=(n_chains, 3))
+        u -= np.sum(u * d, axis=1, keepdims=True) * d
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        d = np.cos(alpha)[:, None] * d + np.sin(alpha)[:, None] * u
+        x[:, i] = x[:, i - 1] + step * d
+    return x
+
+
+def peptide_solvent(n_chains=8738, n_res=40, n_solvent=1048592, box=50.0, mesh=160, seed=1002):
+    rng = np.random.default_rng(seed)
+    box = np.full(3, float(box))
+    bb = _bent_backbone(rng, n_chains, n_res, box)
+    side_dir = rng.normal(size=(n_chains, n_res, 3))
+    side_dir /= np.linalg.norm(side_dir, axis=2, keepdims=True)
+    sc1 = bb + 0.30 * side_dir
+    sc2 = sc1 + 0.30 * side_dir
+    chain = np.stack([bb, sc1, sc2], axis=2).reshape(n_chains, 3 * n_res, 3)
+    x = np.concatenate([np.mod(chain.reshape(-1, 3), box), rng.uniform(0.0, 1.0, (n_solvent, 3)) * box])
+    per_chain = 3 * n_res
+    types = np.concatenate([np.tile(np.array([0, 1, 2], dtype=np.int32), n_chains * n_res),
+                            np.full(n_solvent, 3, dtype=np.int32)])
+    base = per_chain * np.arange(n_chains)[:, None]
+    r = np.arange(n_res)[None, :]
+    bbi = base + 3 * r                                  # backbone bead of residue r
+    ba = np.concatenate([bbi[:, :-1].ravel(), bbi.ravel(), (bbi + 1).ravel()])
+    bb_ = np.concatenate([bbi[:, 1:].ravel(), (bbi + 1).ravel(), (bbi + 2).ravel()])
+    r0 = np.concatenate([np.full(n_chains * (n_res - 1), 0.38), np.full(2 * n_chains * n_res, 0.30)])
+    bonds = (ba, bb_, r0, np.full(len(ba), 1250.0))
+    # angles BB(i-1)-BB(i)-SC1(i) and BB(i)-SC1(i)-SC2(i)
+    aa = np.concatenate([bbi[:, :-1].ravel(), bbi.ravel()])
+    ab = np.concatenate([bbi[:, 1:].ravel(), (bbi + 1).ravel()])
+    ac = np.concatenate([(bbi[:, 1:] + 1).ravel(), (bbi + 2).ravel()])
+    t0 = np.concatenate([np.full(n_chains * (n_res - 1), np.radians(100.0)), np.full(n_chains * n_res, np.pi)])
+    angles = (aa, ab, ac, t0, np.full(len(aa), 25.0))
+    # backbone dihedrals, combined bending-torsion (type 1)
+    da = bbi[:, :-3].ravel()
+    m4 = len(da)
+    coeff = np.zeros((m4, 6, 5), dtype=np.float32)
+    coeff[:, 0, 1:] = rng.uniform(-5.0, 5.0, (m4, 4))
+    coeff[:, 4, 0] = 30.0
+    last = np.zeros((n_chains, n_res - 3), dtype=np.int32)
+    last[:, -1] = 1
+    dihedrals = (da, da + 3, da + 6, da + 9, coeff, np.ones(m4, dtype=np.int32), last.ravel())
+    # +-1 charges on 10 % of the beads, chosen among side beads and solvent
+    n = len(types)
+    n_q = (n // 10) // 2 * 2
+    candidates = np.flatnonzero(types != 0)
+    pick = rng.choice(candidates, n_q, replace=False)
+    charges = np.zeros(n, dtype=np.float32)
+    charges[pick[: n_q // 2]] = 1.0
+    charges[pick[n_q // 2:]] = -1.0
+    c = rng.uniform(0.0, 30.0, (4, 4))
+    chi = np.triu(c, 1)
+    chi = chi + chi.T
+    v = maxwell_velocities(rng, n, 72.0)
+    s = System("peptide_solvent", box, (mesh,) * 3, 4, ["BB", "SC1", "SC2", "W"], types, x, v, bonds, angles, chi,
+               0.03, 0.5, 0.03, 1, 500, charges=charges, dielectric=45.0)
+    s.extra["dihedrals"] = dihedrals
+    return s
 
 
 def hamiltonian_form(system):

@@ -100,6 +100,18 @@ int hymd_field_energy(hymd_pm *pm, const float *phi, int32_t n_types, const doub
                       double constant, double *energy);
 int hymd_field_q_energy(hymd_pm *pm, const float *phi_q, const float *psi, double self_energy, double *energy);
 
+/* Spectral moments of the filtered densities phi~_t (n_types <= 8), for the
+ * energy and its chi / kappa / sigma derivatives (no reference counterpart:
+ * the differentiable extension of update_field):
+ *   moments = [P_ts (t<=s, row-major), Q_ts (t<=s), phi~_t(k=0)],
+ *   P_ts = sum_k Re(phi~_t^* phi~_s), Q_ts = sum_k |k|^2 Re(phi~_t^* phi~_s)
+ * over the full spectrum (so dV sum_cells phi~_t phi~_s = V P_ts).  With
+ * force_mesh != NULL the force grids of (coupling, shift) are written too.
+ * Synchronises the stream. */
+int hymd_field_moments(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
+                       const double *mass, double sigma, double *moments, const double *coupling,
+                       const double *shift, float *force_mesh);
+
 /* compute_self_energy_q (field.py:203-238): host double out */
 int hymd_self_energy_q(const float *charges, int64_t n, double sigma, double elec_factor, double *out,
                        void *stream);

@@ -120,6 +120,7 @@ def test_native_step_equals_python_step(hip, use_graph):
     s = synth.diblock_melt(n_chains=400, box=7.0, mesh=32, seed=1000)
     a = RespaMD(s)
     b = RespaMD(s)
+    b.set_fused(False)  # the per-kernel inner loop, as the Python step runs it
     for _ in range(3):
         a.step()
     b.run(3, use_graph=use_graph)
@@ -145,6 +146,12 @@ def test_stage_timing(hip):
     from hymd_amd.md import RespaMD
     s = synth.diblock_melt(n_chains=200, box=6.0, mesh=32, seed=1000)
     md = RespaMD(s)
+    assert md.fused
+    md.time_stage("bonded")  # the fused kernel: one launch per outer step
+    md.run(2, use_graph=False)
+    ms, n = md.stage_time()
+    assert n == 2 and ms > 0.0
+    md.set_fused(False)  # per-kernel path: one bonded launch per inner step
     md.time_stage("bonded")
     md.run(2, use_graph=False)
     ms, n = md.stage_time()
@@ -167,15 +174,24 @@ def test_fused_inner_loop_matches_per_kernel_path(hip, make):
     b = RespaMD(s)
     a.set_fused(False)
     assert b.fused and not a.fused
+    # one outer step: the same additions on the same per-term FP64 values up
+    # to the compiler's contraction choices in two different kernels
+    a.run(1, use_graph=False)
+    b.run(1, use_graph=False)
+    torch.cuda.synchronize()
+    assert (a.x - b.x).abs().max().item() < 1e-12
+    assert (a.v - b.v).abs().max().item() < 1e-11
+    assert rel_err(b.f_fast.cpu().numpy(), a.f_fast.cpu().numpy()) < 1e-6
     for use_graph in (False, True):
         a.run(3, use_graph=use_graph)
         b.run(3, use_graph=use_graph)
     ea = a.run(1, energies=True)
     eb = b.run(1, energies=True)
     torch.cuda.synchronize()
+    # 8 outer steps later the last-bit differences have grown with the dynamics
     dx = (a.x - b.x).abs().max().item()
     dv = (a.v - b.v).abs().max().item()
-    assert dx < 1e-12 and dv < 1e-10, (dx, dv)
+    assert dx < 1e-8 and dv < 1e-7, (dx, dv)
     for k in ("field", "bond", "angle", "kinetic"):
         assert ea[k] == pytest.approx(eb[k], rel=1e-10)
     a.close()
