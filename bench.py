@@ -27,22 +27,31 @@ if ROOT not in sys.path:
 
 METRIC = "particle-steps/s, 1M beads/128³ mesh (fp32 vs fp64 ref) + batched χ-grads/s"
 HBM_PEAK_GBS = 8000.0
-KERNEL_NAMES = {"bonded": "topo_gather_kernel", "kick_drift": "kick_drift_kernel",
-                "field": "update_field (paint + FFT + k-space kernels)", "readout": "readout_kernel"}
+KERNEL_NAMES = {"bonded": "topo_gather_kernel", "fused": "fused_inner_kernel", "kick_drift": "kick_drift_kernel",
+                "field": "update_field (sort + paint + FFT + k-space kernels)", "readout": "readout_kernel"}
 
 
 def algorithmic_bytes(kernel, n, n_bonds, n_angles, T, mesh):
-    """Compulsory HBM bytes of one launch (measurement)."""
+    """Compulsory HBM bytes of one launch of a stage."""
     G = mesh[0] * mesh[1] * mesh[2]
+    if kernel == "fused":       # whole inner loop: x,v f64 read+write, F_slow f32 in, F_fast f32 in+out,
+        # x32 out, per-particle term list (4 B offset + 4 B per entry: 2 per bond, 3 per angle),
+        # bond (a|b, r0, k) 20 B, angle (a|b|c, theta0, k) 24 B
+        return 148 * n + 28 * n_bonds + 36 * n_angles
     if kernel == "bonded":      # x f64 in, F f32 out, bond a,b,r0,k and angle a,b,c,t0,k
         return 36 * n + 24 * n_bonds + 28 * n_angles
     if kernel == "kick_drift":  # v,x f64 read+write, F f32 in, x f32 out
         return 120 * n
-    if kernel == "field":       # paint + r2c + k-pass + c2r of one update
+    if kernel == "field":       # one update_field: cell sort, gather paint, r2c, k-space pass, c2r of 3T grids
         Gc = mesh[0] * mesh[1] * (mesh[2] // 2 + 1)
-        return (16 * n + 8 * T * G) + T * (4 * G + 8 * Gc) + 16 * T * Gc + 3 * T * (8 * Gc + 4 * G)
-    if kernel == "readout":     # x f32 + type in, 3 grids per type once, F f32 out
-        return 28 * n + 12 * T * G
+        sort = (16 * n) + (16 * G) + (40 * n)       # keys; scan of the cell counts; scatter x, type, index
+        paint = 16 * n + 4 * G + 4 * T * G          # sorted x + type, cell starts, float densities out
+        fwd = T * (4 * G + 8 * Gc) + 2 * 16 * T * Gc  # z r2c + y, x passes (read + write)
+        kspace = 8 * T * Gc + 24 * T * Gc
+        inv = 2 * 16 * 3 * T * Gc + 3 * T * (8 * Gc + 4 * G)
+        return sort + paint + fwd + kspace + inv
+    if kernel == "readout":     # sorted x f32 + type + row index in, 3 grids per type once, F f32 out
+        return 32 * n + 12 * T * G
     raise ValueError(kernel)
 
 
@@ -77,7 +86,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--roofline-kernel", default="bonded", choices=["bonded", "kick_drift", "field", "readout"])
+    ap.add_argument("--roofline-kernel", default="bonded", choices=["bonded", "kick_drift", "field", "readout"],
+                    help="stage to time; 'bonded' is the fused inner loop when the topology allows it")
     ap.add_argument("--cpu-steps", type=int, default=4, help="oracle outer steps for cpu_baseline (0 = skip)")
     ap.add_argument("--small", action="store_true", help="reduced system for quick checks (not a bench line)")
     ap.add_argument("--no-graph", action="store_true", help="launch each step's kernels eagerly (no hipGraph)")
@@ -125,11 +135,12 @@ def main():
     eager_ms = 1e3 * (time.perf_counter() - t0) / rsteps
     dur_ms, launches = md.stage_time()
     md.time_stage(None)
-    nbytes = algorithmic_bytes(args.roofline_kernel, s.n, len(s.bonds[0]), len(s.angles[0]), s.n_types, s.mesh)
+    kname = "fused" if (args.roofline_kernel == "bonded" and md.fused) else args.roofline_kernel
+    nbytes = algorithmic_bytes(kname, s.n, len(s.bonds[0]), len(s.angles[0]), s.n_types, s.mesh)
     achieved = nbytes / (dur_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": KERNEL_NAMES[args.roofline_kernel], "avg_launch_us": round(dur_ms * 1e3, 2),
+                "kernel": KERNEL_NAMES[kname], "avg_launch_us": round(dur_ms * 1e3, 2),
                 "launches_timed": launches, "algorithmic_bytes": int(nbytes),
                 "eager_ms_per_step": round(eager_ms, 4)}
 

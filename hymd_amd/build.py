@@ -16,7 +16,7 @@ OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhymd_amd.so")
 ARCH = os.environ.get("HYMD_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["common.hip", "fft.hip", "field.hip", "bonded.hip", "integrator.hip", "fused.hip", "md.hip"]
+SOURCES = ["common.hip", "scan.hip", "fft.hip", "field.hip", "bonded.hip", "integrator.hip", "fused.hip", "md.hip"]
 HEADERS = ["common.h", "fft.h", "bonded.h"]
 
 

@@ -61,68 +61,6 @@ __global__ void csr_sort_kernel(const int *__restrict__ off, int64_t n, uint32_t
     }
 }
 
-// exclusive scan, three phases (block totals, scan of totals, local scans)
-constexpr int kScanItems = 4;
-constexpr int kScanTile = kBlock * kScanItems;
-
-__device__ int block_exclusive_scan(int v, int *sh, int &total) {
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
-        int add = (int)threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
-        __syncthreads();
-        sh[threadIdx.x] += add;
-        __syncthreads();
-    }
-    total = sh[blockDim.x - 1];
-    const int incl = sh[threadIdx.x];
-    __syncthreads();
-    return incl - v;
-}
-
-__global__ void scan_tiles_kernel(const int *__restrict__ in, int *__restrict__ out, int64_t n,
-                                  int *__restrict__ tile_sums) {
-    __shared__ int sh[kBlock];
-    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
-    int v[kScanItems], s = 0;
-    for (int j = 0; j < kScanItems; ++j) {
-        v[j] = (base + j < n) ? in[base + j] : 0;
-        s += v[j];
-    }
-    int total;
-    int run = block_exclusive_scan(s, sh, total);
-    for (int j = 0; j < kScanItems; ++j) {
-        if (base + j < n) out[base + j] = run;
-        run += v[j];
-    }
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
-}
-
-__global__ void scan_sums_kernel(int *__restrict__ sums, int ntiles, int *__restrict__ grand_total) {
-    __shared__ int sh[kBlock];
-    const int per = (ntiles + blockDim.x - 1) / blockDim.x;
-    const int b = threadIdx.x * per;
-    int s = 0;
-    for (int j = 0; j < per; ++j)
-        if (b + j < ntiles) s += sums[b + j];
-    int total;
-    int run = block_exclusive_scan(s, sh, total);
-    for (int j = 0; j < per; ++j)
-        if (b + j < ntiles) {
-            const int v = sums[b + j];
-            sums[b + j] = run;
-            run += v;
-        }
-    if (threadIdx.x == 0) *grand_total = total;
-}
-
-__global__ void scan_add_kernel(int *__restrict__ out, int64_t n, const int *__restrict__ sums,
-                                const int *__restrict__ grand_total) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] += sums[i / kScanTile];
-    if (i == n) out[n] = *grand_total;
-}
-
 struct Csr {
     int64_t n = 0, m = 0;
     DevBuf counts;   // n ints (reused as cursor)
@@ -138,20 +76,17 @@ static int build_csr(Csr &c, const TermIdx &ti, int64_t m, int64_t n, hipStream_
     HYMD_REQUIRE(m * ti.nroles < (int64_t(1) << 31), "too many term members");
     c.n = n;
     c.m = m;
-    const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
     HYMD_TRY(c.counts.reserve(sizeof(int) * std::max<int64_t>(n, 1)));
     HYMD_TRY(c.offsets.reserve(sizeof(int) * (n + 1)));
     HYMD_TRY(c.entries.reserve(sizeof(uint32_t) * std::max<int64_t>(m * ti.nroles, 1)));
-    HYMD_TRY(c.tiles.reserve(sizeof(int) * (ntiles + 2)));
+    HYMD_TRY(c.tiles.reserve(sizeof(int) * (scan_tiles(n) + 2)));
     int *counts = (int *)c.counts.ptr, *off = (int *)c.offsets.ptr, *tiles = (int *)c.tiles.ptr;
-    int *grand = tiles + ntiles, *err = tiles + ntiles + 1;
+    int *err = tiles + scan_tiles(n) + 1;
     HYMD_HIP(hipMemsetAsync(counts, 0, sizeof(int) * std::max<int64_t>(n, 1), s));
     HYMD_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
     if (m > 0) csr_count_kernel<<<grid_for(m), kBlock, 0, s>>>(ti, m, n, counts, err);
-    if (n > 0) scan_tiles_kernel<<<(unsigned)ntiles, kBlock, 0, s>>>(counts, off, n, tiles);
-    scan_sums_kernel<<<1, kBlock, 0, s>>>(tiles, (int)ntiles, grand);
-    scan_add_kernel<<<grid_for(n + 1), kBlock, 0, s>>>(off, n, tiles, grand);
     HYMD_HIP(hipGetLastError());
+    HYMD_TRY(exclusive_scan(counts, off, n, tiles, nullptr, false, s));
     if (n > 0) HYMD_HIP(hipMemcpyAsync(counts, off, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
     if (m > 0) csr_fill_kernel<<<grid_for(m), kBlock, 0, s>>>(ti, m, n, counts, (uint32_t *)c.entries.ptr);
     if (n > 0) csr_sort_kernel<<<grid_for(n), kBlock, 0, s>>>(off, n, (uint32_t *)c.entries.ptr);

@@ -114,6 +114,12 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *out_
 // out[NV] with a single block; launch with kBlock threads.
 int reduce_partials(const double *partials, int nblocks, int nv, double *out, hipStream_t s);
 
+// Exclusive scan of n int32 counts into out[0..n] (out[n] = total); tiles is
+// scratch of scan_tiles(n) + 1 ints; cursor (optional) receives a copy of
+// out[0..n-1]; zero_input clears the counts after reading them.
+int64_t scan_tiles(int64_t n);
+int exclusive_scan(int *in, int *out, int64_t n, int *tiles, int *cursor, bool zero_input, hipStream_t s);
+
 // Scratch growing buffer owned by the library (per handle or per thread).
 struct DevBuf {
     void *ptr = nullptr;
@@ -130,6 +136,10 @@ struct TopoTerms {
     const double *r0, *k2, *t0, *k3;
 };
 TopoTerms topology_terms(const hymd_topology *t);
+
+// compute_field_force at the particle set sorted by the last update_field on
+// this handle (same positions and types)
+int field_readout_last(hymd_pm *pm, int n_types, const float *force_mesh, float *force);
 
 // Pinned host staging for small results.
 struct HostBuf {

@@ -80,20 +80,123 @@ __global__ void zero_kernel(unsigned long long *__restrict__ p, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
+// Spatial sort: particles binned by their CIC base cell (counting sort), then
+// paint as a gather over cells and readout in cell order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int base_cell(const float *__restrict__ x, int64_t i, const Mesh &m) {
+    const CicAxis cx = cic_axis((double)x[3 * i + 0], m.inv_h[0], m.mx);
+    const CicAxis cy = cic_axis((double)x[3 * i + 1], m.inv_h[1], m.my);
+    const CicAxis cz = cic_axis((double)x[3 * i + 2], m.inv_h[2], m.mz);
+    return (cx.i0 * m.my + cy.i0) * m.mz + cz.i0;
+}
+
+__global__ void __launch_bounds__(kBlock) cell_key_kernel(const float *__restrict__ x, int64_t n, Mesh m,
+                                                           int *__restrict__ keys, int *__restrict__ counts) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int key = base_cell(x, i, m);
+        keys[i] = key;
+        atomicAdd(counts + key, 1);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) cell_scatter_kernel(const float *__restrict__ x,
+                                                               const int32_t *__restrict__ types,
+                                                               const float *__restrict__ w, int64_t n,
+                                                               const int *__restrict__ keys,
+                                                               int *__restrict__ cursor, int *__restrict__ perm,
+                                                               float *__restrict__ xs, int32_t *__restrict__ ts,
+                                                               float *__restrict__ ws) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int pos = atomicAdd(cursor + keys[i], 1);
+        perm[pos] = (int)i;
+        xs[3 * (int64_t)pos] = x[3 * i];
+        xs[3 * (int64_t)pos + 1] = x[3 * i + 1];
+        xs[3 * (int64_t)pos + 2] = x[3 * i + 2];
+        ts[pos] = types ? types[i] : 0;
+        if (ws) ws[pos] = w[i];
+    }
+}
+
+// One thread per cell: sum the fixed-point corner deposits of the particles
+// binned in the 8 cells whose upper corners include this one (the same int64
+// values paint_fixed_kernel adds atomically, so the sums are identical and
+// independent of order), then write rho = sum * 2^-32 / dV per type.
+template <int MAXT>
+__global__ void __launch_bounds__(kBlock) paint_gather_kernel(const float *__restrict__ xs,
+                                                               const int32_t *__restrict__ ts,
+                                                               const float *__restrict__ ws,
+                                                               const int *__restrict__ start, Mesh m, PaintArgs pa,
+                                                               double out_scale, float *__restrict__ out,
+                                                               long long *__restrict__ out_fixed) {
+#pragma clang fp contract(off)
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; cell < m.g; cell += stride) {
+        const int iz = (int)(cell % m.mz);
+        const int iy = (int)((cell / m.mz) % m.my);
+        const int ix = (int)(cell / ((int64_t)m.mz * m.my));
+        long long acc[MAXT];
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t) acc[t] = 0;
+        for (int c = 0; c < 8; ++c) {
+            const int a = c >> 2, b = (c >> 1) & 1, d = c & 1;
+            const int bx = a ? (ix == 0 ? m.mx - 1 : ix - 1) : ix;
+            const int by = b ? (iy == 0 ? m.my - 1 : iy - 1) : iy;
+            const int bz = d ? (iz == 0 ? m.mz - 1 : iz - 1) : iz;
+            const int bin = (bx * m.my + by) * m.mz + bz;
+            for (int q = start[bin]; q < start[bin + 1]; ++q) {
+                const int t = ts[q];
+                if ((unsigned)t >= (unsigned)pa.n_types) continue;
+                const double mass = ws ? (double)ws[q] : pa.mass[t];
+                const CicAxis cx = cic_axis((double)xs[3 * (int64_t)q + 0], m.inv_h[0], m.mx);
+                const CicAxis cy = cic_axis((double)xs[3 * (int64_t)q + 1], m.inv_h[1], m.my);
+                const CicAxis cz = cic_axis((double)xs[3 * (int64_t)q + 2], m.inv_h[2], m.mz);
+                long long qc;
+                if (c < 7) {
+                    const double wgt = ((a ? cx.w1 : cx.w0) * (b ? cy.w1 : cy.w0)) * (d ? cz.w1 : cz.w0);
+                    qc = __double2ll_rn((wgt * mass) * pa.scale);
+                } else {
+                    long long s7 = 0;
+#pragma unroll
+                    for (int e = 0; e < 7; ++e) {
+                        const int ea = e >> 2, eb = (e >> 1) & 1, ed = e & 1;
+                        const double wgt = ((ea ? cx.w1 : cx.w0) * (eb ? cy.w1 : cy.w0)) * (ed ? cz.w1 : cz.w0);
+                        s7 += __double2ll_rn((wgt * mass) * pa.scale);
+                    }
+                    qc = __double2ll_rn(mass * pa.scale) - s7;
+                }
+#pragma unroll
+                for (int u = 0; u < MAXT; ++u)
+                    if (u == t) acc[u] += qc;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t) {
+            if (t >= pa.n_types) break;
+            out[(int64_t)t * m.g + cell] = (float)((double)acc[t] * out_scale);
+            if (out_fixed) out_fixed[(int64_t)t * m.g + cell] = acc[t];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // readout: out[i, c] = w_i * sum_corners W * grid[type_i * ncomp + c][corner]
 // ---------------------------------------------------------------------------
+// perm != NULL: particle i of the (sorted) inputs is written to row perm[i]
 template <typename PosT>
 __global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict__ grids, int ncomp,
                                                           const PosT *__restrict__ x,
                                                           const int32_t *__restrict__ types,
                                                           const float *__restrict__ scale_per_particle,
                                                           int64_t n, int n_types, Mesh m,
-                                                          float *__restrict__ out) {
+                                                          float *__restrict__ out, const int *__restrict__ perm) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const int t = types ? types[i] : 0;
+        const int64_t o = perm ? (int64_t)perm[i] : i;
         if ((unsigned)t >= (unsigned)n_types) {
-            for (int c = 0; c < ncomp; ++c) out[ncomp * i + c] = 0.f;
+            for (int c = 0; c < ncomp; ++c) out[ncomp * o + c] = 0.f;
             continue;
         }
         const CicAxis cx = cic_axis((double)x[3 * i + 0], m.inv_h[0], m.mx);
@@ -109,7 +212,7 @@ __global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict
             for (int q = 0; q < ncomp; ++q) acc[q] += w * (double)grids[((int64_t)t * ncomp + q) * m.g + idx];
         }
         const double s = scale_per_particle ? (double)scale_per_particle[i] : 1.0;
-        for (int q = 0; q < ncomp; ++q) out[ncomp * i + q] = (float)(acc[q] * s);
+        for (int q = 0; q < ncomp; ++q) out[ncomp * o + q] = (float)(acc[q] * s);
     }
 }
 
@@ -366,6 +469,11 @@ struct hymd_pm {
     DevBuf result;    // reduced doubles
     HostBuf host;     // pinned readback
     int kblocks;      // blocks of the k-space passes
+    // spatial sort of the last painted particle set (cell-binned order)
+    DevBuf keys, perm, xs, ts, ws;  // per particle
+    DevBuf counts, start, cursor, tiles;  // per cell (counts kept zero between sorts)
+    int64_t sorted_n = 0;
+    bool sorted_weights = false;
 };
 
 static int set_mesh(hymd_pm *pm, const int32_t mesh[3], const double box[3]) {
@@ -422,9 +530,15 @@ extern "C" int hymd_pm_create(hymd_pm **out, const int32_t mesh[3], const double
     if (st == HYMD_OK) st = pm->partials.reserve(sizeof(double) * (pm->kblocks + 16));
     if (st == HYMD_OK) st = pm->result.reserve(sizeof(double) * 16);
     if (st == HYMD_OK) st = pm->host.reserve(sizeof(double) * 16);
+    if (st == HYMD_OK) st = pm->counts.reserve(sizeof(int) * (m.g + 2));
+    if (st == HYMD_OK) st = pm->start.reserve(sizeof(int) * (m.g + 1));
+    if (st == HYMD_OK) st = pm->cursor.reserve(sizeof(int) * m.g);
+    if (st == HYMD_OK) st = pm->tiles.reserve(sizeof(int) * (scan_tiles(m.g) + 1));
     if (st == HYMD_OK) {
         zero_kernel<<<grid_for(m.g * max_types), kBlock, 0, pm->stream>>>(
             (unsigned long long *)pm->paint.ptr, m.g * max_types);
+        zero_kernel<<<grid_for(m.g / 2 + 1), kBlock, 0, pm->stream>>>((unsigned long long *)pm->counts.ptr,
+                                                                       m.g / 2 + 1);
         hipError_t e = hipGetLastError();
         if (e == hipSuccess) e = hipStreamSynchronize(pm->stream);
         if (e != hipSuccess) {
@@ -474,6 +588,79 @@ static FieldK field_params(int T, const double *coupling, const double *shift, d
     return p;
 }
 
+// Bin particles by CIC base cell: perm[pos] = original index, xs/ts/ws the
+// sorted positions, types (0 when types == NULL) and weights (if w != NULL),
+// start[c] .. start[c+1] the particles based in cell c.
+static int sort_particles(hymd_pm *pm, const float *x, const int32_t *types, const float *w, int64_t n) {
+    const Mesh &m = pm->mesh;
+    hipStream_t s = pm->stream;
+    HYMD_REQUIRE(n < (int64_t(1) << 31) && m.g < (int64_t(1) << 31), "too many particles or cells to sort");
+    const size_t nn = (size_t)std::max<int64_t>(n, 1);
+    HYMD_TRY(pm->keys.reserve(sizeof(int) * nn));
+    HYMD_TRY(pm->perm.reserve(sizeof(int) * nn));
+    HYMD_TRY(pm->xs.reserve(sizeof(float) * 3 * nn));
+    HYMD_TRY(pm->ts.reserve(sizeof(int32_t) * nn));
+    if (w) HYMD_TRY(pm->ws.reserve(sizeof(float) * nn));
+    if (n > 0)
+        cell_key_kernel<<<grid_for(n), kBlock, 0, s>>>(x, n, m, (int *)pm->keys.ptr, (int *)pm->counts.ptr);
+    HYMD_HIP(hipGetLastError());
+    HYMD_TRY(exclusive_scan((int *)pm->counts.ptr, (int *)pm->start.ptr, m.g, (int *)pm->tiles.ptr,
+                            (int *)pm->cursor.ptr, true, s));
+    if (n > 0)
+        cell_scatter_kernel<<<grid_for(n), kBlock, 0, s>>>(x, types, w, n, (const int *)pm->keys.ptr,
+                                                           (int *)pm->cursor.ptr, (int *)pm->perm.ptr,
+                                                           (float *)pm->xs.ptr, (int32_t *)pm->ts.ptr,
+                                                           w ? (float *)pm->ws.ptr : nullptr);
+    HYMD_HIP(hipGetLastError());
+    pm->sorted_n = n;
+    pm->sorted_weights = (w != nullptr);
+    return HYMD_OK;
+}
+
+// Gather paint of the sorted particles into float grids rho_t = paint/dV
+// (T grids), optionally also the int64 fixed-point sums.
+static int paint_sorted(hymd_pm *pm, int T, const double *mass, float *out, long long *out_fixed) {
+    PaintArgs pa;
+    pa.n_types = T;
+    pa.scale = std::ldexp(1.0, HYMD_FIXED_SHIFT);
+    for (int t = 0; t < HYMD_MAX_TYPES; ++t) pa.mass[t] = (mass && t < T) ? mass[t] : 1.0;
+    const double out_scale = std::ldexp(1.0, -HYMD_FIXED_SHIFT) / cell_volume(pm);
+    const float *ws = pm->sorted_weights ? (const float *)pm->ws.ptr : nullptr;
+    const Mesh &m = pm->mesh;
+    const int nb = grid_for(m.g);
+    const float *xs = (const float *)pm->xs.ptr;
+    const int32_t *ts = (const int32_t *)pm->ts.ptr;
+    const int *st = (const int *)pm->start.ptr;
+    if (T <= 4)
+        paint_gather_kernel<4><<<nb, kBlock, 0, pm->stream>>>(xs, ts, ws, st, m, pa, out_scale, out, out_fixed);
+    else if (T <= 8)
+        paint_gather_kernel<8><<<nb, kBlock, 0, pm->stream>>>(xs, ts, ws, st, m, pa, out_scale, out, out_fixed);
+    else
+        paint_gather_kernel<16><<<nb, kBlock, 0, pm->stream>>>(xs, ts, ws, st, m, pa, out_scale, out, out_fixed);
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
+}
+
+// Readout of ncomp grids per type at the last sorted particle set, written to
+// the particles' original rows (scaled by their weights if the sort had them).
+static int readout_sorted(hymd_pm *pm, int ncomp, int n_types, const float *grids, float *out, bool weighted) {
+    const int64_t n = pm->sorted_n;
+    if (n > 0)
+        readout_kernel<float><<<grid_for(n), kBlock, 0, pm->stream>>>(
+            grids, ncomp, (const float *)pm->xs.ptr, (const int32_t *)pm->ts.ptr,
+            weighted ? (const float *)pm->ws.ptr : nullptr, n, n_types, pm->mesh, out, (const int *)pm->perm.ptr);
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
+}
+
+namespace hymd {
+// compute_field_force at the particles painted by the last update_field on
+// this handle (positions and types unchanged since)
+int field_readout_last(hymd_pm *pm, int n_types, const float *force_mesh, float *force) {
+    return readout_sorted(pm, 3, n_types, force_mesh, force, false);
+}
+}  // namespace hymd
+
 static int launch_paint(hymd_pm *pm, const float *x, const int32_t *types, const float *weights, int64_t n,
                         int32_t n_types, const double *mass, unsigned long long *grid) {
     PaintArgs pa;
@@ -500,14 +687,15 @@ extern "C" int hymd_paint_fixed(hymd_pm *pm, const float *x, const int32_t *type
                                 int32_t accumulate) {
     HYMD_REQUIRE(pm && grid && (x || n == 0), "null argument");
     HYMD_REQUIRE(n >= 0, "negative particle count");
-    HYMD_REQUIRE(n_types >= 1 && n_types <= HYMD_MAX_TYPES, "n_types out of range");
+    HYMD_REQUIRE(n_types >= 1 && n_types <= pm->max_types, "n_types out of range");
     const int ngrids = types ? n_types : 1;
     if (!accumulate) {
-        const int64_t cells = pm->mesh.g * ngrids;
-        zero_kernel<<<grid_for(cells), kBlock, 0, pm->stream>>>((unsigned long long *)grid, cells);
-        HYMD_HIP(hipGetLastError());
+        // the hot path: cell sort + gather (int64 sums written to grid)
+        HYMD_TRY(sort_particles(pm, x, types, weights, n));
+        return paint_sorted(pm, ngrids, mass, (float *)pm->paint.ptr, (long long *)grid);
     }
-    return launch_paint(pm, x, types, weights, n, types ? n_types : 1, mass, (unsigned long long *)grid);
+    // accumulate into the caller's grid with one 64-bit atomic per corner
+    return launch_paint(pm, x, types, weights, n, ngrids, mass, (unsigned long long *)grid);
 }
 
 extern "C" int hymd_fft_r2c(hymd_pm *pm, const float *in, void *out, int32_t batch) {
@@ -528,7 +716,7 @@ extern "C" int hymd_readout(hymd_pm *pm, const float *grid, const float *x, int6
     HYMD_REQUIRE(pm && grid && out && (x || n == 0) && n >= 0, "bad argument");
     if (n > 0)
         readout_kernel<float><<<grid_for(n), kBlock, 0, pm->stream>>>(grid, 1, x, nullptr, nullptr, n, 1, pm->mesh,
-                                                                        out);
+                                                                        out, nullptr);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }
@@ -549,11 +737,12 @@ extern "C" int hymd_update_field(hymd_pm *pm, const float *x, const int32_t *typ
     float2 *spec = (float2 *)pm->spec.ptr;
     float2 *grad = (float2 *)pm->grad.ptr;
 
-    // 1. paint (grid is zero on entry; the z pass clears it again)
-    HYMD_TRY(launch_paint(pm, x, types, nullptr, n, T, mass, grid));
-    // 2. r2c of rho_t = paint_t / dV
-    const double scale = std::ldexp(1.0, -HYMD_FIXED_SHIFT) / cell_volume(pm);
-    HYMD_TRY(fft_r2c_fixed(pm->fft, (int64_t *)grid, spec, T, scale, true, s));
+    // 1. cell sort + gather paint of rho_t = paint_t / dV (float grids)
+    float *rho = (float *)grid;  // the handle's paint buffer holds T float grids here
+    HYMD_TRY(sort_particles(pm, x, types, nullptr, n));
+    HYMD_TRY(paint_sorted(pm, T, mass, rho, nullptr));
+    // 2. r2c
+    HYMD_TRY(fft_r2c_real(pm->fft, rho, spec, T, s));
     // 3. fused filter / W' / filter / gradient
     FieldK p = field_params(T, coupling, shift, constant, sigma, m);
     p.want_energy = energy ? 1 : 0;
@@ -610,11 +799,11 @@ extern "C" int hymd_field_moments(hymd_pm *pm, const float *x, const int32_t *ty
     const Mesh &m = pm->mesh;
     const int T = n_types;
     hipStream_t s = pm->stream;
-    unsigned long long *grid = (unsigned long long *)pm->paint.ptr;
+    float *rho = (float *)pm->paint.ptr;
     float2 *spec = (float2 *)pm->spec.ptr;
-    HYMD_TRY(launch_paint(pm, x, types, nullptr, n, T, mass, grid));
-    const double scale = std::ldexp(1.0, -HYMD_FIXED_SHIFT) / cell_volume(pm);
-    HYMD_TRY(fft_r2c_fixed(pm->fft, (int64_t *)grid, spec, T, scale, true, s));
+    HYMD_TRY(sort_particles(pm, x, types, nullptr, n));
+    HYMD_TRY(paint_sorted(pm, T, mass, rho, nullptr));
+    HYMD_TRY(fft_r2c_real(pm->fft, rho, spec, T, s));
     KGeom g = kgeom(pm);
     const int np = T * (T + 1) / 2;
     const int nv = 2 * np + T;
@@ -653,11 +842,8 @@ extern "C" int hymd_compute_field_force(hymd_pm *pm, const float *x, const int32
                                         int32_t n_types, const float *force_mesh, float *force) {
     HYMD_REQUIRE(pm && force_mesh && force && (x || n == 0) && (types || n == 0), "null argument");
     HYMD_REQUIRE(n >= 0 && n_types >= 1 && n_types <= HYMD_MAX_TYPES, "bad sizes");
-    if (n > 0)
-        readout_kernel<float><<<grid_for(n), kBlock, 0, pm->stream>>>(force_mesh, 3, x, types, nullptr, n, n_types,
-                                                                        pm->mesh, force);
-    HYMD_HIP(hipGetLastError());
-    return HYMD_OK;
+    HYMD_TRY(sort_particles(pm, x, types, nullptr, n));
+    return readout_sorted(pm, 3, n_types, force_mesh, force, false);
 }
 
 extern "C" int hymd_update_field_force_q(hymd_pm *pm, const float *x, const float *charges, int64_t n,
@@ -667,16 +853,12 @@ extern "C" int hymd_update_field_force_q(hymd_pm *pm, const float *x, const floa
     HYMD_REQUIRE(pm && elec_forces && (n == 0 || (x && charges)) && n >= 0, "bad argument");
     const Mesh &m = pm->mesh;
     hipStream_t s = pm->stream;
-    unsigned long long *grid = (unsigned long long *)pm->paint.ptr;
     float2 *spec = (float2 *)pm->spec.ptr;
     float2 *out = (float2 *)pm->grad.ptr;
-    HYMD_TRY(launch_paint(pm, x, nullptr, charges, n, 1, nullptr, grid));
-    const double scale = std::ldexp(1.0, -HYMD_FIXED_SHIFT) / cell_volume(pm);
-    if (phi_q) {
-        fixed_to_float_kernel<<<grid_for(m.g), kBlock, 0, s>>>((const long long *)grid, phi_q, m.g, scale);
-        HYMD_HIP(hipGetLastError());
-    }
-    HYMD_TRY(fft_r2c_fixed(pm->fft, (int64_t *)grid, spec, 1, scale, true, s));
+    float *rho = phi_q ? phi_q : (float *)pm->paint.ptr;
+    HYMD_TRY(sort_particles(pm, x, nullptr, charges, n));
+    HYMD_TRY(paint_sorted(pm, 1, nullptr, rho, nullptr));
+    HYMD_TRY(fft_r2c_real(pm->fft, rho, spec, 1, s));
     PmeK p;
     p.sigma2 = sigma * sigma;
     p.inv_g = 1.0 / (double)m.g;
@@ -693,9 +875,7 @@ extern "C" int hymd_update_field_force_q(hymd_pm *pm, const float *x, const floa
     HYMD_TRY(fft_inverse_xy(pm->fft, out, psi ? 4 : 3, s));
     HYMD_TRY(fft_c2r_z(pm->fft, out, field, 3, 1.f, s));
     if (psi) HYMD_TRY(fft_c2r_z(pm->fft, out + 3 * m.gc, psi, 1, 1.f, s));
-    if (n > 0)
-        readout_kernel<float><<<grid_for(n), kBlock, 0, s>>>(field, 3, x, nullptr, charges, n, 1, m, elec_forces);
-    HYMD_HIP(hipGetLastError());
+    HYMD_TRY(readout_sorted(pm, 3, 1, field, elec_forces, true));
     if (energy_q) {
         double e = 0.0;
         HYMD_TRY(read_result(pm, pm->kblocks, 1, &e));

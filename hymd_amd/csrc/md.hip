@@ -65,7 +65,7 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
                                    m->force_mesh, nullptr, nullptr, nullptr, energies ? &energies[0] : nullptr));
         mark(m, 3);
         mark(m, 4);
-        HYMD_TRY(hymd_compute_field_force(m->pm, m->x32, m->types, m->n, m->T, m->force_mesh, m->f_field));
+        HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));
         mark(m, 4);
         return HYMD_OK;
     }
@@ -90,7 +90,7 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
                                m->force_mesh, nullptr, nullptr, nullptr, energies ? &energies[0] : nullptr));
     mark(m, 3);
     mark(m, 4);
-    HYMD_TRY(hymd_compute_field_force(m->pm, m->x32, m->types, m->n, m->T, m->force_mesh, m->f_field));
+    HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));
     mark(m, 4);
     HYMD_TRY(hymd_respa_kick(m->v, m->n, m->f_fast, half_in, m->f_field, half_out, m->work));
     return HYMD_OK;

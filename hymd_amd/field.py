@@ -122,15 +122,16 @@ class ParticleMesh:
              ctypes.byref(e) if want_energy else None)
         return e.value if want_energy else None
 
-    def paint_fixed(self, x32, types, n_types, mass=None, weights=None, out=None):
-        """int64 fixed-point CIC grids (value = round(w m 2^32)), (T or 1, Mx, My, Mz)."""
+    def paint_fixed(self, x32, types, n_types, mass=None, weights=None, out=None, accumulate=False):
+        """int64 fixed-point CIC grids (value = round(w m 2^32)), (T or 1, Mx, My, Mz);
+        ``accumulate`` adds into ``out`` with atomics instead of overwriting."""
         ng = n_types if types is not None else 1
         if out is None:
             out = torch.zeros((ng,) + self.real_shape, dtype=torch.int64, device=self.device)
         call("hymd_paint_fixed", self.handle, dptr(x32, torch.float32, "positions"),
              dptr(types, torch.int32, "types"), dptr(weights, torch.float32, "weights"), int(x32.shape[0]),
              int(n_types), darr(mass if mass is not None else [1.0] * int(n_types)),
-             dptr(out, torch.int64, "grid"), 0)
+             dptr(out, torch.int64, "grid"), 1 if accumulate else 0)
         return out
 
     def r2c(self, real):

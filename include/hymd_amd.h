@@ -120,7 +120,10 @@ int hymd_self_energy_q(const float *charges, int64_t n, double sigma, double ele
  * that assemble their own pipelines). */
 /* CIC paint into int64 fixed-point grids (value = round(w*m*2^32)); grid is
  * (n_types or 1, Mx,My,Mz); if types==NULL all particles go to grid 0 with weight
- * weights[i] (device float32) or mass[0]. accumulate=0 zeroes the grid first. */
+ * weights[i] (device float32) or mass[0].  accumulate=0 overwrites the grid
+ * (cell sort + per-cell gather, the update_field path); accumulate=1 adds to
+ * it (one 64-bit integer atomic per particle corner).  Both give identical
+ * integers. */
 int hymd_paint_fixed(hymd_pm *pm, const float *x, const int32_t *types, const float *weights, int64_t n,
                      int32_t n_types, const double *mass, int64_t *grid, int32_t accumulate);
 /* unnormalised batched 3-D r2c / c2r on (batch,Mx,My,Mz) real <-> (batch,Mx,My,Mz/2+1) complex64 */

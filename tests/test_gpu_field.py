@@ -33,7 +33,11 @@ def test_paint_fixed_bit_exact(hip, mesh, box):
     types = rng.integers(0, T, n).astype(np.int32)
     mass = [1.0, 0.75, 2.5]
     pm = _pm(mesh, box, T)
-    grid = pm.paint_fixed(_dev(x32, torch.float32), _dev(types, torch.int32), T, mass=mass).cpu().numpy()
+    xd, td = _dev(x32, torch.float32), _dev(types, torch.int32)
+    grid = pm.paint_fixed(xd, td, T, mass=mass).cpu().numpy()           # cell sort + gather
+    scat = torch.zeros((T,) + tuple(mesh), dtype=torch.int64, device="cuda")
+    pm.paint_fixed(xd, td, T, mass=mass, out=scat, accumulate=True)      # 64-bit atomics
+    np.testing.assert_array_equal(grid, scat.cpu().numpy())
     for t in range(T):
         sel = types == t
         ref = F.paint_fixed(x32[sel].astype(np.float64), mesh, box, mass=mass[t])
