@@ -35,9 +35,9 @@ def algorithmic_bytes(kernel, n, n_bonds, n_angles, T, mesh):
     """Compulsory HBM bytes of one launch of a stage."""
     G = mesh[0] * mesh[1] * mesh[2]
     if kernel == "fused":       # whole inner loop: x,v f64 read+write, F_slow f32 in, F_fast f32 in+out,
-        # x32 out, per-particle term list (4 B offset + 4 B per entry: 2 per bond, 3 per angle),
-        # bond (a|b, r0, k) 20 B, angle (a|b|c, theta0, k) 24 B
-        return 148 * n + 28 * n_bonds + 36 * n_angles
+        # x32 out, row keys out (8 B), per-particle term list (4 B offset + 4 B per entry: 2 per bond,
+        # 3 per angle), bond (a|b, r0, k) 20 B, angle (a|b|c, theta0, k) 24 B
+        return 156 * n + 28 * n_bonds + 36 * n_angles
     if kernel == "bonded":      # x f64 in, F f32 out, bond a,b,r0,k and angle a,b,c,t0,k
         return 36 * n + 24 * n_bonds + 28 * n_angles
     if kernel == "kick_drift":  # v,x f64 read+write, F f32 in, x f32 out
@@ -53,6 +53,19 @@ def algorithmic_bytes(kernel, n, n_bonds, n_angles, T, mesh):
     if kernel == "readout":     # sorted x f32 + type + row index in, 3 grids per type once, F f32 out
         return 32 * n + 12 * T * G
     raise ValueError(kernel)
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    calibrated FETCH_SIZE + WRITE_SIZE summary in profiles/, or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_lipid_water_1M_128_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
 
 
 def cpu_baseline(system, steps):
@@ -77,6 +90,152 @@ def cpu_baseline(system, steps):
                           f"single-threaded paint/readout and bonded C loops"}
 
 
+def batched_chi_gradients(n_replicas, iters):
+    """Second metric: replica-gradients/s of the batched path (one captured
+    launch sequence over all replicas per evaluation: bin, paint, r2c,
+    moments, then the host energy / chi / kappa / sigma derivatives) against
+    the per-replica path (one field-moments call per replica)."""
+    import torch
+
+    from hymd_amd import synth
+    from hymd_amd.autograd import energy_and_gradients, moments
+    from hymd_amd.batch import ReplicaBatch
+    from hymd_amd.field import ParticleMesh
+    rs = synth.replica_set(n_replicas=n_replicas, seed=1003)
+    x = torch.as_tensor(rs.positions.astype(np.float32)).cuda()
+    t = torch.as_tensor(rs.types).cuda()
+    b = ReplicaBatch(rs.meshes, rs.boxes, rs.counts, 2)
+    for _ in range(2):
+        b.energy_and_gradients(x, t, rs.chi, rs.kappa, rs.sigma)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        b.energy_and_gradients(x, t, rs.chi, rs.kappa, rs.sigma)
+    torch.cuda.synchronize()
+    batched = n_replicas * iters / (time.perf_counter() - t0)
+    b.close()
+    # per-replica path: one handle per replica, one moments call (and its
+    # synchronisation) per replica and evaluation
+    o = np.concatenate([[0], np.cumsum(rs.counts)])
+    pms = [ParticleMesh(tuple(int(m) for m in rs.meshes[r]), rs.boxes[r], 2) for r in range(n_replicas)]
+    rho0 = rs.counts / np.prod(rs.boxes, axis=1)
+
+    def one_pass():
+        for r in range(n_replicas):
+            P, Q, phi0 = moments(pms[r], x[o[r]:o[r + 1]], t[o[r]:o[r + 1]], 2, rs.sigma[r])
+            energy_and_gradients(P, Q, phi0, rs.chi[r], rs.kappa[r], rho0[r], rho0[r], rs.sigma[r],
+                                 np.prod(rs.boxes[r]))
+
+    one_pass()
+    torch.cuda.synchronize()
+    passes = max(1, iters // 4)
+    t0 = time.perf_counter()
+    for _ in range(passes):
+        one_pass()
+    torch.cuda.synchronize()
+    single = n_replicas * passes / (time.perf_counter() - t0)
+    return {"value": batched, "unit": "replica-gradients/s", "replicas": n_replicas,
+            "particles": int(rs.counts.sum()), "meshes": sorted({int(m) for m in rs.meshes[:, 0]}),
+            "ms_per_batch": 1e3 * n_replicas / batched, "per_replica_path": single, "speedup": batched / single,
+            "data": "synthetic A5B5 melts, seed 1003; gradients dE/dchi, dE/dkappa, dE/dsigma per replica"}
+
+
+def peptide_solvent_stages(reps, check):
+    """The 2,097,152-bead peptide/solvent workload (160^3 mesh shared by the
+    density and PME, CBT dihedrals with dipole reconstruction): every force
+    stage of one outer step timed on the device (HIP events on the launch
+    stream, mean of `reps` calls), and, with `check`, the same forces from the
+    FP64 oracle at the same configuration, with its host time."""
+    import torch
+
+    from hymd_amd import synth
+    from hymd_amd.field import ParticleMesh
+    from hymd_amd.force import BondedTopology, compute_dihedral_forces__fortran__double, \
+        dipole_forces_redistribution
+    s = synth.peptide_solvent(seed=1002)
+    A, b, c0 = synth.hamiltonian_form(s)
+    T, n = s.n_types, s.n
+    ke, sigma = 138.935458, s.sigma
+    elec = ke / s.dielectric
+    x64 = torch.as_tensor(s.positions).cuda()
+    x32 = x64.float().contiguous()
+    types = torch.as_tensor(s.types).cuda()
+    q = torch.as_tensor(s.charges).cuda()
+    pm = ParticleMesh(s.mesh, s.box, T)
+    fm = pm.create("real", batch=(T, 3))
+    f_field = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    f_q = torch.empty_like(f_field)
+    f_fast = torch.empty_like(f_field)
+    topo = BondedTopology(n, s.bonds, s.angles)
+    da, db, dc, dd, coeff, dty, last = s.extra["dihedrals"]
+    m4 = len(da)
+    dih = [torch.as_tensor(np.asarray(v, dtype=np.int32)).cuda() for v in (da, db, dc, dd)]
+    coeff_d = torch.as_tensor(coeff).cuda()
+    dty_d = torch.as_tensor(dty).cuda()
+    last_d = torch.as_tensor(last).cuda()
+    f_dih = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    dip = torch.zeros((m4, 4, 3), dtype=torch.float64, device="cuda")
+    tmat = torch.zeros((m4, 6, 3, 3), dtype=torch.float64, device="cuda")
+    f_dip = torch.randn((m4, 4, 3), dtype=torch.float64, device="cuda")
+    f_red = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
+    e_self = float(ke / s.dielectric * np.sqrt(1.0 / (2.0 * np.pi * sigma * sigma)) * np.sum(s.charges.astype(np.float64) ** 2))
+    box = s.box
+    stages = {
+        "field_update": lambda: pm.field_update(x32, types, T, A, b, c0, [1.0] * T, sigma, fm),
+        "field_readout": lambda: pm.readout_forces(x32, types, T, fm, f_field),
+        "pme": lambda: pm.pme(x32, q, sigma, elec, e_self, f_q),
+        "bonds_angles": lambda: topo.forces(x64, f_fast, box),
+        "dihedrals_cbt_dipoles": lambda: compute_dihedral_forces__fortran__double(
+            f_dih, x64, dip, tmat, box, *dih, coeff_d, dty_d, last_d, 1),
+        "dipole_redistribution": lambda: dipole_forces_redistribution(f_red, f_dip, tmat, *dih, dty_d, last_d),
+    }
+    ms = {}
+    for name, fn in stages.items():
+        fn()
+        torch.cuda.synchronize()
+        st = torch.cuda.current_stream()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(st)
+        for _ in range(reps):
+            fn()
+        ev1.record(st)
+        torch.cuda.synchronize()
+        ms[name] = ev0.elapsed_time(ev1) / reps
+    out = {"workload": "peptide_solvent: %d beads (%d chains x 40 residues x 3 beads + solvent), %.0f nm box, "
+                       "%d^3 mesh for density and PME, %d types, %d charges, %d bonds, %d angles, %d CBT "
+                       "dihedrals with dipoles" % (n, 8738, s.box[0], s.mesh[0], T, int(np.count_nonzero(s.charges)),
+                                                   len(s.bonds[0]), len(s.angles[0]), m4),
+           "device_ms": ms, "device_ms_outer_step_forces": sum(ms.values())}
+    if check:
+        from oracle import bonded as OB
+        from oracle import field as OF
+        OF.set_workers(min(os.cpu_count() or 1, 16))
+        xo = s.positions.astype(np.float32).astype(np.float64)
+        cpu = {}
+        t0 = time.perf_counter()
+        ref = OF.update_field(xo, s.types, T, s.mesh, s.box, sigma, OF.Quadratic(A, b, c0))
+        fr = OF.compute_field_force(xo, s.types, ref["force_mesh"], s.mesh, s.box)
+        cpu["field_update_and_readout"] = 1e3 * (time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        rq = OF.update_field_force_q(xo, s.charges, s.mesh, s.box, sigma, ke, s.dielectric)
+        cpu["pme"] = 1e3 * (time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        fdo, _, _, _ = OB.cdf(s.positions, s.box, da, db, dc, dd, coeff, dty, last, 1)
+        cpu["dihedrals_cbt_dipoles"] = 1e3 * (time.perf_counter() - t0)
+
+        def err(got, want):
+            return {"rel_rms": float(np.sqrt(np.mean((got - want) ** 2)) / np.sqrt(np.mean(want ** 2))),
+                    "rel_max": float(np.abs(got - want).max() / np.abs(want).max())}
+        out["accuracy_vs_fp64_oracle"] = {"field_force": err(f_field.cpu().numpy(), fr),
+                                          "pme_force": err(f_q.cpu().numpy(), rq["forces"]),
+                                          "dihedral_force": err(f_dih.cpu().numpy(), fdo)}
+        out["cpu_oracle_ms"] = cpu
+        out["cpu_oracle_cores"] = min(os.cpu_count() or 1, 16)
+    pm.close()
+    topo.close()
+    return out
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -89,7 +248,13 @@ def main():
     ap.add_argument("--roofline-kernel", default="bonded", choices=["bonded", "kick_drift", "field", "readout"],
                     help="stage to time; 'bonded' is the fused inner loop when the topology allows it")
     ap.add_argument("--cpu-steps", type=int, default=4, help="oracle outer steps for cpu_baseline (0 = skip)")
+    ap.add_argument("--batch-replicas", type=int, default=64, help="replicas of the batched chi-gradient metric (0 = skip)")
+    ap.add_argument("--batch-iters", type=int, default=20)
+    ap.add_argument("--peptide-reps", type=int, default=10,
+                    help="timed calls per force stage of the peptide/solvent workload (0 = skip)")
     ap.add_argument("--small", action="store_true", help="reduced system for quick checks (not a bench line)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 10 device copies of MIB MiB (calibrates HBM byte counters) and exit")
     ap.add_argument("--no-graph", action="store_true", help="launch each step's kernels eagerly (no hipGraph)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -97,6 +262,17 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_probe > 0:
+        # a plain copy of known size: FETCH_SIZE and WRITE_SIZE per dispatch of
+        # this kernel calibrate the counters against bytes moved
+        a = torch.ones(args.copy_probe * 2 ** 18, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(10):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": a.numel() * 4, "copies": 10}))
+        return
 
     from hymd_amd import synth
     from hymd_amd.md import RespaMD
@@ -138,8 +314,9 @@ def main():
     kname = "fused" if (args.roofline_kernel == "bonded" and md.fused) else args.roofline_kernel
     nbytes = algorithmic_bytes(kname, s.n, len(s.bonds[0]), len(s.angles[0]), s.n_types, s.mesh)
     achieved = nbytes / (dur_ms * 1e-3) / 1e9
+    traffic = None if args.small else measured_traffic(KERNEL_NAMES[kname])
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": KERNEL_NAMES[kname], "avg_launch_us": round(dur_ms * 1e3, 2),
                 "launches_timed": launches, "algorithmic_bytes": int(nbytes),
                 "eager_ms_per_step": round(eager_ms, 4)}
@@ -157,6 +334,23 @@ def main():
             "field_force_rel_max": float(np.abs(f_field0 - fo_field).max() / np.abs(fo_field).max()),
             "bonded_force_rel_rms": float(np.sqrt(np.mean((f_fast0 - fo_fast) ** 2)) / np.sqrt(np.mean(fo_fast ** 2))),
         }
+
+    batch = None
+    if args.batch_replicas > 0:
+        md.close()
+        del md
+        torch.cuda.empty_cache()
+        log(f"batched chi gradients: {args.batch_replicas} replicas")
+        batch = batched_chi_gradients(args.batch_replicas, args.batch_iters)
+        log(f"batched chi gradients: {batch['value']:.4g} replica-gradients/s "
+            f"({batch['speedup']:.2f}x the per-replica path)")
+
+    peptide = None
+    if args.peptide_reps > 0 and not args.small:
+        torch.cuda.empty_cache()
+        log("peptide/solvent force stages")
+        peptide = peptide_solvent_stages(args.peptide_reps, check=args.cpu_steps > 0)
+        log(f"peptide/solvent: {peptide['device_ms_outer_step_forces']:.3f} ms of device force stages")
 
     out = {
         "metric": METRIC,
@@ -179,6 +373,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "accuracy_vs_fp64_oracle": accuracy,
+        "batched_chi_grads": batch,
+        "peptide_solvent_forces": peptide,
     }
     print(json.dumps(out))
 

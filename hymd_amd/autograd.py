@@ -68,6 +68,24 @@ def energy_and_gradients(P, Q, phi0, chi, kappa, rho0, a, sigma, volume):
     return E, dchi, dkappa, dsigma, A
 
 
+def energy_and_gradients_batched(P, Q, phi0, chi, kappa, rho0, a, sigma, volume):
+    """energy_and_gradients over R replicas at once: P, Q, chi (R,T,T), phi0
+    (R,T), the rest (R,).  Returns arrays E, dchi (R,T,T), dkappa, dsigma."""
+    T = P.shape[1]
+    chi = np.array(chi, dtype=np.float64)
+    chi[:, np.arange(T), np.arange(T)] = 0.0
+    kappa, rho0, a, sigma, V = (np.asarray(z, dtype=np.float64) for z in (kappa, rho0, a, sigma, volume))
+    inv = 1.0 / (kappa * rho0)
+    A = inv[:, None, None] + chi / rho0[:, None, None]
+    s0 = phi0.sum(axis=1)
+    E = V * (0.5 * np.einsum("rts,rts->r", A, P) - a * inv * s0 + 0.5 * a * a * inv)
+    dchi = (V / rho0)[:, None, None] * P
+    dchi[:, np.arange(T), np.arange(T)] = 0.0
+    dkappa = V * (-0.5 * P.sum(axis=(1, 2)) + a * s0 - 0.5 * a * a) / (kappa * kappa * rho0)
+    dsigma = -sigma * V * np.einsum("rts,rts->r", A, Q)
+    return E, dchi, dkappa, dsigma
+
+
 class FieldEnergy(torch.autograd.Function):
     """E = FieldEnergy.apply(x, chi, kappa, sigma, pm, types, rho0, a).
 

@@ -16,8 +16,9 @@ OBJDIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libhymd_amd.so")
 ARCH = os.environ.get("HYMD_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["common.hip", "scan.hip", "fft.hip", "field.hip", "bonded.hip", "integrator.hip", "fused.hip", "md.hip"]
-HEADERS = ["common.h", "fft.h", "bonded.h"]
+SOURCES = ["common.hip", "scan.hip", "fft.hip", "field.hip", "batch.hip", "bonded.hip", "integrator.hip", "fused.hip",
+           "md.hip"]
+HEADERS = ["common.h", "fft.h", "field_dev.h", "bonded.h"]
 
 
 def _hipcc():

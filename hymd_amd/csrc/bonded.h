@@ -29,12 +29,22 @@ __device__ __forceinline__ double get(V3 a, int i) { return i == 0 ? a.x : (i ==
 
 struct Box {
     double l[3];
+    double inv[3];  // 1 / l
 };
+inline Box make_box3(const double *l) {
+    Box b;
+    for (int d = 0; d < 3; ++d) {
+        b.l[d] = l[d];
+        b.inv[d] = 1.0 / l[d];
+    }
+    return b;
+}
 
-// minimum image, Fortran NINT (round half away from zero)
+// minimum image, Fortran NINT (round half away from zero) of d / L, with the
+// quotient taken as d * (1/L)
 __device__ __forceinline__ V3 mic(V3 v, const Box &b) {
-    return v3(v.x - b.l[0] * round(v.x / b.l[0]), v.y - b.l[1] * round(v.y / b.l[1]),
-              v.z - b.l[2] * round(v.z / b.l[2]));
+    return v3(v.x - b.l[0] * round(v.x * b.inv[0]), v.y - b.l[1] * round(v.y * b.inv[1]),
+              v.z - b.l[2] * round(v.z * b.inv[2]));
 }
 
 template <typename PosT>
@@ -45,16 +55,21 @@ __device__ __forceinline__ V3 load3(const PosT *__restrict__ r, int64_t i) {
 // --- harmonic bond: returns fa (force on b; a gets -fa) ------------------
 __device__ __forceinline__ V3 bond_term(V3 rab, double r0, double k, double &energy) {
     const double rn = norm(rab);
-    const double df = k * (rn - r0);
     const double dr = rn - r0;
+    const double df = k * dr;
     energy = 0.5 * k * (dr * dr);
-    return v3(-df * rab.x / rn, -df * rab.y / rn, -df * rab.z / rn);
+    const double s = -df / rn;
+    return v3(s * rab.x, s * rab.y, s * rab.z);
 }
 
 // --- harmonic angle in theta: fa, fc (a gets -fa, c -fc, b +fa+fc) --------
+// sin(theta) = sqrt((1 - cos)(1 + cos)) (both factors exact near cos = +-1),
+// and two divisions (1 / (|ra||rc|), ff / sin) instead of eight.
 __device__ __forceinline__ bool angle_term(V3 ra, V3 rc, double t0, double k, V3 &fa, V3 &fc, double &energy) {
     const double na = norm(ra), nc = norm(rc);
-    const V3 ea = ra / na, ec = rc / nc;
+    const double inv = 1.0 / (na * nc);
+    const double ina = nc * inv, inc = na * inv;
+    const V3 ea = ina * ra, ec = inc * rc;
     const double cosphi = dot(ea, ec);
     if (!(cosphi * cosphi < 1.0)) {
         fa = fc = v3(0, 0, 0);
@@ -62,13 +77,12 @@ __device__ __forceinline__ bool angle_term(V3 ra, V3 rc, double t0, double k, V3
         return false;
     }
     const double theta = acos(cosphi);
-    const double sinphi = sin(theta);
+    const double sinphi = sqrt((1.0 - cosphi) * (1.0 + cosphi));
     const double d = theta - t0;
     const double ff = k * d;
-    const double xrasin = -ff / (na * sinphi);
-    const double xrcsin = -ff / (nc * sinphi);
-    fa = xrasin * (ec - cosphi * ea);
-    fc = xrcsin * (ea - cosphi * ec);
+    const double q = -ff / sinphi;
+    fa = (q * ina) * (ec - cosphi * ea);
+    fc = (q * inc) * (ea - cosphi * ec);
     energy = 0.5 * ff * d;
     return true;
 }
@@ -161,10 +175,8 @@ __device__ __noinline__ void reconstruct(V3 rab, V3 rb, V3 rcb, const Box &box, 
         const V3 r0 = rb + 0.5 * rcb;
         const V3 d = (0.5 * delta) * (cos_psi * v + sin_psi * (cos_theta * n + sin_theta * m));
         V3 d1 = r0 + d, d2 = r0 - d;
-        o.dip[0] = v3(d1.x - box.l[0] * round(d1.x / box.l[0]), d1.y - box.l[1] * round(d1.y / box.l[1]),
-                      d1.z - box.l[2] * round(d1.z / box.l[2]));
-        o.dip[1] = v3(d2.x - box.l[0] * round(d2.x / box.l[0]), d2.y - box.l[1] * round(d2.y / box.l[1]),
-                      d2.z - box.l[2] * round(d2.z / box.l[2]));
+        o.dip[0] = mic(d1, box);
+        o.dip[1] = mic(d2, box);
         Mat3 Vb, Wb, Vc, Wa;
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) {

@@ -387,11 +387,7 @@ static int finish_reduce(Workspace &w, int nblocks, int nv, double *out, hipStre
     return HYMD_OK;
 }
 
-static Box make_box(const double box[3]) {
-    Box b;
-    for (int d = 0; d < 3; ++d) b.l[d] = box[d];
-    return b;
-}
+static Box make_box(const double box[3]) { return make_box3(box); }
 
 }  // namespace hymd
 

@@ -59,14 +59,53 @@ __device__ __forceinline__ CicAxis cic_axis(double x, double inv_h, int M) {
     double g = x * inv_h;
     double fl = floor(g);
     double f = g - fl;
-    long long ii = (long long)fl;
-    ii %= M;
-    if (ii < 0) ii += M;
-    c.i0 = (int)ii;
+    // i0 = fl mod M, without 64-bit integer division: positions inside the
+    // box take the first branch; the rest reduce exactly in float64 (integer
+    // values below 2^53).  Non-finite input lands on node 0.
+    int ii;
+    if (fl >= 0.0 && fl < (double)M) {
+        ii = (int)fl;
+    } else {
+        const double q = floor(fl / (double)M);
+        ii = (int)(fl - (double)M * q);
+        if (ii >= M) ii -= M;
+        if (ii < 0) ii += M;
+        if ((unsigned)ii >= (unsigned)M) ii = 0;
+    }
+    c.i0 = ii;
     c.i1 = (c.i0 + 1 == M) ? 0 : c.i0 + 1;
     c.w0 = 1.0 - f;
     c.w1 = f;
     return c;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  xcd_block maps the hardware block id to a logical one so that every
+// XCD walks one contiguous range of logical blocks (a bijection on [0, nb)),
+// which keeps neighbouring tiles, and the cache lines they share, in one L2.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
+    const unsigned per = nb / kXcds, rem = nb % kXcds, x = b % kXcds, q = b / kXcds;
+    return x * per + (x < rem ? x : rem) + q;
+}
+
+// np.mod(x, L) for L > 0 as numpy computes it: fmod, negatives shifted up by
+// L (so a tiny negative x gives L itself), zero returned as +0.  After a
+// drift x lies in [-L, 2L), where x - L (exact by Sterbenz) and x + L are
+// those values; anything else takes the general route.
+__device__ __forceinline__ double np_mod(double x, double L) {
+    double r;
+    if (x >= 0.0 && x < L) {
+        r = x;
+    } else if (x >= L && x < 2.0 * L) {
+        r = x - L;
+    } else if (x < 0.0 && x >= -L) {
+        r = x + L;
+    } else {
+        r = fmod(x, L);
+        if (r != 0.0 && r < 0.0) r += L;
+    }
+    return r == 0.0 ? 0.0 : r;
 }
 
 // mesh geometry passed by value to kernels
@@ -113,6 +152,13 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *out_
 // Reduce `nblocks` partial rows of NV doubles (row-major) in fixed order into
 // out[NV] with a single block; launch with kBlock threads.
 int reduce_partials(const double *partials, int nblocks, int nv, double *out, hipStream_t s);
+// Clear nwords 32-bit words with a fill kernel (safe inside captured graphs).
+int zero_words(void *ptr, int64_t nwords, hipStream_t s);
+
+// nbatch independent reductions: batch r reduces nblocks[r] rows from row
+// row0[r] into out[r * nv ..] (tables in device memory), same arithmetic.
+int reduce_partials_batched(const double *partials, const int *nblocks, const int64_t *row0, int nv, double *out,
+                            int nbatch, hipStream_t s);
 
 // Exclusive scan of n int32 counts into out[0..n] (out[n] = total); tiles is
 // scratch of scan_tiles(n) + 1 ints; cursor (optional) receives a copy of
@@ -140,6 +186,26 @@ TopoTerms topology_terms(const hymd_topology *t);
 // compute_field_force at the particle set sorted by the last update_field on
 // this handle (same positions and types)
 int field_readout_last(hymd_pm *pm, int n_types, const float *force_mesh, float *force);
+
+// Row-binning keys written by a producer kernel instead of the field path's
+// own key pass: keys[i] = (mesh row of particle i's CIC base cell, its rank
+// among that row's particles), counts[row] incremented once per particle.
+struct RowKeyTarget {
+    int2 *keys;
+    int *counts;
+    double inv_h[2];  // cells per nm along x, y
+    int mx, my;
+};
+int field_key_target(hymd_pm *pm, int64_t n, RowKeyTarget *t);
+// hymd_update_field for force grids only, with the keys of the float32
+// positions x already produced into field_key_target()'s arrays
+int update_field_keyed(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
+                       const double *coupling, const double *shift, double constant, const double *mass,
+                       double sigma, float *force_mesh, double *energy);
+// hymd_fused_inner that also produces the row keys of the final positions
+int fused_inner_impl(hymd_fused *f, double *x, double *v, float *x32, float *f_fast, const float *f_slow, double h_in,
+                     double h_out, double dt, int32_t n_inner, int32_t pending, const double box[3], double *energies,
+                     double *virials, void *stream, const RowKeyTarget *key);
 
 // Pinned host staging for small results.
 struct HostBuf {

@@ -56,25 +56,51 @@ void HostBuf::release() {
 
 // One block sums `nblocks` rows of `nv` doubles: thread j accumulates column
 // entries with stride, then a fixed tree combines the threads.
+// Fixed-order reduction of nblocks rows of nv doubles (one workgroup): each
+// column is summed by one wavefront, lanes striding over the rows, then a
+// shuffle tree, so the order depends on nblocks and nv only.
+__device__ void reduce_rows(const double *__restrict__ partials, int nblocks, int nv, double *__restrict__ out) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (int j = wid; j < nv; j += nw) {
+        double acc = 0.0;
+        for (int b = lane; b < nblocks; b += 64) acc += partials[(int64_t)b * nv + j];
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+        if (lane == 0) out[j] = acc;
+    }
+}
+
 __global__ void reduce_partials_kernel(const double *__restrict__ partials, int nblocks, int nv,
                                        double *__restrict__ out) {
-    __shared__ double red[kBlock];
-    for (int j = 0; j < nv; ++j) {
-        double acc = 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += blockDim.x) acc += partials[(int64_t)b * nv + j];
-        red[threadIdx.x] = acc;
-        __syncthreads();
-        for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-            if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[j] = red[0];
-        __syncthreads();
-    }
+    reduce_rows(partials, nblocks, nv, out);
+}
+
+// workgroup r reduces nblocks[r] rows starting at row row0[r] into out[r * nv ..]
+__global__ void reduce_partials_batched_kernel(const double *__restrict__ partials, const int *__restrict__ nblocks,
+                                               const int64_t *__restrict__ row0, int nv, double *__restrict__ out) {
+    const int r = blockIdx.x;
+    reduce_rows(partials + row0[r] * nv, nblocks[r], nv, out + (int64_t)r * nv);
+}
+
+__global__ void zero_words_kernel(uint32_t *__restrict__ p, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
+}
+
+int zero_words(void *ptr, int64_t nwords, hipStream_t s) {
+    if (nwords > 0) zero_words_kernel<<<grid_for(nwords), kBlock, 0, s>>>((uint32_t *)ptr, nwords);
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
 }
 
 int reduce_partials(const double *partials, int nblocks, int nv, double *out, hipStream_t s) {
     reduce_partials_kernel<<<1, kBlock, 0, s>>>(partials, nblocks, nv, out);
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
+}
+
+int reduce_partials_batched(const double *partials, const int *nblocks, const int64_t *row0, int nv, double *out,
+                            int nbatch, hipStream_t s) {
+    if (nbatch > 0) reduce_partials_batched_kernel<<<nbatch, kBlock, 0, s>>>(partials, nblocks, row0, nv, out);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }

@@ -16,6 +16,9 @@
 namespace hymd {
 
 constexpr int kMaxStages = 12;
+// Loads are issued kLoadBatch at a time per thread (all in flight before the
+// first LDS store) so that every wave keeps several HBM requests outstanding.
+constexpr int kLoadBatch = 8;
 
 struct FftAxis {
     int n;        // complex line length
@@ -32,20 +35,23 @@ struct FftPlan {
     DevBuf tw_x, tw_y, tw_zh;    // forward stage twiddles (n-1 entries)
     DevBuf post;                 // exp(-2 pi i k / Mz), k = 0..Mz/2
     size_t lds_x, lds_y, lds_z;  // dynamic LDS bytes per kernel
+    size_t lds_plane;            // y-z plane kernels (0: planes do not fit, separate passes)
 };
 
 int fft_plan_init(FftPlan &plan, const Mesh &mesh);
 
-// r2c of `batch` int64 fixed-point grids: value = q * scale; clears the input
-// rows after reading when `clear` (so the paint grid is zero for the next call).
-int fft_r2c_fixed(const FftPlan &plan, int64_t *in, float2 *out, int batch, double scale, bool clear,
-                  hipStream_t s);
+// 3-D real transforms of `batch` consecutive grids (unnormalised).  Where a
+// My x (Mz/2+1) plane fits in LDS the z and y passes run as one plane kernel
+// per (x-plane, grid), otherwise as separate line passes.
+//   r2c:   fft_r2c_zy (z r2c + y) then fft_forward_x;  fft_r2c_real = both
+//   c2r:   fft_inverse_x then fft_c2r_yz (y + z c2r, times `scale`);
+//          fft_c2r = both.  The spectra passed in may be overwritten.
+int fft_r2c_zy(const FftPlan &plan, const float *in, float2 *out, int batch, hipStream_t s);
+int fft_forward_x(const FftPlan &plan, float2 *data, int batch, hipStream_t s);
 int fft_r2c_real(const FftPlan &plan, const float *in, float2 *out, int batch, hipStream_t s);
-// forward complex passes only (y then x) / inverse passes (x then y), in place
-int fft_forward_xy(const FftPlan &plan, float2 *data, int batch, hipStream_t s);
-int fft_inverse_xy(const FftPlan &plan, float2 *data, int batch, hipStream_t s);
-// z c2r of `batch` half spectra (after fft_inverse_xy) into real grids, times `scale`
-int fft_c2r_z(const FftPlan &plan, const float2 *in, float *out, int batch, float scale, hipStream_t s);
+int fft_inverse_x(const FftPlan &plan, float2 *data, int batch, hipStream_t s);
+int fft_c2r_yz(const FftPlan &plan, float2 *in, float *out, int batch, float scale, hipStream_t s);
+int fft_c2r(const FftPlan &plan, float2 *data, float *out, int batch, float scale, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // device building blocks
@@ -172,6 +178,74 @@ __device__ __forceinline__ float2 *fft_lds(float2 *a, float2 *b, int W, const Ff
         p *= R;
     }
     return a;
+}
+
+// ---------------------------------------------------------------------------
+// In-place Stockham stage over lines with arbitrary strides (line l, element e
+// at buf[l * sl + e * se]): every thread first gathers its butterflies (up to
+// KB of radix R) into registers and transforms them, a barrier, then all
+// write back.  Consecutive threads take consecutive lines.  Capacity:
+// nlines * N <= kPlaneThreads * KB * R.
+// ---------------------------------------------------------------------------
+constexpr int kPlaneThreads = 512;
+constexpr size_t kPlaneLdsBytes = 150 * 1024;  // of the 160 KiB a gfx950 workgroup may use
+constexpr int kPlaneElems = 32;  // complex values per thread per stage (KB * R <= 32)
+
+template <int R, int KB>
+__device__ __forceinline__ void stockham_inplace(float2 *buf, int se, int sl, int nlines, int N, int p,
+                                                 const float2 *__restrict__ tw, float sgn) {
+    const int NR = N / R;
+    const int total = nlines * NR;
+    float2 u[KB][8];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+        const int b = threadIdx.x + kb * blockDim.x;
+        if (b >= total) continue;
+        const int i = b / nlines, l = b - i * nlines, k = i % p;
+        const float2 *s = buf + l * sl;
+#pragma unroll
+        for (int j = 0; j < R; ++j) u[kb][j] = s[(i + j * NR) * se];
+        if (p > 1) {
+#pragma unroll
+            for (int j = 1; j < R; ++j) {
+                float2 w = tw[k * (R - 1) + j - 1];
+                if (sgn > 0.f) w.y = -w.y;
+                u[kb][j] = cmul(u[kb][j], w);
+            }
+        }
+        small_dft<R>(u[kb], sgn);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+        const int b = threadIdx.x + kb * blockDim.x;
+        if (b >= total) continue;
+        const int i = b / nlines, l = b - i * nlines, k = i % p;
+        float2 *d = buf + l * sl;
+        const int ob = (i / p) * p * R + k;
+#pragma unroll
+        for (int j = 0; j < R; ++j) d[(ob + j * p) * se] = u[kb][j];
+    }
+    __syncthreads();
+}
+
+// In-place transform of nlines lines (strides as above); the caller barriers
+// before (data ready) and may read the result right after.
+__device__ __forceinline__ void fft_inplace(float2 *buf, int se, int sl, int nlines, const FftAxis &ax,
+                                            const float2 *__restrict__ tw, float sgn) {
+    int p = 1;
+    for (int s = 0; s < ax.nstages; ++s) {
+        const int R = ax.radix[s];
+        const float2 *tws = tw + ax.twoff[s];
+        switch (R) {
+            case 2: stockham_inplace<2, kPlaneElems / 2>(buf, se, sl, nlines, ax.n, p, tws, sgn); break;
+            case 3: stockham_inplace<3, kPlaneElems / 3>(buf, se, sl, nlines, ax.n, p, tws, sgn); break;
+            case 4: stockham_inplace<4, kPlaneElems / 4>(buf, se, sl, nlines, ax.n, p, tws, sgn); break;
+            case 5: stockham_inplace<5, kPlaneElems / 5>(buf, se, sl, nlines, ax.n, p, tws, sgn); break;
+            default: stockham_inplace<8, kPlaneElems / 8>(buf, se, sl, nlines, ax.n, p, tws, sgn); break;
+        }
+        p *= R;
+    }
 }
 
 }  // namespace hymd

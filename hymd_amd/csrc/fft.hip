@@ -71,10 +71,23 @@ int fft_plan_init(FftPlan &plan, const Mesh &mesh) {
     plan.wx = lines_for(mesh.mx);
     plan.wy = lines_for(mesh.my);
     plan.wz = std::max(1, 2048 / (mesh.mz / 2));
-    plan.lds_x = sizeof(float2) * (size_t)(plan.ax.n + 2 * plan.wx * plan.ax.ld);
-    plan.lds_y = sizeof(float2) * (size_t)(plan.ay.n + 2 * plan.wy * plan.ay.ld);
+    // strided passes: twiddles, two line buffers, W int64 line offsets
+    plan.lds_x = sizeof(float2) * (size_t)(plan.ax.n + 2 * plan.wx * plan.ax.ld) + sizeof(int64_t) * plan.wx;
+    plan.lds_y = sizeof(float2) * (size_t)(plan.ay.n + 2 * plan.wy * plan.ay.ld) + sizeof(int64_t) * plan.wy;
     plan.lds_z = sizeof(float2) * (size_t)(plan.azh.n + mesh.zh + 2 * plan.wz * plan.azh.ld);
-    const size_t lds_max = 64 * 1024;  // default dynamic-LDS limit without an opt-in attribute
+    // y-z plane kernels: twiddles (y, z half), post twiddles, one My x zh plane;
+    // every in-place stage must fit kPlaneElems values per thread
+    plan.lds_plane = sizeof(float2) * ((size_t)mesh.my + (size_t)plan.azh.n + 2 * (size_t)mesh.zh +
+                                       (size_t)mesh.my * mesh.zh);
+    auto fits = [](const FftAxis &ax, int nlines) {
+        for (int st = 0; st < ax.nstages; ++st) {
+            const int R = ax.radix[st];
+            if ((int64_t)nlines * ax.n > (int64_t)kPlaneThreads * (kPlaneElems / R) * R) return false;
+        }
+        return true;
+    };
+    if (plan.lds_plane > kPlaneLdsBytes || !fits(plan.ay, mesh.zh) || !fits(plan.azh, mesh.my)) plan.lds_plane = 0;
+    const size_t lds_max = 64 * 1024;  // the line kernels stay within the default dynamic-LDS limit
     if (plan.lds_x > lds_max || plan.lds_y > lds_max || plan.lds_z > lds_max) {
         set_error("mesh too large for the LDS-staged FFT");
         return HYMD_ERR_UNSUPPORTED;
@@ -87,12 +100,10 @@ int fft_plan_init(FftPlan &plan, const Mesh &mesh) {
 // Mz/2 complex z[n] = x[2n] + i x[2n+1]; FFT; split
 //   X[k] = (Z[k] + Z*[Nh-k])/2 + e^{-2 pi i k/Mz} (Z[k] - Z*[Nh-k])/(2i).
 // ---------------------------------------------------------------------------
-template <bool FIXED>
-__global__ void __launch_bounds__(kBlock) r2c_z_kernel(const void *__restrict__ in_, float2 *__restrict__ out,
+__global__ void __launch_bounds__(kBlock) r2c_z_kernel(const float *__restrict__ in_, float2 *__restrict__ out,
                                                         int64_t rows, int mz, int zh, FftAxis ax,
                                                         const float2 *__restrict__ tw_glob,
-                                                        const float2 *__restrict__ post_glob, double scale, int W,
-                                                        int clear) {
+                                                        const float2 *__restrict__ post_glob, int W) {
     extern __shared__ float2 smem[];
     const int nh = ax.n;
     float2 *tw = smem;
@@ -101,37 +112,43 @@ __global__ void __launch_bounds__(kBlock) r2c_z_kernel(const void *__restrict__ 
     float2 *B = A + W * ax.ld;
     for (int i = threadIdx.x; i < nh - 1; i += blockDim.x) tw[i] = tw_glob[i];
     for (int i = threadIdx.x; i < zh; i += blockDim.x) post[i] = post_glob[i];
-    const int64_t row0 = (int64_t)blockIdx.x * W;
-    const int64_t bofs = (int64_t)blockIdx.y * rows;
-    for (int idx = threadIdx.x; idx < W * nh; idx += blockDim.x) {
-        const int l = idx / nh, e = idx - l * nh;
-        const int64_t row = row0 + l;
-        float2 z = make_float2(0.f, 0.f);
-        if (row < rows) {
-            const int64_t off = (bofs + row) * mz + 2 * e;
-            if (FIXED) {
-                long long *p = (long long *)in_ + off;
-                long long q0 = p[0], q1 = p[1];
-                z = make_float2((float)((double)q0 * scale), (float)((double)q1 * scale));
-                if (clear) { p[0] = 0; p[1] = 0; }
-            } else {
-                z = *reinterpret_cast<const float2 *>((const float *)in_ + off);
+    const unsigned flat = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const unsigned bx = flat % gridDim.x, by = flat / gridDim.x;
+    const int64_t row0 = (int64_t)bx * W;
+    const int64_t bofs = (int64_t)by * rows;
+    // the W rows are contiguous: element idx of the block is complex z[idx % nh] of row idx / nh
+    const int64_t nvalid = (int64_t)(rows - row0 < W ? rows - row0 : W) * nh;
+    const int64_t gofs = (bofs + row0) * nh;  // in complex (pairs of reals) units
+    const int total = W * nh;
+    for (int i0 = threadIdx.x; i0 < total; i0 += kBlock * kLoadBatch) {
+        float2 v[kLoadBatch];
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kBlock;
+            v[j] = make_float2(0.f, 0.f);
+            if (idx < nvalid) v[j] = reinterpret_cast<const float2 *>(in_)[gofs + idx];
+        }
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kBlock;
+            if (idx < total) {
+                const int l = idx / nh;
+                A[l * ax.ld + (idx - l * nh)] = v[j];
             }
         }
-        A[l * ax.ld + e] = z;
     }
     __syncthreads();
     float2 *Z = fft_lds(A, B, W, ax, tw, -1.f);
-    for (int idx = threadIdx.x; idx < W * zh; idx += blockDim.x) {
+    const int64_t nout = (int64_t)(rows - row0 < W ? rows - row0 : W) * zh;
+    float2 *o = out + (bofs + row0) * zh;
+    for (int idx = threadIdx.x; idx < nout; idx += blockDim.x) {
         const int l = idx / zh, k = idx - l * zh;
-        const int64_t row = row0 + l;
-        if (row >= rows) continue;
         const float2 *zr = Z + l * ax.ld;
         float2 zk = zr[k == nh ? 0 : k];
         float2 zc = cconj(zr[k == 0 ? 0 : nh - k]);
         float2 a = cscale(cadd(zk, zc), 0.5f);
         float2 b = cmul_i(csub(zk, zc), -0.5f);
-        out[(bofs + row) * zh + k] = cadd(a, cmul(post[k], b));
+        o[idx] = cadd(a, cmul(post[k], b));
     }
 }
 
@@ -149,12 +166,25 @@ __global__ void __launch_bounds__(kBlock) c2r_z_kernel(const float2 *__restrict_
     float2 *B = A + W * ax.ld;
     for (int i = threadIdx.x; i < nh - 1; i += blockDim.x) tw[i] = tw_glob[i];
     for (int i = threadIdx.x; i < zh; i += blockDim.x) post[i] = post_glob[i];
-    const int64_t row0 = (int64_t)blockIdx.x * W;
-    const int64_t bofs = (int64_t)blockIdx.y * rows;
-    for (int idx = threadIdx.x; idx < W * zh; idx += blockDim.x) {
-        const int l = idx / zh, k = idx - l * zh;
-        const int64_t row = row0 + l;
-        A[l * ax.ld + k] = (row < rows) ? in[(bofs + row) * zh + k] : make_float2(0.f, 0.f);
+    const unsigned flat = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const unsigned bx = flat % gridDim.x, by = flat / gridDim.x;
+    const int64_t row0 = (int64_t)bx * W;
+    const int64_t bofs = (int64_t)by * rows;
+    const int wv = (int)(rows - row0 < W ? rows - row0 : W);
+    const float2 *src = in + (bofs + row0) * zh;   // W contiguous half-spectrum rows (ld = zh)
+    const int nvalid = wv * zh, total = W * zh;
+    for (int i0 = threadIdx.x; i0 < total; i0 += kBlock * kLoadBatch) {
+        float2 v[kLoadBatch];
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kBlock;
+            v[j] = idx < nvalid ? src[idx] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kBlock;
+            if (idx < total) A[idx] = v[j];
+        }
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < W * nh; idx += blockDim.x) {
@@ -168,101 +198,284 @@ __global__ void __launch_bounds__(kBlock) c2r_z_kernel(const float2 *__restrict_
     }
     __syncthreads();
     float2 *z = fft_lds(B, A, W, ax, tw, 1.f);
-    for (int idx = threadIdx.x; idx < W * nh; idx += blockDim.x) {
+    float2 *dst = reinterpret_cast<float2 *>(out) + (bofs + row0) * nh;
+    for (int idx = threadIdx.x; idx < wv * nh; idx += blockDim.x) {
         const int l = idx / nh, e = idx - l * nh;
-        const int64_t row = row0 + l;
-        if (row >= rows) continue;
-        float2 v = z[l * ax.ld + e];
-        *reinterpret_cast<float2 *>(out + (bofs + row) * mz + 2 * e) = make_float2(v.x * scale, v.y * scale);
+        const float2 v = z[l * ax.ld + e];
+        dst[idx] = make_float2(v.x * scale, v.y * scale);
     }
 }
 
 // In-place complex pass along a strided axis.  Line l -> base (l / inner) *
-// outer + (l % inner), element e at base + e * es.  W consecutive lines per
-// workgroup (consecutive in memory except at `inner` boundaries).
+// outer + (l % inner), element e at base + e * es.  2^wshift consecutive
+// lines per workgroup (consecutive in memory except at `inner` boundaries).
 __global__ void __launch_bounds__(kBlock) fft_strided_kernel(float2 *__restrict__ data, int64_t batch_stride,
-                                                              int64_t nlines, int64_t inner, int64_t outer,
-                                                              int64_t es, FftAxis ax,
-                                                              const float2 *__restrict__ tw_glob, float sgn, int W) {
+                                                              int nlines, int inner, int64_t outer, int64_t es,
+                                                              FftAxis ax, const float2 *__restrict__ tw_glob,
+                                                              float sgn, int wshift) {
     extern __shared__ float2 smem[];
     const int n = ax.n;
+    const int W = 1 << wshift;
     float2 *tw = smem;
     float2 *A = tw + n;
     float2 *B = A + W * ax.ld;
+    int64_t *lbase = reinterpret_cast<int64_t *>(B + W * ax.ld);  // W line offsets (-1: past the end)
     for (int i = threadIdx.x; i < n - 1; i += blockDim.x) tw[i] = tw_glob[i];
-    float2 *d = data + (int64_t)blockIdx.y * batch_stride;
-    const int64_t line0 = (int64_t)blockIdx.x * W;
-    for (int idx = threadIdx.x; idx < W * n; idx += blockDim.x) {
-        const int e = idx / W, l = idx - e * W;
-        const int64_t line = line0 + l;
-        float2 v = make_float2(0.f, 0.f);
-        if (line < nlines) v = d[(line / inner) * outer + (line % inner) + e * es];
-        A[l * ax.ld + e] = v;
+    const unsigned flat = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const unsigned bx = flat % gridDim.x, by = flat / gridDim.x;
+    float2 *d = data + (int64_t)by * batch_stride;
+    const int line0 = (int)bx * W;
+    if ((int)threadIdx.x < W) {
+        const int line = line0 + (int)threadIdx.x;
+        lbase[threadIdx.x] = line < nlines ? (int64_t)(line / inner) * outer + (line % inner) : -1;
+    }
+    __syncthreads();
+    const int nel = W * n;
+    for (int i0 = threadIdx.x; i0 < nel; i0 += kBlock * kLoadBatch) {
+        float2 v[kLoadBatch];
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kBlock;
+            v[j] = make_float2(0.f, 0.f);
+            if (idx < nel) {
+                const int64_t b = lbase[idx & (W - 1)];
+                if (b >= 0) v[j] = d[b + (int64_t)(idx >> wshift) * es];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kBlock;
+            if (idx < nel) A[(idx & (W - 1)) * ax.ld + (idx >> wshift)] = v[j];
+        }
     }
     __syncthreads();
     float2 *r = fft_lds(A, B, W, ax, tw, sgn);
-    for (int idx = threadIdx.x; idx < W * n; idx += blockDim.x) {
-        const int e = idx / W, l = idx - e * W;
-        const int64_t line = line0 + l;
-        if (line < nlines) d[(line / inner) * outer + (line % inner) + e * es] = r[l * ax.ld + e];
+    for (int idx = threadIdx.x; idx < nel; idx += blockDim.x) {
+        const int l = idx & (W - 1), e = idx >> wshift;
+        const int64_t b = lbase[l];
+        if (b >= 0) d[b + (int64_t)e * es] = r[l * ax.ld + e];
     }
+}
+
+// ---------------------------------------------------------------------------
+// y-z plane kernels: one workgroup per (x-plane, grid) holds the plane's My x
+// (Mz/2+1) half spectrum in LDS and runs the z and y passes in place, so a
+// plane crosses HBM once per direction instead of twice.
+// ---------------------------------------------------------------------------
+
+// forward: My rows of Mz reals -> packed complex FFT of Mz/2 per row + split
+// (as r2c_z_kernel) -> FFT along y of the Mz/2+1 columns -> out plane
+__global__ void __launch_bounds__(kPlaneThreads) r2c_zy_plane_kernel(const float *__restrict__ in,
+                                                                      float2 *__restrict__ out, int mx, int my,
+                                                                      int mz, FftAxis ay, FftAxis azh,
+                                                                      const float2 *__restrict__ twy_g,
+                                                                      const float2 *__restrict__ twz_g,
+                                                                      const float2 *__restrict__ post_g) {
+    extern __shared__ float2 smem[];
+    const int nh = azh.n, zh = nh + 1;
+    float2 *twy = smem;
+    float2 *twz = twy + my;
+    float2 *post = twz + nh;
+    float2 *D = post + zh;  // my x zh, row stride zh
+    for (int i = threadIdx.x; i < my - 1; i += blockDim.x) twy[i] = twy_g[i];
+    for (int i = threadIdx.x; i < nh - 1; i += blockDim.x) twz[i] = twz_g[i];
+    for (int i = threadIdx.x; i < zh; i += blockDim.x) post[i] = post_g[i];
+    const unsigned flat = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int64_t plane = (int64_t)(flat / gridDim.x) * mx + (flat % gridDim.x);  // grid * mx + x
+    // rows of Mz reals as nh complex, row stride zh in D
+    const float2 *src = reinterpret_cast<const float2 *>(in) + plane * my * nh;
+    const int nel = my * nh;
+    for (int i0 = threadIdx.x; i0 < nel; i0 += kPlaneThreads * kLoadBatch) {
+        float2 v[kLoadBatch];
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kPlaneThreads;
+            v[j] = idx < nel ? src[idx] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kPlaneThreads;
+            if (idx < nel) {
+                const int y = idx / nh;
+                D[y * zh + (idx - y * nh)] = v[j];
+            }
+        }
+    }
+    __syncthreads();
+    fft_inplace(D, 1, zh, my, azh, twz, -1.f);
+    // split in place: the thread of pair (k, nh - k) writes X[k] and X[nh - k]
+    // (k = 0 writes X[0] and X[nh]; k = nh/2 is its own pair)
+    const int npairs = nh / 2 + 1;
+    for (int idx = threadIdx.x; idx < my * npairs; idx += blockDim.x) {
+        const int y = idx / npairs, k = idx - y * npairs;
+        float2 *row = D + y * zh;
+        const int kc = k == 0 ? 0 : nh - k;
+        const float2 zk = row[k], zc = row[kc];
+        // X[k] = (Z[k] + Z*[nh-k])/2 + post[k] (Z[k] - Z*[nh-k])/(2i), Z[nh] := Z[0]
+        const float2 a1 = cscale(cadd(zk, cconj(zc)), 0.5f);
+        const float2 b1 = cmul_i(csub(zk, cconj(zc)), -0.5f);
+        const float2 xk = cadd(a1, cmul(post[k], b1));
+        const int km = k == 0 ? nh : kc;  // the partner output index
+        const float2 a2 = cscale(cadd(zc, cconj(zk)), 0.5f);
+        const float2 b2 = cmul_i(csub(zc, cconj(zk)), -0.5f);
+        const float2 xm = cadd(a2, cmul(post[km], b2));
+        row[k] = xk;
+        if (km != k) row[km] = xm;
+    }
+    __syncthreads();
+    fft_inplace(D, zh, 1, zh, ay, twy, -1.f);
+    float2 *dst = out + plane * my * zh;
+    for (int idx = threadIdx.x; idx < my * zh; idx += blockDim.x) dst[idx] = D[idx];
+}
+
+// inverse: FFT along y of the Mz/2+1 columns -> per row Z'[k] = (X[k] + X*[nh-k])
+// + i e^{2 pi i k/Mz} (X[k] - X*[nh-k]) (as c2r_z_kernel) -> inverse FFT of
+// Mz/2 -> My rows of Mz reals, times `scale`
+__global__ void __launch_bounds__(kPlaneThreads) c2r_yz_plane_kernel(const float2 *__restrict__ in,
+                                                                      float *__restrict__ out, int mx, int my,
+                                                                      int mz, FftAxis ay, FftAxis azh,
+                                                                      const float2 *__restrict__ twy_g,
+                                                                      const float2 *__restrict__ twz_g,
+                                                                      const float2 *__restrict__ post_g,
+                                                                      float scale) {
+    extern __shared__ float2 smem[];
+    const int nh = azh.n, zh = nh + 1;
+    float2 *twy = smem;
+    float2 *twz = twy + my;
+    float2 *post = twz + nh;
+    float2 *D = post + zh;  // my x zh, row stride zh
+    for (int i = threadIdx.x; i < my - 1; i += blockDim.x) twy[i] = twy_g[i];
+    for (int i = threadIdx.x; i < nh - 1; i += blockDim.x) twz[i] = twz_g[i];
+    for (int i = threadIdx.x; i < zh; i += blockDim.x) post[i] = post_g[i];
+    const unsigned flat = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int64_t plane = (int64_t)(flat / gridDim.x) * mx + (flat % gridDim.x);
+    const float2 *src = in + plane * my * zh;
+    const int nel = my * zh;
+    for (int i0 = threadIdx.x; i0 < nel; i0 += kPlaneThreads * kLoadBatch) {
+        float2 v[kLoadBatch];
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kPlaneThreads;
+            v[j] = idx < nel ? src[idx] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kPlaneThreads;
+            if (idx < nel) D[idx] = v[j];
+        }
+    }
+    __syncthreads();
+    fft_inplace(D, zh, 1, zh, ay, twy, 1.f);
+    // pre-process in place, pairs (k, nh - k) per thread; k = 0 reads X[nh]
+    const int npairs = nh / 2 + 1;
+    for (int idx = threadIdx.x; idx < my * npairs; idx += blockDim.x) {
+        const int y = idx / npairs, k = idx - y * npairs;
+        float2 *row = D + y * zh;
+        const int kc = nh - k;  // k = 0 -> X[nh]
+        const float2 xk = row[k], xc = row[kc];
+        const float2 s1 = cadd(xk, cconj(xc));
+        const float2 d1 = cmul(cconj(post[k]), csub(xk, cconj(xc)));
+        const float2 zk = cadd(s1, cmul_i(d1, 1.f));
+        if (k != 0 && kc != k) {
+            const float2 s2 = cadd(xc, cconj(xk));
+            const float2 d2 = cmul(cconj(post[kc]), csub(xc, cconj(xk)));
+            row[kc] = cadd(s2, cmul_i(d2, 1.f));
+        }
+        row[k] = zk;
+    }
+    __syncthreads();
+    fft_inplace(D, 1, zh, my, azh, twz, 1.f);
+    float2 *dst = reinterpret_cast<float2 *>(out) + plane * my * nh;
+    for (int idx = threadIdx.x; idx < my * nh; idx += blockDim.x) {
+        const int y = idx / nh;
+        const float2 v = D[y * zh + (idx - y * nh)];
+        dst[idx] = make_float2(v.x * scale, v.y * scale);
+    }
+}
+
+static int log2_exact(int w) {
+    int s = 0;
+    while ((1 << s) < w) ++s;
+    return s;
 }
 
 static int launch_strided(const FftPlan &plan, float2 *data, int batch, int axis, float sgn, hipStream_t s) {
     const Mesh &m = plan.mesh;
     if (axis == 1) {  // y: lines (x, kz), stride zh
-        const int64_t nlines = (int64_t)m.mx * m.zh;
+        const int nlines = m.mx * m.zh;
         dim3 grid((unsigned)((nlines + plan.wy - 1) / plan.wy), (unsigned)batch);
         fft_strided_kernel<<<grid, kBlock, plan.lds_y, s>>>(data, m.gc, nlines, m.zh, (int64_t)m.my * m.zh, m.zh,
-                                                             plan.ay, (const float2 *)plan.tw_y.ptr, sgn, plan.wy);
+                                                             plan.ay, (const float2 *)plan.tw_y.ptr, sgn,
+                                                             log2_exact(plan.wy));
     } else {  // x: lines (y, kz) contiguous, stride my*zh
-        const int64_t nlines = (int64_t)m.my * m.zh;
+        const int nlines = m.my * m.zh;
         dim3 grid((unsigned)((nlines + plan.wx - 1) / plan.wx), (unsigned)batch);
         fft_strided_kernel<<<grid, kBlock, plan.lds_x, s>>>(data, m.gc, nlines, nlines, 0, nlines, plan.ax,
-                                                             (const float2 *)plan.tw_x.ptr, sgn, plan.wx);
+                                                             (const float2 *)plan.tw_x.ptr, sgn,
+                                                             log2_exact(plan.wx));
     }
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }
 
-int fft_forward_xy(const FftPlan &plan, float2 *data, int batch, hipStream_t s) {
-    HYMD_TRY(launch_strided(plan, data, batch, 1, -1.f, s));
-    HYMD_TRY(launch_strided(plan, data, batch, 0, -1.f, s));
+// Opt in to more than 64 KiB of dynamic LDS for the plane kernels (once).
+static int plane_lds_opt_in() {
+    static bool done = false;
+    if (!done) {
+        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&r2c_zy_plane_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLdsBytes));
+        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&c2r_yz_plane_kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLdsBytes));
+        done = true;
+    }
     return HYMD_OK;
 }
 
-int fft_inverse_xy(const FftPlan &plan, float2 *data, int batch, hipStream_t s) {
-    HYMD_TRY(launch_strided(plan, data, batch, 0, 1.f, s));
-    HYMD_TRY(launch_strided(plan, data, batch, 1, 1.f, s));
-    return HYMD_OK;
+int fft_forward_x(const FftPlan &plan, float2 *data, int batch, hipStream_t s) {
+    return launch_strided(plan, data, batch, 0, -1.f, s);
 }
 
-int fft_r2c_fixed(const FftPlan &plan, int64_t *in, float2 *out, int batch, double scale, bool clear,
-                  hipStream_t s) {
+int fft_inverse_x(const FftPlan &plan, float2 *data, int batch, hipStream_t s) {
+    return launch_strided(plan, data, batch, 0, 1.f, s);
+}
+
+int fft_r2c_zy(const FftPlan &plan, const float *in, float2 *out, int batch, hipStream_t s) {
     const Mesh &m = plan.mesh;
+    if (plan.lds_plane) {
+        HYMD_TRY(plane_lds_opt_in());
+        dim3 grid((unsigned)m.mx, (unsigned)batch);
+        r2c_zy_plane_kernel<<<grid, kPlaneThreads, plan.lds_plane, s>>>(
+            in, out, m.mx, m.my, m.mz, plan.ay, plan.azh, (const float2 *)plan.tw_y.ptr,
+            (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr);
+        HYMD_HIP(hipGetLastError());
+        return HYMD_OK;
+    }
     const int64_t rows = (int64_t)m.mx * m.my;
     dim3 grid((unsigned)((rows + plan.wz - 1) / plan.wz), (unsigned)batch);
-    r2c_z_kernel<true><<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, m.mz, m.zh, plan.azh,
-                                                         (const float2 *)plan.tw_zh.ptr,
-                                                         (const float2 *)plan.post.ptr, scale, plan.wz,
-                                                         clear ? 1 : 0);
+    r2c_z_kernel<<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, m.mz, m.zh, plan.azh,
+                                                   (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr,
+                                                   plan.wz);
     HYMD_HIP(hipGetLastError());
-    return fft_forward_xy(plan, out, batch, s);
+    return launch_strided(plan, out, batch, 1, -1.f, s);
 }
 
 int fft_r2c_real(const FftPlan &plan, const float *in, float2 *out, int batch, hipStream_t s) {
-    const Mesh &m = plan.mesh;
-    const int64_t rows = (int64_t)m.mx * m.my;
-    dim3 grid((unsigned)((rows + plan.wz - 1) / plan.wz), (unsigned)batch);
-    r2c_z_kernel<false><<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, m.mz, m.zh, plan.azh,
-                                                          (const float2 *)plan.tw_zh.ptr,
-                                                          (const float2 *)plan.post.ptr, 1.0, plan.wz, 0);
-    HYMD_HIP(hipGetLastError());
-    return fft_forward_xy(plan, out, batch, s);
+    HYMD_TRY(fft_r2c_zy(plan, in, out, batch, s));
+    return fft_forward_x(plan, out, batch, s);
 }
 
-int fft_c2r_z(const FftPlan &plan, const float2 *in, float *out, int batch, float scale, hipStream_t s) {
+int fft_c2r_yz(const FftPlan &plan, float2 *in, float *out, int batch, float scale, hipStream_t s) {
     const Mesh &m = plan.mesh;
+    if (plan.lds_plane) {
+        HYMD_TRY(plane_lds_opt_in());
+        dim3 grid((unsigned)m.mx, (unsigned)batch);
+        c2r_yz_plane_kernel<<<grid, kPlaneThreads, plan.lds_plane, s>>>(
+            in, out, m.mx, m.my, m.mz, plan.ay, plan.azh, (const float2 *)plan.tw_y.ptr,
+            (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr, scale);
+        HYMD_HIP(hipGetLastError());
+        return HYMD_OK;
+    }
+    HYMD_TRY(launch_strided(plan, in, batch, 1, 1.f, s));
     const int64_t rows = (int64_t)m.mx * m.my;
     dim3 grid((unsigned)((rows + plan.wz - 1) / plan.wz), (unsigned)batch);
     c2r_z_kernel<<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, m.mz, m.zh, plan.azh,
@@ -270,6 +483,11 @@ int fft_c2r_z(const FftPlan &plan, const float2 *in, float *out, int batch, floa
                                                    scale, plan.wz);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
+}
+
+int fft_c2r(const FftPlan &plan, float2 *data, float *out, int batch, float scale, hipStream_t s) {
+    HYMD_TRY(fft_inverse_x(plan, data, batch, s));
+    return fft_c2r_yz(plan, data, out, batch, scale, s);
 }
 
 }  // namespace hymd

@@ -14,18 +14,13 @@
 #include <cmath>
 
 #include "fft.h"
+#include "field_dev.h"
 
 namespace hymd {
 
 // ---------------------------------------------------------------------------
 // paint
 // ---------------------------------------------------------------------------
-struct PaintArgs {
-    int n_types;
-    double scale;  // 2^HYMD_FIXED_SHIFT
-    double mass[HYMD_MAX_TYPES];
-};
-
 // One thread per particle; each corner deposits round(w*m*2^32) with a 64-bit
 // integer atomic (order-independent => bit-reproducible grids).  The 8th
 // corner takes the remainder so each particle deposits exactly round(m*2^32).
@@ -80,36 +75,39 @@ __global__ void zero_kernel(unsigned long long *__restrict__ p, int64_t n) {
 }
 
 // ---------------------------------------------------------------------------
-// Spatial sort: particles binned by their CIC base cell (counting sort), then
-// paint as a gather over cells and readout in cell order.
+// Spatial binning: particles are counting-sorted by the mesh row (ix, iy) of
+// their CIC base cell, so the particles whose corners can land in a row sit
+// in four contiguous ranges.  Paint is then a per-row gather, readout walks
+// the particles row by row.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int base_cell(const float *__restrict__ x, int64_t i, const Mesh &m) {
+__device__ __forceinline__ int base_row(const float *__restrict__ x, int64_t i, const Mesh &m) {
     const CicAxis cx = cic_axis((double)x[3 * i + 0], m.inv_h[0], m.mx);
     const CicAxis cy = cic_axis((double)x[3 * i + 1], m.inv_h[1], m.my);
-    const CicAxis cz = cic_axis((double)x[3 * i + 2], m.inv_h[2], m.mz);
-    return (cx.i0 * m.my + cy.i0) * m.mz + cz.i0;
+    return cx.i0 * m.my + cy.i0;
 }
 
-__global__ void __launch_bounds__(kBlock) cell_key_kernel(const float *__restrict__ x, int64_t n, Mesh m,
-                                                           int *__restrict__ keys, int *__restrict__ counts) {
+// key = mesh row of the particle's base cell; slot = its arrival rank in
+// that row (a returning atomic), so the scatter needs no second atomic.
+__global__ void __launch_bounds__(kBlock) row_key_kernel(const float *__restrict__ x, int64_t n, Mesh m,
+                                                          int2 *__restrict__ keys, int *__restrict__ counts) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int key = base_cell(x, i, m);
-        keys[i] = key;
-        atomicAdd(counts + key, 1);
+        const int key = base_row(x, i, m);
+        keys[i] = make_int2(key, atomicAdd(counts + key, 1));
     }
 }
 
-__global__ void __launch_bounds__(kBlock) cell_scatter_kernel(const float *__restrict__ x,
-                                                               const int32_t *__restrict__ types,
-                                                               const float *__restrict__ w, int64_t n,
-                                                               const int *__restrict__ keys,
-                                                               int *__restrict__ cursor, int *__restrict__ perm,
-                                                               float *__restrict__ xs, int32_t *__restrict__ ts,
-                                                               float *__restrict__ ws) {
+__global__ void __launch_bounds__(kBlock) key_scatter_kernel(const float *__restrict__ x,
+                                                              const int32_t *__restrict__ types,
+                                                              const float *__restrict__ w, int64_t n,
+                                                              const int2 *__restrict__ keys,
+                                                              const int *__restrict__ start, int *__restrict__ perm,
+                                                              float *__restrict__ xs, int32_t *__restrict__ ts,
+                                                              float *__restrict__ ws) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int pos = atomicAdd(cursor + keys[i], 1);
+        const int2 kr = keys[i];
+        const int pos = start[kr.x] + kr.y;
         perm[pos] = (int)i;
         xs[3 * (int64_t)pos] = x[3 * i];
         xs[3 * (int64_t)pos + 1] = x[3 * i + 1];
@@ -119,128 +117,120 @@ __global__ void __launch_bounds__(kBlock) cell_scatter_kernel(const float *__res
     }
 }
 
-// One thread per cell: sum the fixed-point corner deposits of the particles
-// binned in the 8 cells whose upper corners include this one (the same int64
-// values paint_fixed_kernel adds atomically, so the sums are identical and
-// independent of order), then write rho = sum * 2^-32 / dV per type.
-template <int MAXT>
-__global__ void __launch_bounds__(kBlock) paint_gather_kernel(const float *__restrict__ xs,
-                                                               const int32_t *__restrict__ ts,
-                                                               const float *__restrict__ ws,
-                                                               const int *__restrict__ start, Mesh m, PaintArgs pa,
-                                                               double out_scale, float *__restrict__ out,
-                                                               long long *__restrict__ out_fixed) {
+int launch_key_scatter(const float *x, const int32_t *types, const float *w, int64_t n, const int2 *keys,
+                       const int *start, int *perm, float *xs, int32_t *ts, float *ws, hipStream_t s) {
+    if (n > 0) key_scatter_kernel<<<grid_for(n), kBlock, 0, s>>>(x, types, w, n, keys, start, perm, xs, ts, ws);
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
+}
+
+// One workgroup per mesh row (ix, iy) and chunk of types [t0, t0 + tc): the
+// particles binned in rows (ix - a, iy - b), a, b in {0, 1}, add their corner
+// deposits (a, b, d) at z = iz + d of this row into int64 LDS accumulators.
+// The deposits are the fixed-point values of the scatter paint (FP64 weight
+// products, contraction off; the 8th corner takes the remainder so every
+// particle deposits exactly round(m 2^32)), and integer addition makes the
+// sums independent of order.  The row is then written once per type as
+// rho = sum * out_scale (and the int64 sums if out_fixed != NULL).
+__global__ void __launch_bounds__(kBlock) paint_rows_kernel(const float *__restrict__ xs,
+                                                             const int32_t *__restrict__ ts,
+                                                             const float *__restrict__ ws,
+                                                             const int *__restrict__ start, Mesh m, PaintArgs pa,
+                                                             int tc, double out_scale, float *__restrict__ out,
+                                                             long long *__restrict__ out_fixed) {
 #pragma clang fp contract(off)
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; cell < m.g; cell += stride) {
-        const int iz = (int)(cell % m.mz);
-        const int iy = (int)((cell / m.mz) % m.my);
-        const int ix = (int)(cell / ((int64_t)m.mz * m.my));
-        long long acc[MAXT];
+    extern __shared__ unsigned long long acc[];  // tc * mz
+    const int row = (int)xcd_block(blockIdx.x, gridDim.x);
+    const int t0 = (int)blockIdx.y * tc;
+    const int nt = min(tc, pa.n_types - t0);
+    const int ix = row / m.my, iy = row - ix * m.my;
+    for (int i = threadIdx.x; i < nt * m.mz; i += blockDim.x) acc[i] = 0ull;
+    int rs[4], rn[4];
 #pragma unroll
-        for (int t = 0; t < MAXT; ++t) acc[t] = 0;
-        for (int c = 0; c < 8; ++c) {
-            const int a = c >> 2, b = (c >> 1) & 1, d = c & 1;
-            const int bx = a ? (ix == 0 ? m.mx - 1 : ix - 1) : ix;
-            const int by = b ? (iy == 0 ? m.my - 1 : iy - 1) : iy;
-            const int bz = d ? (iz == 0 ? m.mz - 1 : iz - 1) : iz;
-            const int bin = (bx * m.my + by) * m.mz + bz;
-            for (int q = start[bin]; q < start[bin + 1]; ++q) {
-                const int t = ts[q];
-                if ((unsigned)t >= (unsigned)pa.n_types) continue;
-                const double mass = ws ? (double)ws[q] : pa.mass[t];
-                const CicAxis cx = cic_axis((double)xs[3 * (int64_t)q + 0], m.inv_h[0], m.mx);
-                const CicAxis cy = cic_axis((double)xs[3 * (int64_t)q + 1], m.inv_h[1], m.my);
-                const CicAxis cz = cic_axis((double)xs[3 * (int64_t)q + 2], m.inv_h[2], m.mz);
-                long long qc;
-                if (c < 7) {
-                    const double wgt = ((a ? cx.w1 : cx.w0) * (b ? cy.w1 : cy.w0)) * (d ? cz.w1 : cz.w0);
-                    qc = __double2ll_rn((wgt * mass) * pa.scale);
-                } else {
-                    long long s7 = 0;
-#pragma unroll
-                    for (int e = 0; e < 7; ++e) {
-                        const int ea = e >> 2, eb = (e >> 1) & 1, ed = e & 1;
-                        const double wgt = ((ea ? cx.w1 : cx.w0) * (eb ? cy.w1 : cy.w0)) * (ed ? cz.w1 : cz.w0);
-                        s7 += __double2ll_rn((wgt * mass) * pa.scale);
-                    }
-                    qc = __double2ll_rn(mass * pa.scale) - s7;
-                }
-#pragma unroll
-                for (int u = 0; u < MAXT; ++u)
-                    if (u == t) acc[u] += qc;
-            }
+    for (int q = 0; q < 4; ++q) {
+        const int a = q >> 1, b = q & 1;
+        const int bx = a ? (ix == 0 ? m.mx - 1 : ix - 1) : ix;
+        const int by = b ? (iy == 0 ? m.my - 1 : iy - 1) : iy;
+        const int br = bx * m.my + by;
+        rs[q] = start[br];
+        rn[q] = start[br + 1] - rs[q];
+    }
+    __syncthreads();
+    const int tot = rn[0] + rn[1] + rn[2] + rn[3];
+    for (int v = threadIdx.x; v < tot; v += blockDim.x) {
+        int q = 0, off = v;
+        while (q < 3 && off >= rn[q]) {
+            off -= rn[q];
+            ++q;
         }
-#pragma unroll
-        for (int t = 0; t < MAXT; ++t) {
-            if (t >= pa.n_types) break;
-            out[(int64_t)t * m.g + cell] = (float)((double)acc[t] * out_scale);
-            if (out_fixed) out_fixed[(int64_t)t * m.g + cell] = acc[t];
-        }
+        const int p = rs[q] + off;
+        const int t = ts[p] - t0;
+        if ((unsigned)t >= (unsigned)nt) continue;
+        const int a = q >> 1, b = q & 1;
+        const double mass = ws ? (double)ws[p] : pa.mass[t0 + t];
+        row_deposit(xs + 3 * (int64_t)p, a, b, mass, pa.scale, m.inv_h, m.mx, m.my, m.mz, acc + t * m.mz);
+    }
+    __syncthreads();
+    const int64_t rbase = (int64_t)row * m.mz;
+    for (int i = threadIdx.x; i < nt * m.mz; i += blockDim.x) {
+        const int t = i / m.mz, z = i - t * m.mz;
+        const int64_t o = (int64_t)(t0 + t) * m.g + rbase + z;
+        const long long v = (long long)acc[i];
+        out[o] = (float)((double)v * out_scale);
+        if (out_fixed) out_fixed[o] = v;
     }
 }
 
 // ---------------------------------------------------------------------------
 // readout: out[i, c] = w_i * sum_corners W * grid[type_i * ncomp + c][corner]
 // ---------------------------------------------------------------------------
-// perm != NULL: particle i of the (sorted) inputs is written to row perm[i]
-template <typename PosT>
-__global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict__ grids, int ncomp,
-                                                          const PosT *__restrict__ x,
+// perm != NULL: particle i of the (sorted) inputs is written to row perm[i].
+// Blocks are XCD-remapped so that consecutive (cell-sorted) particle ranges,
+// which read the same grid rows, share one L2.
+template <int NCOMP>
+__global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict__ grids,
+                                                          const float *__restrict__ x,
                                                           const int32_t *__restrict__ types,
                                                           const float *__restrict__ scale_per_particle,
                                                           int64_t n, int n_types, Mesh m,
                                                           float *__restrict__ out, const int *__restrict__ perm) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int64_t first = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    for (int64_t i = first; i < n; i += stride) {
         const int t = types ? types[i] : 0;
         const int64_t o = perm ? (int64_t)perm[i] : i;
         if ((unsigned)t >= (unsigned)n_types) {
-            for (int c = 0; c < ncomp; ++c) out[ncomp * o + c] = 0.f;
+#pragma unroll
+            for (int c = 0; c < NCOMP; ++c) out[NCOMP * o + c] = 0.f;
             continue;
         }
         const CicAxis cx = cic_axis((double)x[3 * i + 0], m.inv_h[0], m.mx);
         const CicAxis cy = cic_axis((double)x[3 * i + 1], m.inv_h[1], m.my);
         const CicAxis cz = cic_axis((double)x[3 * i + 2], m.inv_h[2], m.mz);
-        double acc[3] = {0.0, 0.0, 0.0};
+        const float *gt = grids + (int64_t)t * NCOMP * m.g;
+        float v[8][NCOMP];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int a = c >> 2, b = (c >> 1) & 1, d = c & 1;
+            const int64_t idx =
+                ((int64_t)(a ? cx.i1 : cx.i0) * m.my + (b ? cy.i1 : cy.i0)) * m.mz + (d ? cz.i1 : cz.i0);
+#pragma unroll
+            for (int q = 0; q < NCOMP; ++q) v[c][q] = gt[q * m.g + idx];
+        }
+        double acc[NCOMP];
+#pragma unroll
+        for (int q = 0; q < NCOMP; ++q) acc[q] = 0.0;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             const int a = c >> 2, b = (c >> 1) & 1, d = c & 1;
             const double w = ((a ? cx.w1 : cx.w0) * (b ? cy.w1 : cy.w0)) * (d ? cz.w1 : cz.w0);
-            const int64_t idx =
-                ((int64_t)(a ? cx.i1 : cx.i0) * m.my + (b ? cy.i1 : cy.i0)) * m.mz + (d ? cz.i1 : cz.i0);
-            for (int q = 0; q < ncomp; ++q) acc[q] += w * (double)grids[((int64_t)t * ncomp + q) * m.g + idx];
+#pragma unroll
+            for (int q = 0; q < NCOMP; ++q) acc[q] += w * (double)v[c][q];
         }
         const double s = scale_per_particle ? (double)scale_per_particle[i] : 1.0;
-        for (int q = 0; q < ncomp; ++q) out[ncomp * o + q] = (float)(acc[q] * s);
+#pragma unroll
+        for (int q = 0; q < NCOMP; ++q) out[NCOMP * o + q] = (float)(acc[q] * s);
     }
-}
-
-// ---------------------------------------------------------------------------
-// k-space helpers
-// ---------------------------------------------------------------------------
-struct KGeom {
-    int mx, my, mz, zh;
-    int64_t gc;
-    double kfac[3];  // 2 pi / L_d
-};
-
-__device__ __forceinline__ void kvec(const KGeom &g, int64_t kk, double k[3], double kd[3], double &wgt) {
-    const int64_t plane = (int64_t)g.my * g.zh;
-    const int ix = (int)(kk / plane);
-    const int64_t r = kk - (int64_t)ix * plane;
-    const int iy = (int)(r / g.zh);
-    const int iz = (int)(r - (int64_t)iy * g.zh);
-    const int nx = (2 * ix < g.mx) ? ix : ix - g.mx;
-    const int ny = (2 * iy < g.my) ? iy : iy - g.my;
-    k[0] = g.kfac[0] * nx;
-    k[1] = g.kfac[1] * ny;
-    k[2] = g.kfac[2] * iz;
-    // derivative transfer: zero on the Nyquist planes of even edges
-    kd[0] = (2 * ix == g.mx) ? 0.0 : k[0];
-    kd[1] = (2 * iy == g.my) ? 0.0 : k[1];
-    kd[2] = (2 * iz == g.mz) ? 0.0 : k[2];
-    wgt = (iz == 0 || 2 * iz == g.mz) ? 1.0 : 2.0;
 }
 
 struct FieldK {
@@ -321,56 +311,117 @@ __global__ void __launch_bounds__(kBlock) field_kspace_kernel(float2 *__restrict
     if (partials) block_reduce_store<1>(e_acc, partials + blockIdx.x);
 }
 
-// Spectral moments of the filtered densities phi~_t = H rho_t / G (pmesh
-// normalisation), summed over the full spectrum through the half spectrum
-// weights: P_ts = sum_k w Re(phi~_t^* phi~_s), Q_ts = sum_k w |k|^2 Re(...),
-// t <= s packed row-major, then phi~_t(0).  Every quadratic Hamiltonian's
-// energy and its derivatives in chi, kappa and sigma follow from them.
+// Forward x pass fused with the k-space pass.  One workgroup owns W
+// consecutive (y, kz) lines of all T density spectra (after their z and y
+// passes): it runs their forward FFTs along x, then the k-space pass of
+// field_kspace_kernel at every k of those lines (same arithmetic, so the same
+// bits) and writes the 3T gradient spectra -i k_d V_t for the inverse passes,
+// so the T transformed spectra never make a round trip through HBM.  LDS:
+// twiddles, two buffers of T W lines.
+template <int MAXT>
+__global__ void __launch_bounds__(kBlock) field_xline_kernel(const float2 *__restrict__ spec,
+                                                              float2 *__restrict__ grad, KGeom g, FieldK p,
+                                                              FftAxis ax, const float2 *__restrict__ tw_glob,
+                                                              int wshift, double *__restrict__ partials) {
+    extern __shared__ float2 smem[];
+    const int n = ax.n;  // Mx
+    const int W = 1 << wshift;
+    const int T = p.T;
+    float2 *tw = smem;
+    float2 *A = tw + n;
+    float2 *B = A + T * W * ax.ld;
+    const int plane = g.my * g.zh;  // lines, and the element stride along x
+    const int line0 = (int)xcd_block(blockIdx.x, gridDim.x) * W;
+    const int wv = min(W, plane - line0);
+    for (int i = threadIdx.x; i < n - 1; i += blockDim.x) tw[i] = tw_glob[i];
+    // load T x W lines, W consecutive lines contiguous in memory
+    const int nel = T * W * n;
+    for (int i0 = threadIdx.x; i0 < nel; i0 += kBlock * kLoadBatch) {
+        float2 v[kLoadBatch];
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kBlock;
+            v[j] = make_float2(0.f, 0.f);
+            if (idx < nel) {
+                const int l = idx & (W - 1), te = idx >> wshift, t = te / n, e = te - t * n;
+                if (l < wv) v[j] = spec[(int64_t)t * g.gc + (int64_t)e * plane + line0 + l];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kBlock;
+            if (idx < nel) {
+                const int l = idx & (W - 1), te = idx >> wshift, t = te / n, e = te - t * n;
+                A[(t * W + l) * ax.ld + e] = v[j];
+            }
+        }
+    }
+    __syncthreads();
+    const float2 *F = fft_lds(A, B, T * W, ax, tw, -1.f);
+    // k-space pass, writing -i k_d V_t(k)
+    double e_acc[1] = {0.0};
+    for (int idx = threadIdx.x; idx < W * n; idx += blockDim.x) {
+        const int l = idx & (W - 1), e = idx >> wshift;
+        if (l >= wv) continue;
+        const int64_t kk = (int64_t)e * plane + line0 + l;
+        double k[3], kd[3], wgt;
+        kvec(g, kk, k, kd, wgt);
+        const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        const float H = (float)exp(-0.5 * p.sigma2 * k2);
+        const float hn = (float)((double)H * p.inv_g);
+        float2 ph[MAXT];
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t)
+            if (t < T) ph[t] = cscale(F[(t * W + l) * ax.ld + e], hn);
+        if (p.want_energy) {
+            double en = 0.0;
+#pragma unroll
+            for (int t = 0; t < MAXT; ++t) {
+                if (t >= T) continue;
+#pragma unroll
+                for (int u = 0; u < MAXT; ++u) {
+                    if (u >= T) continue;
+                    en += 0.5 * p.Ad[t * HYMD_MAX_TYPES + u] *
+                          ((double)ph[t].x * (double)ph[u].x + (double)ph[t].y * (double)ph[u].y);
+                }
+            }
+            en *= wgt;
+            if (kk == 0) {
+                for (int t = 0; t < T; ++t) en += p.b[t] * (double)ph[t].x;
+                en += p.c0;
+            }
+            e_acc[0] += en;
+        }
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t) {
+            if (t >= T) continue;
+            float2 v = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < MAXT; ++u)
+                if (u < T) {
+                    const float a = (float)p.Ad[t * HYMD_MAX_TYPES + u];
+                    v.x += a * ph[u].x;
+                    v.y += a * ph[u].y;
+                }
+            v = cscale(v, H);
+            if (kk == 0) v.x += (float)p.b[t];
+            float2 *gt = grad + (int64_t)(3 * t) * g.gc + kk;
+            gt[0] = make_float2((float)kd[0] * v.y, -(float)kd[0] * v.x);
+            gt[g.gc] = make_float2((float)kd[1] * v.y, -(float)kd[1] * v.x);
+            gt[2 * g.gc] = make_float2((float)kd[2] * v.y, -(float)kd[2] * v.x);
+        }
+    }
+    if (partials) block_reduce_store<1>(e_acc, partials + blockIdx.x);
+}
+
+// Spectral moments (P, Q, phi~(0)) of the filtered densities, one partial
+// row per block.
 template <int MAXT>
 __global__ void __launch_bounds__(kBlock) field_moments_kernel(const float2 *__restrict__ spec, KGeom g, int T,
                                                                 double sigma2, double inv_g,
                                                                 double *__restrict__ partials) {
-    constexpr int NP = MAXT * (MAXT + 1) / 2;
-    double acc[2 * NP + MAXT];
-#pragma unroll
-    for (int j = 0; j < 2 * NP + MAXT; ++j) acc[j] = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; kk < g.gc; kk += stride) {
-        double k[3], kd[3], wgt;
-        kvec(g, kk, k, kd, wgt);
-        const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
-        const double hn = exp(-0.5 * sigma2 * k2) * inv_g;
-        double re[MAXT], im[MAXT];
-#pragma unroll
-        for (int t = 0; t < MAXT; ++t) {
-            const float2 v = (t < T) ? spec[(int64_t)t * g.gc + kk] : make_float2(0.f, 0.f);
-            re[t] = hn * (double)v.x;
-            im[t] = hn * (double)v.y;
-        }
-        int q = 0;
-#pragma unroll
-        for (int t = 0; t < MAXT; ++t)
-#pragma unroll
-            for (int s = t; s < MAXT; ++s, ++q) {
-                const double p = wgt * (re[t] * re[s] + im[t] * im[s]);
-                acc[q] += p;
-                acc[NP + q] += k2 * p;
-            }
-        if (kk == 0)
-#pragma unroll
-            for (int t = 0; t < MAXT; ++t) acc[2 * NP + t] += re[t];
-    }
-    // compact to the caller's T and reduce
-    double out[2 * NP + MAXT];
-    int o = 0;
-    for (int t = 0; t < T; ++t)
-        for (int s = t; s < T; ++s) out[o++] = acc[t * MAXT - t * (t - 1) / 2 + (s - t)];
-    for (int t = 0; t < T; ++t)
-        for (int s = t; s < T; ++s) out[o++] = acc[NP + t * MAXT - t * (t - 1) / 2 + (s - t)];
-    for (int t = 0; t < T; ++t) out[o++] = acc[2 * NP + t];
-    for (int j = o; j < 2 * NP + MAXT; ++j) out[j] = 0.0;
-    // one row of 2 NP + MAXT doubles per block (the first 2 T(T+1)/2 + T are used)
-    block_reduce_store<2 * NP + MAXT>(out, partials + (int64_t)blockIdx.x * (2 * NP + MAXT));
+    moments_block<MAXT>(spec, g, T, sigma2, inv_g, blockIdx.x, gridDim.x,
+                        partials + (int64_t)blockIdx.x * (2 * (MAXT * (MAXT + 1) / 2) + MAXT));
 }
 
 struct PmeK {
@@ -460,7 +511,7 @@ struct hymd_pm {
     int max_types;
     hipStream_t stream;
     FftPlan fft;
-    DevBuf paint;     // max_types x G int64 (zero between calls)
+    DevBuf paint;     // max_types x G: float densities, or int64 for the accumulate-mode paint
     DevBuf spec;      // max_types x Gc complex
     DevBuf grad;      // max(3 max_types, 4) x Gc complex
     DevBuf cscratch;  // max_types x Gc complex (phi / v_ext inverse transforms)
@@ -469,13 +520,54 @@ struct hymd_pm {
     DevBuf result;    // reduced doubles
     HostBuf host;     // pinned readback
     int kblocks;      // blocks of the k-space passes
-    // spatial sort of the last painted particle set (cell-binned order)
+    DevBuf mpart;     // moments partial rows
+    // spatial sort of the last painted particle set (row-binned order)
     DevBuf keys, perm, xs, ts, ws;  // per particle
-    DevBuf counts, start, cursor, tiles;  // per cell (counts kept zero between sorts)
+    DevBuf counts, start, tiles;  // per mesh row (counts kept zero between sorts)
     int64_t sorted_n = 0;
     bool sorted_weights = false;
 };
 
+// LDS budget of one x-line workgroup (two per CU)
+constexpr size_t kXlineLdsBytes = 80 * 1024;
+
+static size_t xline_lds(const FftPlan &f, int T, int W) {
+    return sizeof(float2) * ((size_t)f.ax.n + 2 * (size_t)T * W * f.ax.ld);
+}
+
+// Lines per x-line workgroup for T types (0: the fused pass does not fit).
+static int xline_width(const FftPlan &f, int T) {
+    for (int W = 16; W >= 2; W >>= 1)
+        if (xline_lds(f, T, W) <= kXlineLdsBytes) return W;
+    return 0;
+}
+
+template <int MAXT>
+static int launch_xline(hymd_pm *pm, const float2 *spec, float2 *grad, const KGeom &g, const FieldK &p, int W,
+                        double *partials, int *nblocks) {
+    const FftPlan &f = pm->fft;
+    const int plane = g.my * g.zh;
+    const int nb = (plane + W - 1) / W;
+    const size_t lds = xline_lds(f, p.T, W);
+    static bool attr_set = false;  // the opt-in is per kernel and process
+    if (!attr_set && lds > 64 * 1024) {
+        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&field_xline_kernel<MAXT>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kXlineLdsBytes));
+        attr_set = true;
+    }
+    int wshift = 0;
+    while ((1 << wshift) < W) ++wshift;
+    field_xline_kernel<MAXT><<<nb, kBlock, lds, pm->stream>>>(spec, grad, g, p, f.ax, (const float2 *)f.tw_x.ptr,
+                                                              wshift, partials);
+    HYMD_HIP(hipGetLastError());
+    *nblocks = nb;
+    return HYMD_OK;
+}
+
+// LDS budget of one row-paint workgroup (int64 accumulators of tc types x Mz)
+constexpr size_t kRowLdsBytes = 32 * 1024;
+
+// Edges <= 1024 keep G and Gc below 2^31, so kernels index cells in 32 bits.
 static int set_mesh(hymd_pm *pm, const int32_t mesh[3], const double box[3]) {
     for (int d = 0; d < 3; ++d) {
         HYMD_REQUIRE(mesh[d] >= 4 && mesh[d] <= 1024 && mesh[d] % 2 == 0,
@@ -527,12 +619,13 @@ extern "C" int hymd_pm_create(hymd_pm **out, const int32_t mesh[3], const double
     if (st == HYMD_OK) st = pm->paint.reserve(sizeof(int64_t) * m.g * max_types);
     if (st == HYMD_OK) st = pm->spec.reserve(sizeof(float2) * m.gc * max_types);
     if (st == HYMD_OK) st = pm->grad.reserve(sizeof(float2) * m.gc * std::max(3 * max_types, 4));
-    if (st == HYMD_OK) st = pm->partials.reserve(sizeof(double) * (pm->kblocks + 16));
+    // energy partial rows: k-space blocks or x-line blocks (at least 2 lines each)
+    if (st == HYMD_OK)
+        st = pm->partials.reserve(sizeof(double) * (std::max<int64_t>(pm->kblocks, (int64_t)m.my * m.zh / 2 + 1) + 16));
     if (st == HYMD_OK) st = pm->result.reserve(sizeof(double) * 16);
     if (st == HYMD_OK) st = pm->host.reserve(sizeof(double) * 16);
     if (st == HYMD_OK) st = pm->counts.reserve(sizeof(int) * (m.g + 2));
     if (st == HYMD_OK) st = pm->start.reserve(sizeof(int) * (m.g + 1));
-    if (st == HYMD_OK) st = pm->cursor.reserve(sizeof(int) * m.g);
     if (st == HYMD_OK) st = pm->tiles.reserve(sizeof(int) * (scan_tiles(m.g) + 1));
     if (st == HYMD_OK) {
         zero_kernel<<<grid_for(m.g * max_types), kBlock, 0, pm->stream>>>(
@@ -588,36 +681,35 @@ static FieldK field_params(int T, const double *coupling, const double *shift, d
     return p;
 }
 
-// Bin particles by CIC base cell: perm[pos] = original index, xs/ts/ws the
-// sorted positions, types (0 when types == NULL) and weights (if w != NULL),
-// start[c] .. start[c+1] the particles based in cell c.
-static int sort_particles(hymd_pm *pm, const float *x, const int32_t *types, const float *w, int64_t n) {
+// Bin particles by the mesh row of their CIC base cell: perm[pos] = original
+// index, xs/ts/ws the sorted positions, types (0 when types == NULL) and
+// weights (if w != NULL), start[r] .. start[r+1] the particles based in row r.
+static int sort_particles(hymd_pm *pm, const float *x, const int32_t *types, const float *w, int64_t n,
+                          bool keys_ready = false) {
     const Mesh &m = pm->mesh;
     hipStream_t s = pm->stream;
-    HYMD_REQUIRE(n < (int64_t(1) << 31) && m.g < (int64_t(1) << 31), "too many particles or cells to sort");
+    HYMD_REQUIRE(n < (int64_t(1) << 31), "too many particles to sort (%lld)", (long long)n);
+    const int64_t nrows = (int64_t)m.mx * m.my;
     const size_t nn = (size_t)std::max<int64_t>(n, 1);
-    HYMD_TRY(pm->keys.reserve(sizeof(int) * nn));
+    HYMD_TRY(pm->keys.reserve(sizeof(int2) * nn));
     HYMD_TRY(pm->perm.reserve(sizeof(int) * nn));
     HYMD_TRY(pm->xs.reserve(sizeof(float) * 3 * nn));
     HYMD_TRY(pm->ts.reserve(sizeof(int32_t) * nn));
     if (w) HYMD_TRY(pm->ws.reserve(sizeof(float) * nn));
-    if (n > 0)
-        cell_key_kernel<<<grid_for(n), kBlock, 0, s>>>(x, n, m, (int *)pm->keys.ptr, (int *)pm->counts.ptr);
+    if (n > 0 && !keys_ready)
+        row_key_kernel<<<grid_for(n), kBlock, 0, s>>>(x, n, m, (int2 *)pm->keys.ptr, (int *)pm->counts.ptr);
     HYMD_HIP(hipGetLastError());
-    HYMD_TRY(exclusive_scan((int *)pm->counts.ptr, (int *)pm->start.ptr, m.g, (int *)pm->tiles.ptr,
-                            (int *)pm->cursor.ptr, true, s));
-    if (n > 0)
-        cell_scatter_kernel<<<grid_for(n), kBlock, 0, s>>>(x, types, w, n, (const int *)pm->keys.ptr,
-                                                           (int *)pm->cursor.ptr, (int *)pm->perm.ptr,
-                                                           (float *)pm->xs.ptr, (int32_t *)pm->ts.ptr,
-                                                           w ? (float *)pm->ws.ptr : nullptr);
-    HYMD_HIP(hipGetLastError());
+    HYMD_TRY(exclusive_scan((int *)pm->counts.ptr, (int *)pm->start.ptr, nrows, (int *)pm->tiles.ptr, nullptr,
+                            true, s));
+    HYMD_TRY(launch_key_scatter(x, types, w, n, (const int2 *)pm->keys.ptr, (const int *)pm->start.ptr,
+                                (int *)pm->perm.ptr, (float *)pm->xs.ptr, (int32_t *)pm->ts.ptr,
+                                w ? (float *)pm->ws.ptr : nullptr, s));
     pm->sorted_n = n;
     pm->sorted_weights = (w != nullptr);
     return HYMD_OK;
 }
 
-// Gather paint of the sorted particles into float grids rho_t = paint/dV
+// Row-gather paint of the sorted particles into float grids rho_t = paint/dV
 // (T grids), optionally also the int64 fixed-point sums.
 static int paint_sorted(hymd_pm *pm, int T, const double *mass, float *out, long long *out_fixed) {
     PaintArgs pa;
@@ -625,18 +717,13 @@ static int paint_sorted(hymd_pm *pm, int T, const double *mass, float *out, long
     pa.scale = std::ldexp(1.0, HYMD_FIXED_SHIFT);
     for (int t = 0; t < HYMD_MAX_TYPES; ++t) pa.mass[t] = (mass && t < T) ? mass[t] : 1.0;
     const double out_scale = std::ldexp(1.0, -HYMD_FIXED_SHIFT) / cell_volume(pm);
-    const float *ws = pm->sorted_weights ? (const float *)pm->ws.ptr : nullptr;
     const Mesh &m = pm->mesh;
-    const int nb = grid_for(m.g);
-    const float *xs = (const float *)pm->xs.ptr;
-    const int32_t *ts = (const int32_t *)pm->ts.ptr;
-    const int *st = (const int *)pm->start.ptr;
-    if (T <= 4)
-        paint_gather_kernel<4><<<nb, kBlock, 0, pm->stream>>>(xs, ts, ws, st, m, pa, out_scale, out, out_fixed);
-    else if (T <= 8)
-        paint_gather_kernel<8><<<nb, kBlock, 0, pm->stream>>>(xs, ts, ws, st, m, pa, out_scale, out, out_fixed);
-    else
-        paint_gather_kernel<16><<<nb, kBlock, 0, pm->stream>>>(xs, ts, ws, st, m, pa, out_scale, out, out_fixed);
+    const int tc = std::max(1, std::min(T, (int)(kRowLdsBytes / (sizeof(long long) * m.mz))));
+    dim3 grid((unsigned)(m.mx * m.my), (unsigned)((T + tc - 1) / tc));
+    paint_rows_kernel<<<grid, kBlock, sizeof(long long) * tc * m.mz, pm->stream>>>(
+        (const float *)pm->xs.ptr, (const int32_t *)pm->ts.ptr,
+        pm->sorted_weights ? (const float *)pm->ws.ptr : nullptr, (const int *)pm->start.ptr, m, pa, tc, out_scale,
+        out, out_fixed);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }
@@ -646,9 +733,16 @@ static int paint_sorted(hymd_pm *pm, int T, const double *mass, float *out, long
 static int readout_sorted(hymd_pm *pm, int ncomp, int n_types, const float *grids, float *out, bool weighted) {
     const int64_t n = pm->sorted_n;
     if (n > 0)
-        readout_kernel<float><<<grid_for(n), kBlock, 0, pm->stream>>>(
-            grids, ncomp, (const float *)pm->xs.ptr, (const int32_t *)pm->ts.ptr,
-            weighted ? (const float *)pm->ws.ptr : nullptr, n, n_types, pm->mesh, out, (const int *)pm->perm.ptr);
+    {
+        const float *xs = (const float *)pm->xs.ptr;
+        const int32_t *ts = (const int32_t *)pm->ts.ptr;
+        const float *ws = weighted ? (const float *)pm->ws.ptr : nullptr;
+        const int *pp = (const int *)pm->perm.ptr;
+        if (ncomp == 3)
+            readout_kernel<3><<<grid_for(n), kBlock, 0, pm->stream>>>(grids, xs, ts, ws, n, n_types, pm->mesh, out, pp);
+        else
+            readout_kernel<1><<<grid_for(n), kBlock, 0, pm->stream>>>(grids, xs, ts, ws, n, n_types, pm->mesh, out, pp);
+    }
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }
@@ -708,23 +802,22 @@ extern "C" int hymd_fft_c2r(hymd_pm *pm, const void *in, float *out, int32_t bat
     const size_t bytes = sizeof(float2) * pm->mesh.gc * batch;
     HYMD_TRY(pm->cscratch.reserve(bytes));
     HYMD_HIP(hipMemcpyAsync(pm->cscratch.ptr, in, bytes, hipMemcpyDeviceToDevice, pm->stream));
-    HYMD_TRY(fft_inverse_xy(pm->fft, (float2 *)pm->cscratch.ptr, batch, pm->stream));
-    return fft_c2r_z(pm->fft, (const float2 *)pm->cscratch.ptr, out, batch, 1.f, pm->stream);
+    return fft_c2r(pm->fft, (float2 *)pm->cscratch.ptr, out, batch, 1.f, pm->stream);
 }
 
 extern "C" int hymd_readout(hymd_pm *pm, const float *grid, const float *x, int64_t n, float *out) {
     HYMD_REQUIRE(pm && grid && out && (x || n == 0) && n >= 0, "bad argument");
     if (n > 0)
-        readout_kernel<float><<<grid_for(n), kBlock, 0, pm->stream>>>(grid, 1, x, nullptr, nullptr, n, 1, pm->mesh,
-                                                                        out, nullptr);
+        readout_kernel<1><<<grid_for(n), kBlock, 0, pm->stream>>>(grid, x, nullptr, nullptr, n, 1, pm->mesh, out,
+                                                                    nullptr);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }
 
-extern "C" int hymd_update_field(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
-                                 const double *coupling, const double *shift, double constant,
-                                 const double *mass, double sigma, float *force_mesh, float *phi,
-                                 void *phi_fourier, float *v_ext, double *energy) {
+static int update_field_impl(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
+                             const double *coupling, const double *shift, double constant, const double *mass,
+                             double sigma, float *force_mesh, float *phi, void *phi_fourier, float *v_ext,
+                             double *energy, bool keys_ready) {
     HYMD_REQUIRE(pm && force_mesh && coupling && (x || n == 0) && (types || n == 0), "null argument");
     HYMD_REQUIRE(n >= 0, "negative particle count");
     HYMD_REQUIRE(n_types >= 1 && n_types <= pm->max_types, "n_types %d exceeds the handle's %d", n_types,
@@ -739,13 +832,32 @@ extern "C" int hymd_update_field(hymd_pm *pm, const float *x, const int32_t *typ
 
     // 1. cell sort + gather paint of rho_t = paint_t / dV (float grids)
     float *rho = (float *)grid;  // the handle's paint buffer holds T float grids here
-    HYMD_TRY(sort_particles(pm, x, types, nullptr, n));
+    HYMD_TRY(sort_particles(pm, x, types, nullptr, n, keys_ready));
     HYMD_TRY(paint_sorted(pm, T, mass, rho, nullptr));
+    FieldK p = field_params(T, coupling, shift, constant, sigma, m);
+    p.want_energy = energy ? 1 : 0;
+    KGeom g0 = kgeom(pm);
+    const int W = xline_width(pm->fft, T);
+    if (!phi && !phi_fourier && !v_ext && W > 0) {
+        // force grids only: z + y forward passes, the forward x pass fused
+        // with the k-space pass, then the inverse x, y and z passes
+        HYMD_TRY(fft_r2c_zy(pm->fft, rho, spec, T, s));
+        double *part = energy ? (double *)pm->partials.ptr : nullptr;
+        int nb = 0;
+        if (T <= 4) HYMD_TRY(launch_xline<4>(pm, spec, grad, g0, p, W, part, &nb));
+        else if (T <= 8) HYMD_TRY(launch_xline<8>(pm, spec, grad, g0, p, W, part, &nb));
+        else HYMD_TRY(launch_xline<16>(pm, spec, grad, g0, p, W, part, &nb));
+        HYMD_TRY(fft_c2r(pm->fft, grad, force_mesh, 3 * T, 1.f, s));
+        if (energy) {
+            double e = 0.0;
+            HYMD_TRY(read_result(pm, nb, 1, &e));
+            *energy = e * (m.box[0] * m.box[1] * m.box[2]);
+        }
+        return HYMD_OK;
+    }
     // 2. r2c
     HYMD_TRY(fft_r2c_real(pm->fft, rho, spec, T, s));
     // 3. fused filter / W' / filter / gradient
-    FieldK p = field_params(T, coupling, shift, constant, sigma, m);
-    p.want_energy = energy ? 1 : 0;
     float2 *phif = (float2 *)phi_fourier;
     if (phi && !phif) {
         HYMD_TRY(pm->cscratch.reserve(sizeof(float2) * m.gc * T));
@@ -762,12 +874,10 @@ extern "C" int hymd_update_field(hymd_pm *pm, const float *x, const int32_t *typ
         field_kspace_kernel<16><<<kb, kBlock, 0, s>>>(spec, grad, phif, v_ext ? 1 : 0, g, p, partials);
     HYMD_HIP(hipGetLastError());
     // 4. force grids
-    HYMD_TRY(fft_inverse_xy(pm->fft, grad, 3 * T, s));
-    HYMD_TRY(fft_c2r_z(pm->fft, grad, force_mesh, 3 * T, 1.f, s));
+    HYMD_TRY(fft_c2r(pm->fft, grad, force_mesh, 3 * T, 1.f, s));
     // 5. optional real-space outputs
     if (v_ext) {
-        HYMD_TRY(fft_inverse_xy(pm->fft, spec, T, s));
-        HYMD_TRY(fft_c2r_z(pm->fft, spec, v_ext, T, 1.f, s));
+        HYMD_TRY(fft_c2r(pm->fft, spec, v_ext, T, 1.f, s));
     }
     if (phi) {
         float2 *work = phif;
@@ -777,8 +887,7 @@ extern "C" int hymd_update_field(hymd_pm *pm, const float *x, const int32_t *typ
                                     hipMemcpyDeviceToDevice, s));
             work = (float2 *)pm->cscratch.ptr;
         }
-        HYMD_TRY(fft_inverse_xy(pm->fft, work, T, s));
-        HYMD_TRY(fft_c2r_z(pm->fft, work, phi, T, 1.f, s));
+        HYMD_TRY(fft_c2r(pm->fft, work, phi, T, 1.f, s));
     }
     if (energy) {
         double e = 0.0;
@@ -787,6 +896,35 @@ extern "C" int hymd_update_field(hymd_pm *pm, const float *x, const int32_t *typ
     }
     return HYMD_OK;
 }
+
+extern "C" int hymd_update_field(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
+                                 const double *coupling, const double *shift, double constant,
+                                 const double *mass, double sigma, float *force_mesh, float *phi,
+                                 void *phi_fourier, float *v_ext, double *energy) {
+    return update_field_impl(pm, x, types, n, n_types, coupling, shift, constant, mass, sigma, force_mesh, phi,
+                             phi_fourier, v_ext, energy, false);
+}
+
+namespace hymd {
+int field_key_target(hymd_pm *pm, int64_t n, RowKeyTarget *t) {
+    HYMD_REQUIRE(pm && t && n >= 0 && n < (int64_t(1) << 31), "bad argument");
+    HYMD_TRY(pm->keys.reserve(sizeof(int2) * (size_t)std::max<int64_t>(n, 1)));
+    t->keys = (int2 *)pm->keys.ptr;
+    t->counts = (int *)pm->counts.ptr;
+    t->inv_h[0] = pm->mesh.inv_h[0];
+    t->inv_h[1] = pm->mesh.inv_h[1];
+    t->mx = pm->mesh.mx;
+    t->my = pm->mesh.my;
+    return HYMD_OK;
+}
+
+int update_field_keyed(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
+                       const double *coupling, const double *shift, double constant, const double *mass,
+                       double sigma, float *force_mesh, double *energy) {
+    return update_field_impl(pm, x, types, n, n_types, coupling, shift, constant, mass, sigma, force_mesh, nullptr,
+                             nullptr, nullptr, energy, true);
+}
+}  // namespace hymd
 
 // paint + r2c + moments (and, if force_mesh != NULL, the force grids of the
 // quadratic Hamiltonian (coupling, shift) as in hymd_update_field)
@@ -807,13 +945,16 @@ extern "C" int hymd_field_moments(hymd_pm *pm, const float *x, const int32_t *ty
     KGeom g = kgeom(pm);
     const int np = T * (T + 1) / 2;
     const int nv = 2 * np + T;
-    const int row = (T <= 4) ? 2 * 10 + 4 : 2 * 36 + 8;  // 2 NP + MAXT of the instance
-    const int nb = std::min(pm->kblocks, 1024);
-    DevBuf part;
+    const int row = moments_row_len(T);
+    const int nb = moments_blocks(m.gc);
+    DevBuf &part = pm->mpart;
     HYMD_TRY(part.reserve(sizeof(double) * (size_t)row * nb));
     HYMD_TRY(pm->result.reserve(sizeof(double) * row));
     HYMD_TRY(pm->host.reserve(sizeof(double) * row));
-    if (T <= 4)
+    if (T <= 2)
+        field_moments_kernel<2><<<nb, kBlock, 0, s>>>(spec, g, T, sigma * sigma, 1.0 / (double)m.g,
+                                                      (double *)part.ptr);
+    else if (T <= 4)
         field_moments_kernel<4><<<nb, kBlock, 0, s>>>(spec, g, T, sigma * sigma, 1.0 / (double)m.g,
                                                       (double *)part.ptr);
     else
@@ -830,8 +971,7 @@ extern "C" int hymd_field_moments(hymd_pm *pm, const float *x, const int32_t *ty
         else
             field_kspace_kernel<8><<<pm->kblocks, kBlock, 0, s>>>(spec, grad, nullptr, 0, g, p, nullptr);
         HYMD_HIP(hipGetLastError());
-        HYMD_TRY(fft_inverse_xy(pm->fft, grad, 3 * T, s));
-        HYMD_TRY(fft_c2r_z(pm->fft, grad, force_mesh, 3 * T, 1.f, s));
+        HYMD_TRY(fft_c2r(pm->fft, grad, force_mesh, 3 * T, 1.f, s));
     }
     HYMD_HIP(hipStreamSynchronize(s));
     for (int j = 0; j < nv; ++j) moments[j] = ((double *)pm->host.ptr)[j];
@@ -872,9 +1012,9 @@ extern "C" int hymd_update_field_force_q(hymd_pm *pm, const float *x, const floa
         HYMD_TRY(pm->rscratch.reserve(sizeof(float) * 3 * m.g));
         field = (float *)pm->rscratch.ptr;
     }
-    HYMD_TRY(fft_inverse_xy(pm->fft, out, psi ? 4 : 3, s));
-    HYMD_TRY(fft_c2r_z(pm->fft, out, field, 3, 1.f, s));
-    if (psi) HYMD_TRY(fft_c2r_z(pm->fft, out + 3 * m.gc, psi, 1, 1.f, s));
+    HYMD_TRY(fft_inverse_x(pm->fft, out, psi ? 4 : 3, s));
+    HYMD_TRY(fft_c2r_yz(pm->fft, out, field, 3, 1.f, s));
+    if (psi) HYMD_TRY(fft_c2r_yz(pm->fft, out + 3 * m.gc, psi, 1, 1.f, s));
     HYMD_TRY(readout_sorted(pm, 3, 1, field, elec_forces, true));
     if (energy_q) {
         double e = 0.0;

@@ -23,7 +23,6 @@
 
 namespace hymd {
 
-constexpr int kMaxTermsPerThread = 4;  // bonds and angles per thread and block
 
 struct FusedArgs {
     double *x, *v;
@@ -42,18 +41,25 @@ struct FusedArgs {
     double box[3];
     int n_inner;
     int pending;                 // apply the previous step's outer half kick first
-    int max_bonds, max_angles, max_entries, max_particles;  // LDS sizing
     double *partials;            // 8 per block (E_bond, pr_bond[3], E_angle, pr_angle[3]) or null
+    RowKeyTarget key;            // row-binning keys of the final positions (key.keys == null: none)
 };
 
-__device__ __forceinline__ double wrap_np_mod(double x, double L) {
-    double r = fmod(x, L);
-    if (r != 0.0 && r < 0.0) r += L;
-    return r;
+// Dynamic LDS of one block: positions (3 doubles per particle), bond slots
+// (3 doubles), angle slots (6 doubles), per-particle entry offsets (int) and
+// entries (2 per bond, 3 per angle).
+// Budgets: 256 particles per block (one per thread) in 26 KB, six blocks per
+// CU; molecules above 256 beads use 512-particle blocks (two per thread) in
+// 52 KB, three per CU.
+constexpr size_t kFusedLds1 = 26 * 1024;
+constexpr size_t kFusedLds2 = 52 * 1024;
+inline size_t fused_lds_bytes(int particles, int bonds, int angles) {
+    return sizeof(double) * (3 * (size_t)particles + 3 * (size_t)bonds + 6 * (size_t)angles) +
+           sizeof(int) * ((size_t)particles + 1) + sizeof(uint32_t) * (2 * (size_t)bonds + 3 * (size_t)angles);
 }
 
 template <int PPT>
-__global__ void __launch_bounds__(kBlock) fused_inner_kernel(FusedArgs A) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT == 1 ? 4 : 3))) fused_inner_kernel(FusedArgs A) {
     extern __shared__ double smem_d[];
     const int blk = blockIdx.x;
     const int p0 = A.blk_start[blk];
@@ -61,38 +67,49 @@ __global__ void __launch_bounds__(kBlock) fused_inner_kernel(FusedArgs A) {
     const int b0 = A.blk_bond_off[blk], nb = A.blk_bond_off[blk + 1] - b0;
     const int a0 = A.blk_ang_off[blk], na = A.blk_ang_off[blk + 1] - a0;
     const int e0 = A.csr_off[p0], ne = A.csr_off[p0 + np] - e0;
-    // LDS: positions, bond slots (fa), angle slots (fa, fc), CSR
-    double *xs = smem_d;                              // 3 * max_particles
-    double *sb = xs + 3 * A.max_particles;            // 3 * max_bonds
-    double *sa = sb + 3 * A.max_bonds;                // 6 * max_angles
-    int *coff = (int *)(sa + 6 * A.max_angles);       // max_particles + 1
-    uint32_t *ent = (uint32_t *)(coff + A.max_particles + 1);  // max_entries
+    // LDS, laid out for this block's own counts: positions, bond slots (fa),
+    // angle slots (fa, fc), entry offsets, entries
+    double *xs = smem_d;                    // 3 * np
+    double *sb = xs + 3 * np;               // 3 * nb
+    double *sa = sb + 3 * nb;               // 6 * na
+    int *coff = (int *)(sa + 6 * na);       // np + 1
+    uint32_t *ent = (uint32_t *)(coff + np + 1);  // ne = 2 nb + 3 na
     for (int j = threadIdx.x; j <= np; j += blockDim.x) coff[j] = A.csr_off[p0 + j] - e0;
     for (int j = threadIdx.x; j < ne; j += blockDim.x) ent[j] = A.csr_ent[e0 + j];
-    const Box box{{A.box[0], A.box[1], A.box[2]}};
+    Box box;
+    for (int d = 0; d < 3; ++d) {
+        box.l[d] = A.box[d];
+        box.inv[d] = 1.0 / A.box[d];
+    }
 
-    double x[PPT][3], v[PPT][3];
-    float ff[PPT][3], fs[PPT][3];
+    // Own particles: positions live in LDS only, velocities and the fast
+    // force in registers.  The outer (slow) half kicks of this step, and the
+    // pending one of the previous step, come first, in the order of the
+    // unfused path.
+    double v[PPT][3];
+    float ff[PPT][3];
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         const int j = threadIdx.x + k * kBlock;
         if (j < np) {
             const int64_t g = 3 * (int64_t)(p0 + j);
             for (int d = 0; d < 3; ++d) {
-                x[k][d] = A.x[g + d];
-                v[k][d] = A.v[g + d];
+                const double fs = (double)A.f_slow[g + d];
+                double vi = A.v[g + d];
+                if (A.pending) vi = vi + A.h_out * fs;
+                v[k][d] = vi + A.h_out * fs;
                 ff[k][d] = A.f_fast[g + d];
-                fs[k][d] = A.f_slow[g + d];
+                xs[3 * j + d] = A.x[g + d];
             }
         }
     }
     // own terms in registers
-    uint32_t bab[kMaxTermsPerThread];
-    double2 bpar[kMaxTermsPerThread];
-    uint64_t aabc[kMaxTermsPerThread];
-    double2 apar[kMaxTermsPerThread];
+    uint32_t bab[PPT];
+    double2 bpar[PPT];
+    uint64_t aabc[PPT];
+    double2 apar[PPT];
 #pragma unroll
-    for (int k = 0; k < kMaxTermsPerThread; ++k) {
+    for (int k = 0; k < PPT; ++k) {
         const int tb = threadIdx.x + k * kBlock;
         if (tb < nb) {
             bab[k] = A.bond_ab[b0 + tb];
@@ -103,33 +120,26 @@ __global__ void __launch_bounds__(kBlock) fused_inner_kernel(FusedArgs A) {
             apar[k] = A.ang_par[a0 + tb];
         }
     }
-    double ev[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 
     for (int it = 0; it < A.n_inner; ++it) {
-        // kicks + drift + wrap of own particles
+        // kicks + drift + wrap of own particles (each thread rewrites only
+        // its own positions, so no barrier is needed before)
 #pragma unroll
         for (int k = 0; k < PPT; ++k) {
             const int j = threadIdx.x + k * kBlock;
             if (j >= np) continue;
             for (int d = 0; d < 3; ++d) {
                 double vi = v[k][d];
-                if (it == 0) {
-                    if (A.pending) vi = vi + A.h_out * (double)fs[k][d];
-                    vi = vi + A.h_out * (double)fs[k][d];
-                } else {
-                    vi = vi + A.h_in * (double)ff[k][d];
-                }
+                if (it > 0) vi = vi + A.h_in * (double)ff[k][d];
                 vi = vi + A.h_in * (double)ff[k][d];
                 v[k][d] = vi;
-                x[k][d] = wrap_np_mod(x[k][d] + A.dt * vi, A.box[d]);
-                xs[3 * j + d] = x[k][d];
+                xs[3 * j + d] = np_mod(xs[3 * j + d] + A.dt * vi, A.box[d]);
             }
         }
         __syncthreads();
-        const bool last = (A.partials != nullptr) && (it == A.n_inner - 1);
         // every term once into its slot
 #pragma unroll
-        for (int k = 0; k < kMaxTermsPerThread; ++k) {
+        for (int k = 0; k < PPT; ++k) {
             const int tb = threadIdx.x + k * kBlock;
             if (tb < nb) {
                 const int ia = bab[k] & 0xffff, ib = bab[k] >> 16;
@@ -141,12 +151,6 @@ __global__ void __launch_bounds__(kBlock) fused_inner_kernel(FusedArgs A) {
                 sb[3 * tb] = fa.x;
                 sb[3 * tb + 1] = fa.y;
                 sb[3 * tb + 2] = fa.z;
-                if (last) {
-                    ev[0] += en;
-                    ev[1] += fa.x * rab.x;
-                    ev[2] += fa.y * rab.y;
-                    ev[3] += fa.z * rab.z;
-                }
             }
             if (tb < na) {
                 const int ia = (int)(aabc[k] & 0xffff), ib = (int)((aabc[k] >> 16) & 0xffff),
@@ -163,12 +167,6 @@ __global__ void __launch_bounds__(kBlock) fused_inner_kernel(FusedArgs A) {
                 sa[6 * tb + 3] = fc.x;
                 sa[6 * tb + 4] = fc.y;
                 sa[6 * tb + 5] = fc.z;
-                if (last) {
-                    ev[4] += en;
-                    ev[5] -= fa.x * ra.x + fc.x * rc.x;
-                    ev[6] -= fa.y * ra.y + fc.y * rc.y;
-                    ev[7] -= fa.z * ra.z + fc.z * rc.z;
-                }
             }
         }
         __syncthreads();
@@ -193,7 +191,7 @@ __global__ void __launch_bounds__(kBlock) fused_inner_kernel(FusedArgs A) {
             }
             for (int d = 0; d < 3; ++d) ff[k][d] = (float)acc[d];
         }
-        // the barrier at the top of the next sub-step orders slot reuse
+        // slots are rewritten only after the next sub-step's barrier
     }
     // second half kick of the last sub-step; write back
 #pragma unroll
@@ -202,14 +200,55 @@ __global__ void __launch_bounds__(kBlock) fused_inner_kernel(FusedArgs A) {
         if (j >= np) continue;
         const int64_t g = 3 * (int64_t)(p0 + j);
         for (int d = 0; d < 3; ++d) {
-            const double vi = v[k][d] + A.h_in * (double)ff[k][d];
-            A.v[g + d] = vi;
-            A.x[g + d] = x[k][d];
-            A.x32[g + d] = (float)x[k][d];
+            const double xi = xs[3 * j + d];
+            A.v[g + d] = v[k][d] + A.h_in * (double)ff[k][d];
+            A.x[g + d] = xi;
+            A.x32[g + d] = (float)xi;
             A.f_fast[g + d] = ff[k][d];
         }
+        if (A.key.keys) {  // the field path's row key of the float32 position
+            const CicAxis cx = cic_axis((double)(float)xs[3 * j], A.key.inv_h[0], A.key.mx);
+            const CicAxis cy = cic_axis((double)(float)xs[3 * j + 1], A.key.inv_h[1], A.key.my);
+            const int row = cx.i0 * A.key.my + cy.i0;
+            A.key.keys[p0 + j] = make_int2(row, atomicAdd(A.key.counts + row, 1));
+        }
     }
-    if (A.partials) block_reduce_store<8>(ev, A.partials + 8 * blk);
+    if (A.partials) {
+        // energies and virials at the positions of the last force evaluation
+        // (unchanged since: the final kick moves velocities only)
+        double ev[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const int tb = threadIdx.x + k * kBlock;
+            if (tb < nb) {
+                const int ia = bab[k] & 0xffff, ib = bab[k] >> 16;
+                const V3 rab = mic(v3(xs[3 * ib] - xs[3 * ia], xs[3 * ib + 1] - xs[3 * ia + 1],
+                                      xs[3 * ib + 2] - xs[3 * ia + 2]),
+                                   box);
+                double en;
+                const V3 fa = bond_term(rab, bpar[k].x, bpar[k].y, en);
+                ev[0] += en;
+                ev[1] += fa.x * rab.x;
+                ev[2] += fa.y * rab.y;
+                ev[3] += fa.z * rab.z;
+            }
+            if (tb < na) {
+                const int ia = (int)(aabc[k] & 0xffff), ib = (int)((aabc[k] >> 16) & 0xffff),
+                          ic = (int)(aabc[k] >> 32);
+                const V3 rb = v3(xs[3 * ib], xs[3 * ib + 1], xs[3 * ib + 2]);
+                const V3 ra = mic(v3(xs[3 * ia], xs[3 * ia + 1], xs[3 * ia + 2]) - rb, box);
+                const V3 rc = mic(v3(xs[3 * ic], xs[3 * ic + 1], xs[3 * ic + 2]) - rb, box);
+                V3 fa, fc;
+                double en;
+                angle_term(ra, rc, apar[k].x, apar[k].y, fa, fc, en);
+                ev[4] += en;
+                ev[5] -= fa.x * ra.x + fc.x * rc.x;
+                ev[6] -= fa.y * ra.y + fc.y * rc.y;
+                ev[7] -= fa.z * ra.z + fc.z * rc.z;
+            }
+        }
+        block_reduce_store<8>(ev, A.partials + 8 * blk);
+    }
 }
 
 }  // namespace hymd
@@ -295,25 +334,41 @@ extern "C" int hymd_fused_create(hymd_fused **out, int64_t n, int64_t m2, const 
         size[r]++;
         last[r] = (int)i;
     }
-    const int ppt = 2;
+    int max_mol = 0;
+    for (int64_t i = 0; i < n; ++i)
+        if (uf.find((int)i) == (int)i) max_mol = std::max(max_mol, size[i]);
+    const int ppt = max_mol <= kBlock ? 1 : 2;
     const int pmax = kBlock * ppt;
+    const size_t kFusedLds = ppt == 1 ? kFusedLds1 : kFusedLds2;
+    // terms per molecule (indexed by its root)
+    std::vector<int> mol_b(n, 0), mol_a(n, 0);
+    for (int64_t t = 0; t < m2; ++t) mol_b[uf.find(A2[t])]++;
+    for (int64_t t = 0; t < m3; ++t) mol_a[uf.find(A3[t])]++;
+    // Pack whole molecules into blocks of at most pmax particles, kBlock *
+    // ppt bonds and angles per thread, and kFusedLds bytes of LDS.
     std::vector<int> blk_start{0};
-    int cur = 0;
+    int cur_p = 0, cur_b = 0, cur_a = 0;
+    const int tmax = kBlock * ppt;
     for (int64_t i = 0; i < n;) {
         const int r = uf.find((int)i);
         if (r != (int)i || last[r] - r + 1 != size[r]) {
             set_error("molecule containing particle %lld is not a contiguous index range", (long long)i);
             return HYMD_ERR_UNSUPPORTED;
         }
-        if (size[r] > pmax) {
-            set_error("a molecule of %d beads exceeds the fused block size %d", size[r], pmax);
+        if (size[r] > pmax || mol_b[r] > tmax || mol_a[r] > tmax ||
+            fused_lds_bytes(size[r], mol_b[r], mol_a[r]) > kFusedLds) {
+            set_error("a molecule of %d beads, %d bonds and %d angles exceeds one fused block", size[r], mol_b[r],
+                      mol_a[r]);
             return HYMD_ERR_UNSUPPORTED;
         }
-        if (cur + size[r] > pmax) {
+        if (cur_p + size[r] > pmax || cur_b + mol_b[r] > tmax || cur_a + mol_a[r] > tmax ||
+            fused_lds_bytes(cur_p + size[r], cur_b + mol_b[r], cur_a + mol_a[r]) > kFusedLds) {
             blk_start.push_back((int)i);
-            cur = 0;
+            cur_p = cur_b = cur_a = 0;
         }
-        cur += size[r];
+        cur_p += size[r];
+        cur_b += mol_b[r];
+        cur_a += mol_a[r];
         i += size[r];
     }
     if (blk_start.back() != (int)n) blk_start.push_back((int)n);
@@ -351,7 +406,7 @@ extern "C" int hymd_fused_create(hymd_fused **out, int64_t n, int64_t m2, const 
         max_a = std::max(max_a, acount[b + 1] - acount[b]);
         max_p = std::max(max_p, blk_start[b + 1] - blk_start[b]);
     }
-    if (max_b > kBlock * kMaxTermsPerThread || max_a > kBlock * kMaxTermsPerThread) {
+    if (max_b > tmax || max_a > tmax) {
         set_error("too many bonded terms per block for the fused kernel");
         return HYMD_ERR_UNSUPPORTED;
     }
@@ -390,10 +445,12 @@ extern "C" int hymd_fused_create(hymd_fused **out, int64_t n, int64_t m2, const 
     f->max_angles = std::max(max_a, 1);
     f->max_entries = std::max(max_e, 1);
     f->max_particles = std::max(max_p, 1);
-    f->lds = sizeof(double) * (3 * f->max_particles + 3 * f->max_bonds + 6 * f->max_angles) +
-             sizeof(int) * (f->max_particles + 1) + sizeof(uint32_t) * f->max_entries;
+    f->lds = 0;
+    for (int b = 0; b < nblk; ++b)
+        f->lds = std::max(f->lds, fused_lds_bytes(blk_start[b + 1] - blk_start[b], bcount[b + 1] - bcount[b],
+                                                  acount[b + 1] - acount[b]));
     int st = HYMD_OK;
-    if (f->lds > 64 * 1024) {
+    if (f->lds > kFusedLds) {
         set_error("fused block needs %zu bytes of LDS", f->lds);
         st = HYMD_ERR_UNSUPPORTED;
     }
@@ -422,9 +479,10 @@ extern "C" int hymd_fused_destroy(hymd_fused *f) {
     return HYMD_OK;
 }
 
-extern "C" int hymd_fused_inner(hymd_fused *f, double *x, double *v, float *x32, float *f_fast, const float *f_slow,
-                                double h_in, double h_out, double dt, int32_t n_inner, int32_t pending,
-                                const double box[3], double *energies, double *virials, void *stream) {
+namespace hymd {
+int fused_inner_impl(hymd_fused *f, double *x, double *v, float *x32, float *f_fast, const float *f_slow, double h_in,
+                     double h_out, double dt, int32_t n_inner, int32_t pending, const double box[3], double *energies,
+                     double *virials, void *stream, const RowKeyTarget *key) {
     HYMD_REQUIRE(f && box && n_inner >= 1 && (f->n == 0 || (x && v && x32 && f_fast && f_slow)), "bad argument");
     hipStream_t s = (hipStream_t)stream;
     FusedArgs A;
@@ -448,13 +506,14 @@ extern "C" int hymd_fused_inner(hymd_fused *f, double *x, double *v, float *x32,
     for (int d = 0; d < 3; ++d) A.box[d] = box[d];
     A.n_inner = n_inner;
     A.pending = pending ? 1 : 0;
-    A.max_bonds = f->max_bonds;
-    A.max_angles = f->max_angles;
-    A.max_entries = f->max_entries;
-    A.max_particles = f->max_particles;
     const bool red = energies || virials;
     A.partials = red ? (double *)f->partials.ptr : nullptr;
-    if (f->nblocks > 0) fused_inner_kernel<2><<<f->nblocks, kBlock, f->lds, s>>>(A);
+    if (key) A.key = *key;
+    else A.key.keys = nullptr;
+    if (f->nblocks > 0) {
+        if (f->ppt == 1) fused_inner_kernel<1><<<f->nblocks, kBlock, f->lds, s>>>(A);
+        else fused_inner_kernel<2><<<f->nblocks, kBlock, f->lds, s>>>(A);
+    }
     HYMD_HIP(hipGetLastError());
     if (red) {
         HYMD_TRY(reduce_partials((const double *)f->partials.ptr, f->nblocks, 8, (double *)f->result.ptr, s));
@@ -472,4 +531,12 @@ extern "C" int hymd_fused_inner(hymd_fused *f, double *x, double *v, float *x32,
             }
     }
     return HYMD_OK;
+}
+}  // namespace hymd
+
+extern "C" int hymd_fused_inner(hymd_fused *f, double *x, double *v, float *x32, float *f_fast, const float *f_slow,
+                                double h_in, double h_out, double dt, int32_t n_inner, int32_t pending,
+                                const double box[3], double *energies, double *virials, void *stream) {
+    return fused_inner_impl(f, x, v, x32, f_fast, f_slow, h_in, h_out, dt, n_inner, pending, box, energies, virials,
+                            stream, nullptr);
 }

@@ -8,14 +8,6 @@
 
 namespace hymd {
 
-// np.mod(x, L) for L > 0: fmod, then shift negatives up by L (can return L
-// itself for tiny negative x, as numpy does).
-__device__ __forceinline__ double np_mod(double x, double L) {
-    double r = fmod(x, L);
-    if (r != 0.0 && r < 0.0) r += L;
-    return r;
-}
-
 __global__ void __launch_bounds__(kBlock) kick_drift_kernel(double *__restrict__ x, double *__restrict__ v,
                                                              float *__restrict__ x32, int64_t n,
                                                              const float *__restrict__ ff, double cf,
@@ -29,7 +21,7 @@ __global__ void __launch_bounds__(kBlock) kick_drift_kernel(double *__restrict__
         double vi = v[i];
         if (fs) vi = vi + cs * (double)fs[i];
         vi = vi + cf * (double)ff[i];
-        const double xi = np_mod(x[i] + dt * vi, L);
+        const double xi = np_mod(x[i] + dt * vi, L);  // common.h
         v[i] = vi;
         x[i] = xi;
         if (x32) x32[i] = (float)xi;

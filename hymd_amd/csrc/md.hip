@@ -53,16 +53,19 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
         // applied by the next step's kernel (or by finish_pending)
         double eb[2];
         mark(m, 1);
-        HYMD_TRY(hymd_fused_inner(m->fused, m->x, m->v, m->x32, m->f_fast, m->f_field, half_in, half_out, m->dt,
-                                  m->respa_inner, pending, m->box, energies ? eb : nullptr, nullptr, m->work));
+        // the inner loop also writes the field path's row keys of the new positions
+        RowKeyTarget key;
+        HYMD_TRY(field_key_target(m->pm, m->n, &key));
+        HYMD_TRY(fused_inner_impl(m->fused, m->x, m->v, m->x32, m->f_fast, m->f_field, half_in, half_out, m->dt,
+                                  m->respa_inner, pending, m->box, energies ? eb : nullptr, nullptr, m->work, &key));
         mark(m, 1);
         if (energies) {
             energies[1] = eb[0];
             energies[2] = eb[1];
         }
         mark(m, 3);
-        HYMD_TRY(hymd_update_field(m->pm, m->x32, m->types, m->n, m->T, m->A, m->b, m->c0, m->mass_t, m->sigma,
-                                   m->force_mesh, nullptr, nullptr, nullptr, energies ? &energies[0] : nullptr));
+        HYMD_TRY(update_field_keyed(m->pm, m->x32, m->types, m->n, m->T, m->A, m->b, m->c0, m->mass_t, m->sigma,
+                                    m->force_mesh, energies ? &energies[0] : nullptr));
         mark(m, 3);
         mark(m, 4);
         HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));

@@ -28,6 +28,12 @@ mass motion removed.  numpy PCG64 streams, seed = 1000 + workload index.
                      (side beads and solvent) carry +-1 with zero net charge;
                      dielectric 45; chi 4x4 symmetric U[0,30]; kappa 0.03;
                      sigma 0.5; dt 0.03; respa_inner 1.
+* ``replica_set``    64 independent A5B5 diblock melts for the batched
+                     chi-gradient workload: N_r ~ U[50k, 400k] rounded down to
+                     whole 10-bead chains, L_r = (N_r / 8.4)^(1/3) nm, mesh
+                     edge the one of {48, 64, 96} nearest L_r / 0.6 nm (only
+                     48 and 64 occur), chi_AB ~ U[5, 25], sigma ~ U[0.4, 0.7];
+                     kappa = 0.05 (left open by the workload, fixed here).
 All accept smaller chain / solvent counts for fast parity cases.
 """
 from dataclasses import dataclass, field
@@ -200,6 +206,48 @@ def peptide_solvent(n_chains=8738, n_res=40, n_solvent=1048592, box=50.0, mesh=1
                0.03, 0.5, 0.03, 1, 500, charges=charges, dielectric=45.0)
     s.extra["dihedrals"] = dihedrals
     return s
+
+
+@dataclass
+class ReplicaSet:
+    counts: np.ndarray      # (R,) particles per replica
+    meshes: np.ndarray      # (R,3) int
+    boxes: np.ndarray       # (R,3) nm
+    positions: np.ndarray   # (sum N_r, 3) float64, replica after replica
+    types: np.ndarray       # (sum N_r,) int32 (0 = A, 1 = B)
+    chi: np.ndarray         # (R,2,2) symmetric, zero diagonal
+    sigma: np.ndarray       # (R,)
+    kappa: np.ndarray       # (R,)
+
+    @property
+    def n_replicas(self):
+        return len(self.counts)
+
+    def replica(self, r):
+        """(positions, types) of replica r."""
+        o = np.concatenate([[0], np.cumsum(self.counts)])
+        return self.positions[o[r]:o[r + 1]], self.types[o[r]:o[r + 1]]
+
+
+def replica_set(n_replicas=64, n_min=50_000, n_max=400_000, meshes=(48, 64, 96), spacing=0.6, density=8.4,
+                seed=1003):
+    rng = np.random.default_rng(seed)
+    counts = (rng.integers(n_min, n_max + 1, n_replicas) // 10) * 10
+    L = (counts / density) ** (1.0 / 3.0)
+    choice = np.asarray(meshes)
+    mesh = choice[np.argmin(np.abs(choice[None, :] - (L / spacing)[:, None]), axis=1)]
+    chi_ab = rng.uniform(5.0, 25.0, n_replicas)
+    sigma = rng.uniform(0.4, 0.7, n_replicas)
+    xs, ts = [], []
+    for r in range(n_replicas):
+        box = np.full(3, L[r])
+        xs.append(random_walk_chains(rng, int(counts[r]) // 10, 10, box))
+        ts.append(np.tile(np.array([0] * 5 + [1] * 5, dtype=np.int32), int(counts[r]) // 10))
+    chi = np.zeros((n_replicas, 2, 2))
+    chi[:, 0, 1] = chi[:, 1, 0] = chi_ab
+    return ReplicaSet(counts.astype(np.int64), np.repeat(mesh[:, None], 3, axis=1).astype(np.int32),
+                      np.repeat(L[:, None], 3, axis=1), np.concatenate(xs) if xs else np.zeros((0, 3)),
+                      np.concatenate(ts) if ts else np.zeros(0, np.int32), chi, sigma, np.full(n_replicas, 0.05))
 
 
 def hamiltonian_form(system):

@@ -112,6 +112,24 @@ int hymd_field_moments(hymd_pm *pm, const float *x, const int32_t *types, int64_
                        const double *mass, double sigma, double *moments, const double *coupling,
                        const double *shift, float *force_mesh);
 
+/* Batched replicas (no reference counterpart: the reference runs one system
+ * per process).  n_replicas independent systems, replica r owning particles
+ * [offsets[r], offsets[r+1]) of the caller's arrays, mesh[3r..3r+2],
+ * box[3r..3r+2] and one mass per type (host arrays, fixed for the handle).
+ * hymd_batch_moments paints, transforms and reduces every replica to the
+ * moments of hymd_field_moments (n_types <= 8; row r of `moments` holds
+ * replica r's 2 T(T+1)/2 + T doubles, bit-identical to hymd_field_moments on
+ * that replica alone) for per-replica filter widths sigma[r].  With use_graph
+ * the launch sequence is captured on the first call (and again when x or
+ * types change) and replayed.  The work runs on the handle's own stream after
+ * the caller's `stream`; the call synchronises before returning. */
+typedef struct hymd_batch hymd_batch;
+int hymd_batch_create(hymd_batch **out, int32_t n_replicas, const int32_t *mesh, const double *box,
+                      const int64_t *offsets, int32_t n_types, const double *mass);
+int hymd_batch_destroy(hymd_batch *b);
+int hymd_batch_moments(hymd_batch *b, const float *x, const int32_t *types, const double *sigma, double *moments,
+                       int32_t use_graph, void *stream);
+
 /* compute_self_energy_q (field.py:203-238): host double out */
 int hymd_self_energy_q(const float *charges, int64_t n, double sigma, double elec_factor, double *out,
                        void *stream);

@@ -1,0 +1,110 @@
+"""Batched replicas against each replica run alone, and their gradients
+against central finite differences of the FP64 oracle energy.
+
+Parity bar: the batched moments equal the single-replica path's bit for bit
+(same kernels, plans, block counts and reduction order); the chi / kappa /
+sigma derivatives match central differences (h = 1e-3 relative) of the FP64
+oracle energy to 1e-5 relative (sigma: 1e-4; the FP32 spectra limit both).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import field as F
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_set():
+    from hymd_amd import synth
+    rs = synth.replica_set(n_replicas=7, n_min=2_000, n_max=12_000, meshes=(16, 24, 32), spacing=0.6, seed=21)
+    # three mesh shapes (three FFT groups, interleaved in replica order), one
+    # replica with a non-cubic mesh and box, one empty replica
+    for r, m in enumerate([16, 24, 16, 32, 24, 16, 32]):
+        rs.meshes[r] = m
+    rs.meshes[2] = (16, 24, 20)
+    rs.boxes[2] = rs.boxes[2] * np.array([0.9, 1.1, 1.0])
+    x2, _ = rs.replica(2)
+    o = np.concatenate([[0], np.cumsum(rs.counts)])
+    rs.positions[o[2]:o[3]] = np.mod(x2, rs.boxes[2])
+    keep = np.ones(len(rs.types), bool)
+    keep[o[4]:o[5]] = False
+    rs.positions, rs.types = rs.positions[keep], rs.types[keep]
+    rs.counts[4] = 0
+    return rs
+
+
+def _batch(rs):
+    from hymd_amd.batch import ReplicaBatch
+    return ReplicaBatch(rs.meshes, rs.boxes, rs.counts, 2)
+
+
+def _alone(rs, r, sigma):
+    from hymd_amd.autograd import moments
+    from hymd_amd.field import ParticleMesh
+    x, t = rs.replica(r)
+    pm = ParticleMesh(tuple(int(m) for m in rs.meshes[r]), rs.boxes[r], 2)
+    return moments(pm, torch.as_tensor(x.astype(np.float32)).cuda(), torch.as_tensor(t).cuda(), 2, sigma)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_equals_each_replica_alone(hip, use_graph):
+    rs = _small_set()
+    b = _batch(rs)
+    x = torch.as_tensor(rs.positions.astype(np.float32)).cuda()
+    t = torch.as_tensor(rs.types).cuda()
+    for sigma in (rs.sigma, rs.sigma * 1.25):   # a graph replay picks up new filter widths
+        got = b.moments(x, t, sigma, use_graph=use_graph)
+        for r in range(rs.n_replicas):
+            P, Q, phi0 = _alone(rs, r, sigma[r])
+            np.testing.assert_array_equal(got[r][0], P)
+            np.testing.assert_array_equal(got[r][1], Q)
+            np.testing.assert_array_equal(got[r][2], phi0)
+    assert np.all(got[4][0] == 0.0)    # the empty replica
+    b.close()
+
+
+def _oracle_energy(x, types, mesh, box, chi, kappa, sigma):
+    rho0 = len(types) / np.prod(box)
+    ham = F.default_with_chi(2, kappa, rho0, rho0, chi)
+    out = F.update_field(x, types, 2, mesh, box, sigma, ham)
+    return F.field_energy(out["phi"], ham, mesh, box)
+
+
+def test_batch_gradients_vs_fd(hip):
+    rs = _small_set()
+    b = _batch(rs)
+    x32 = rs.positions.astype(np.float32)
+    E, dchi, dk, ds = b.energy_and_gradients(torch.as_tensor(x32).cuda(), torch.as_tensor(rs.types).cuda(),
+                                             rs.chi, rs.kappa, rs.sigma)
+    h = 1e-3
+    for r in (0, 2, 5):
+        x, t = rs.replica(r)
+        x = x.astype(np.float32).astype(np.float64)
+        mesh, box = tuple(int(m) for m in rs.meshes[r]), rs.boxes[r]
+        chi, k, sg = rs.chi[r], rs.kappa[r], rs.sigma[r]
+        assert E[r] == pytest.approx(_oracle_energy(x, t, mesh, box, chi, k, sg), rel=1e-6)
+        c = chi[0, 1]
+        cp, cm = chi.copy(), chi.copy()
+        cp[0, 1] = cp[1, 0] = c * (1 + h)
+        cm[0, 1] = cm[1, 0] = c * (1 - h)
+        fd = (_oracle_energy(x, t, mesh, box, cp, k, sg) - _oracle_energy(x, t, mesh, box, cm, k, sg)) / (2 * c * h)
+        assert dchi[r, 0, 1] == pytest.approx(fd, rel=1e-5, abs=1e-6 * abs(E[r]))
+        fd = (_oracle_energy(x, t, mesh, box, chi, k * (1 + h), sg)
+              - _oracle_energy(x, t, mesh, box, chi, k * (1 - h), sg)) / (2 * k * h)
+        assert dk[r] == pytest.approx(fd, rel=1e-5)
+        fd = (_oracle_energy(x, t, mesh, box, chi, k, sg * (1 + h))
+              - _oracle_energy(x, t, mesh, box, chi, k, sg * (1 - h))) / (2 * sg * h)
+        assert ds[r] == pytest.approx(fd, rel=1e-4)
+    b.close()
+
+
+def test_batch_rejects_bad_input(hip):
+    from hymd_amd._lib import HymdError
+    from hymd_amd.batch import ReplicaBatch
+    with pytest.raises(HymdError):
+        ReplicaBatch([(16, 16, 15)], [(5.0, 5.0, 5.0)], [10], 2)     # odd mesh edge
+    b = ReplicaBatch([(16, 16, 16)], [(5.0, 5.0, 5.0)], [10], 2)
+    with pytest.raises(HymdError):
+        b.moments(torch.zeros((9, 3), device="cuda"), torch.zeros(9, dtype=torch.int32, device="cuda"), 0.5)
+    b.close()

@@ -54,8 +54,10 @@ def test_paint_empty(hip):
     assert int(pm.paint_fixed(x, t, 1).abs().sum()) == 0
 
 
+# (16, 200, 200): its y-z plane exceeds the LDS budget, so the separate line
+# passes run; the other meshes use the plane kernels
 @pytest.mark.parametrize("mesh", [(8, 8, 8), (32, 32, 32), (48, 64, 96), (128, 128, 128), (160, 160, 160),
-                                  (20, 30, 36)])
+                                  (20, 30, 36), (16, 200, 200)])
 def test_fft_vs_numpy(hip, mesh):
     rng = np.random.default_rng(sum(mesh))
     pm = _pm(mesh, (1.0, 1.0, 1.0), 1)
@@ -216,3 +218,30 @@ def test_drop_in_field_surface(hip):
     assert e_q == 0.0
     for t in range(2):
         assert rms_rel(phi[t].cpu().numpy(), ref["phi"][t]) < 1e-6
+
+
+@pytest.mark.parametrize("mesh,T", [((32, 48, 40), 3), ((24, 24, 24), 2), ((16, 20, 12), 5)])
+def test_fused_xline_pass_equals_unfused(hip, mesh, T):
+    """The force-grid-only update (forward x pass fused with the k-space pass)
+    against the unfused passes, which run when v_ext is requested.  Same
+    formulas in both; the compiler may contract different products into FMAs
+    in the two kernels, so the grids agree to FP32 rounding, not bit for bit."""
+    rng = np.random.default_rng(5)
+    box = (6.0, 7.5, 5.0)
+    n = 6000
+    x = (rng.uniform(0, 1, (n, 3)) * np.array(box)).astype(np.float32)
+    types = rng.integers(0, T, n).astype(np.int32)
+    A = rng.uniform(0.0, 2.0, (T, T))
+    A = A + A.T
+    b = rng.uniform(-1.0, 0.0, T)
+    pm = _pm(mesh, box, T)
+    xd, td = _dev(x, torch.float32), _dev(types, torch.int32)
+    f1 = pm.create("real", batch=(T, 3))
+    f2 = pm.create("real", batch=(T, 3))
+    v = pm.create("real", batch=T)
+    e1 = pm.field_update(xd, td, T, A, b, 0.3, [1.0] * T, 0.45, f1, want_energy=True)
+    e2 = pm.field_update(xd, td, T, A, b, 0.3, [1.0] * T, 0.45, f2, v_ext=v, want_energy=True)
+    a1, a2 = f1.cpu().numpy(), f2.cpu().numpy()
+    assert np.abs(a1 - a2).max() <= 1e-6 * np.abs(a2).max()
+    assert rms_rel(a1, a2) < 2e-7
+    assert e1 == pytest.approx(e2, rel=1e-9)

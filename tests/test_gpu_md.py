@@ -67,8 +67,11 @@ def test_wrap_matches_np_mod(hip):
     rng = np.random.default_rng(0)
     x0 = rng.uniform(-10, 10, (1000, 3))
     x0[0] = [-1e-20, -1e-17, 0.0]
+    x0[1] = [3.0, -4.0, 10.0]          # L, -L, 2L: zero, returned as +0
+    x0[2] = [-0.0, 8.0, -5.0]
+    x0[3] = [2.9999999999999996, 7.999999999999999, -4.999999999999999]
     v0 = rng.normal(size=(1000, 3))
-    v0[0] = 0.0
+    v0[:4] = 0.0
     x = torch.as_tensor(x0.copy()).cuda()
     v = torch.as_tensor(v0.copy()).cuda()
     zero = torch.zeros((1000, 3), dtype=torch.float32, device="cuda")
@@ -77,6 +80,8 @@ def test_wrap_matches_np_mod(hip):
     # np.mod maps a tiny negative coordinate to L itself (main.py:837)
     np.testing.assert_array_equal(got[0], np.mod(x0[0], box))
     assert got[0, 0] == 3.0 and got[0, 1] == 4.0
+    np.testing.assert_array_equal(got[1:4], np.mod(x0[1:4], box))
+    assert not np.signbit(got[1:3]).any()
     # elsewhere x + dt v may differ by one rounding (fused multiply-add)
     np.testing.assert_allclose(got, np.mod(x0 + 0.01 * v0, box[None, :]), rtol=0, atol=1e-14)
     assert (got >= 0).all() and (got <= box).all()
