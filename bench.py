@@ -201,11 +201,24 @@ def peptide_solvent_stages(reps, check):
         ev1.record(st)
         torch.cuda.synchronize()
         ms[name] = ev0.elapsed_time(ev1) / reps
+    # the whole outer step (respa_inner 1): the library's captured step
+    from hymd_amd.md import RespaMD
+    md = RespaMD(s)
+    md.run(2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    md.run(reps)
+    torch.cuda.synchronize()
+    step_ms = 1e3 * (time.perf_counter() - t0) / reps
+    md.close()
     out = {"workload": "peptide_solvent: %d beads (%d chains x 40 residues x 3 beads + solvent), %.0f nm box, "
                        "%d^3 mesh for density and PME, %d types, %d charges, %d bonds, %d angles, %d CBT "
                        "dihedrals with dipoles" % (n, 8738, s.box[0], s.mesh[0], T, int(np.count_nonzero(s.charges)),
                                                    len(s.bonds[0]), len(s.angles[0]), m4),
-           "device_ms": ms, "device_ms_outer_step_forces": sum(ms.values())}
+           "device_ms": ms, "device_ms_outer_step_forces": sum(ms.values()),
+           "md_ms_per_outer_step": step_ms, "md_particle_steps_per_s": n / (step_ms * 1e-3),
+           "md_path": "RespaMD.run(): one captured outer step (per-kernel inner loop with dihedrals, PME of "
+                      "the charges and of the dipoles) replayed as a hipGraph"}
     if check:
         from oracle import bonded as OB
         from oracle import field as OF
@@ -350,7 +363,8 @@ def main():
         torch.cuda.empty_cache()
         log("peptide/solvent force stages")
         peptide = peptide_solvent_stages(args.peptide_reps, check=args.cpu_steps > 0)
-        log(f"peptide/solvent: {peptide['device_ms_outer_step_forces']:.3f} ms of device force stages")
+        log(f"peptide/solvent: {peptide['device_ms_outer_step_forces']:.3f} ms of device force stages, "
+            f"{peptide['md_ms_per_outer_step']:.3f} ms per outer step")
 
     out = {
         "metric": METRIC,

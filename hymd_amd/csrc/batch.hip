@@ -68,13 +68,18 @@ __global__ void __launch_bounds__(kBlock) batch_key_kernel(const float *__restri
                                                             const RepGeom *__restrict__ geo, int2 *__restrict__ keys,
                                                             int *__restrict__ counts) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int r = find_range(poff, R, i);
-        const RepGeom &G = geo[slot_of[r]];
-        const CicAxis cx = cic_axis((double)x[3 * i + 0], G.inv_h[0], G.mx);
-        const CicAxis cy = cic_axis((double)x[3 * i + 1], G.inv_h[1], G.my);
-        const int key = G.row_off + cx.i0 * G.my + cy.i0;
-        keys[i] = make_int2(key, atomicAdd(counts + key, 1));
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {  // uniform per wavefront
+        const int64_t i = i0 + threadIdx.x;
+        int key = -1;
+        if (i < n) {
+            const int r = find_range(poff, R, i);
+            const RepGeom &G = geo[slot_of[r]];
+            const CicAxis cx = cic_axis((double)x[3 * i + 0], G.inv_h[0], G.mx);
+            const CicAxis cy = cic_axis((double)x[3 * i + 1], G.inv_h[1], G.my);
+            key = G.row_off + cx.i0 * G.my + cy.i0;
+        }
+        const int rank = rank_in_bin(key, counts);
+        if (i < n) keys[i] = make_int2(key, rank);
     }
 }
 

@@ -6,6 +6,7 @@
 // (term, role) so each particle sums its contributions in the reference's
 // term order.  One thread per particle then recomputes its terms in FP64.
 #include <cmath>
+#include <memory>
 #include <mutex>
 
 #include "bonded.h"
@@ -213,14 +214,13 @@ struct DihArgs {
     int dipole_flag;
 };
 
-// One dihedral term (compute_dihedral_forces.f90:35-134) evaluated for the
-// particle in `role`; the role-0 thread also owns the term's energy, clamp
-// count and dipole / transfer-matrix outputs.
+// One dihedral term (compute_dihedral_forces.f90:35-134): the force on each of
+// its four particles (roles 0..3 = a, b, c, d), its energy and clamp count,
+// and its dipoles / transfer matrices (zero where the term has none, as the
+// reference's zeroed outputs).
 template <typename PosT>
-__device__ V3 dihedral_contrib(const PosT *__restrict__ r, const DihArgs &D, uint32_t e, const Box &box, double *ev,
-                               PosT *__restrict__ dipoles, PosT *__restrict__ tmat) {
-    const int64_t t = e >> 2;
-    const int role = e & 3;
+__device__ void dihedral_term(const PosT *__restrict__ r, const DihArgs &D, int64_t t, const Box &box, double *ev,
+                              PosT *__restrict__ dipoles, PosT *__restrict__ tmat, V3 (&out)[4]) {
     const V3 ra = load3(r, D.a[t]), rb = load3(r, D.b[t]), rc = load3(r, D.c[t]), rd = load3(r, D.d[t]);
     const V3 f = mic(ra - rb, box), g = mic(rb - rc, box), h = mic(rd - rc, box);
     const V3 v = cross(f, g), w = cross(h, g);
@@ -232,7 +232,7 @@ __device__ V3 dihedral_contrib(const PosT *__restrict__ r, const DihArgs &D, uin
     const float *cf = D.coeff + 30 * t;
     const int dt = D.dtype[t];
     double energy = 0.0, df = 0.0;
-    V3 out = v3(0, 0, 0);
+    for (int q = 0; q < 4; ++q) out[q] = v3(0, 0, 0);
     if (dt == 0 || dt == 1) {
         cosine_series(cf, cf + 5, phi, energy, df);
         int zc = 0, zd = 0;
@@ -242,51 +242,56 @@ __device__ V3 dihedral_contrib(const PosT *__restrict__ r, const DihArgs &D, uin
         }
         if (zc != 5 && zd != 5) cosine_series(cf + 10, cf + 15, phi, energy, df);
     }
-    const bool own = (role == 0);
+    bool dip0 = false, dip1 = false;
     if (dt == 1) {
         Recon rec;
         reconstruct(f, rb, -g, box, cf + 20, cf + 25, phi, D.dipole_flag, rec);
         energy += rec.energy;
         df += rec.df;
-        if (role == 0) out = out - rec.fa;
-        if (role == 1) out = out - rec.fb;
-        if (role == 2) out = out - rec.fc;
-        if (own) {
-            ev[1] += rec.clamped ? 1.0 : 0.0;
-            if (rec.has_dipole) {
-                for (int p = 0; p < 2; ++p) {
-                    dipoles[12 * t + 3 * p + 0] = (PosT)rec.dip[p].x;
-                    dipoles[12 * t + 3 * p + 1] = (PosT)rec.dip[p].y;
-                    dipoles[12 * t + 3 * p + 2] = (PosT)rec.dip[p].z;
-                }
-                for (int x = 0; x < 3; ++x)
-                    for (int i = 0; i < 3; ++i)
-                        for (int j = 0; j < 3; ++j) tmat[54 * t + 9 * x + 3 * i + j] = (PosT)rec.tm[x][i][j];
+        out[0] = out[0] - rec.fa;
+        out[1] = out[1] - rec.fb;
+        out[2] = out[2] - rec.fc;
+        ev[1] += rec.clamped ? 1.0 : 0.0;
+        if (rec.has_dipole) {
+            dip0 = true;
+            for (int p = 0; p < 2; ++p) {
+                dipoles[12 * t + 3 * p + 0] = (PosT)rec.dip[p].x;
+                dipoles[12 * t + 3 * p + 1] = (PosT)rec.dip[p].y;
+                dipoles[12 * t + 3 * p + 2] = (PosT)rec.dip[p].z;
             }
+            for (int x = 0; x < 3; ++x)
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) tmat[54 * t + 9 * x + 3 * i + j] = (PosT)rec.tm[x][i][j];
         }
         if (D.last[t] == 1) {
             Recon rec2;
             reconstruct(g, rc, h, box, cf + 20, cf + 25, phi, D.dipole_flag, rec2);
             energy += rec2.energy;
             df += rec2.df;
-            if (role == 1) out = out - rec2.fa;
-            if (role == 2) out = out - rec2.fb;
-            if (role == 3) out = out - rec2.fc;
-            if (own) {
-                ev[1] += rec2.clamped ? 1.0 : 0.0;
-                if (rec2.has_dipole) {
-                    for (int p = 0; p < 2; ++p) {
-                        dipoles[12 * t + 6 + 3 * p + 0] = (PosT)rec2.dip[p].x;
-                        dipoles[12 * t + 6 + 3 * p + 1] = (PosT)rec2.dip[p].y;
-                        dipoles[12 * t + 6 + 3 * p + 2] = (PosT)rec2.dip[p].z;
-                    }
-                    for (int x = 0; x < 3; ++x)
-                        for (int i = 0; i < 3; ++i)
-                            for (int j = 0; j < 3; ++j)
-                                tmat[54 * t + 27 + 9 * x + 3 * i + j] = (PosT)rec2.tm[x][i][j];
+            out[1] = out[1] - rec2.fa;
+            out[2] = out[2] - rec2.fb;
+            out[3] = out[3] - rec2.fc;
+            ev[1] += rec2.clamped ? 1.0 : 0.0;
+            if (rec2.has_dipole) {
+                dip1 = true;
+                for (int p = 0; p < 2; ++p) {
+                    dipoles[12 * t + 6 + 3 * p + 0] = (PosT)rec2.dip[p].x;
+                    dipoles[12 * t + 6 + 3 * p + 1] = (PosT)rec2.dip[p].y;
+                    dipoles[12 * t + 6 + 3 * p + 2] = (PosT)rec2.dip[p].z;
                 }
+                for (int x = 0; x < 3; ++x)
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) tmat[54 * t + 27 + 9 * x + 3 * i + j] = (PosT)rec2.tm[x][i][j];
             }
         }
+    }
+    if (!dip0) {
+        for (int k = 0; k < 6; ++k) dipoles[12 * t + k] = (PosT)0;
+        for (int k = 0; k < 27; ++k) tmat[54 * t + k] = (PosT)0;
+    }
+    if (!dip1) {
+        for (int k = 6; k < 12; ++k) dipoles[12 * t + k] = (PosT)0;
+        for (int k = 27; k < 54; ++k) tmat[54 * t + k] = (PosT)0;
     }
     if (dt == 2) {
         const double eq = (double)cf[0], kf = (double)cf[1];
@@ -296,30 +301,54 @@ __device__ V3 dihedral_contrib(const PosT *__restrict__ r, const DihArgs &D, uin
     const V3 sc = (f_dot_g / (v_sq * g_norm)) * v - (h_dot_g / (w_sq * g_norm)) * w;
     const V3 fa = (-df * g_norm / v_sq) * v;
     const V3 fd = (df * g_norm / w_sq) * w;
-    if (role == 0) out = out + fa;
-    if (role == 1) out = out + (df * sc - fa);
-    if (role == 2) out = out + (-df * sc - fd);
-    if (role == 3) out = out + fd;
-    if (own) ev[0] += energy;
-    return out;
+    out[0] = out[0] + fa;
+    out[1] = out[1] + (df * sc - fa);
+    out[2] = out[2] + (-df * sc - fd);
+    out[3] = out[3] + fd;
+    ev[0] += energy;
 }
 
+// One thread per dihedral: its four particle forces into slots (term-major,
+// 4 roles x 3 doubles), energies and clamps into per-block partials.
 template <typename PosT>
-__global__ void __launch_bounds__(kBlock) dihedral_gather_kernel(const PosT *__restrict__ r, PosT *__restrict__ f,
-                                                                  int64_t n, const int *__restrict__ off,
-                                                                  const uint32_t *__restrict__ ent, DihArgs D, Box box,
-                                                                  PosT *__restrict__ dipoles, PosT *__restrict__ tmat,
-                                                                  double *__restrict__ partials) {
+__global__ void __launch_bounds__(kBlock) dihedral_term_kernel(const PosT *__restrict__ r, int64_t m, DihArgs D,
+                                                                Box box, double *__restrict__ slots,
+                                                                PosT *__restrict__ dipoles, PosT *__restrict__ tmat,
+                                                                double *__restrict__ partials) {
     double ev[2] = {0, 0};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < m; t += stride) {
+        V3 out[4];
+        dihedral_term(r, D, t, box, ev, dipoles, tmat, out);
+        double *sl = slots + 12 * t;
+        for (int q = 0; q < 4; ++q) {
+            sl[3 * q] = out[q].x;
+            sl[3 * q + 1] = out[q].y;
+            sl[3 * q + 2] = out[q].z;
+        }
+    }
+    if (partials) block_reduce_store<2>(ev, partials + 2 * blockIdx.x);
+}
+
+// Per particle: the slots of its dihedral entries summed in term order (the
+// reference's per-particle order), every particle written (zero if none).
+template <typename PosT>
+__global__ void __launch_bounds__(kBlock) dihedral_slot_gather_kernel(PosT *__restrict__ f, int64_t n,
+                                                                       const int *__restrict__ off,
+                                                                       const uint32_t *__restrict__ ent,
+                                                                       const double *__restrict__ slots) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) {
         V3 acc = v3(0, 0, 0);
-        for (int j = off[p]; j < off[p + 1]; ++j) acc = acc + dihedral_contrib(r, D, ent[j], box, ev, dipoles, tmat);
+        for (int j = off[p]; j < off[p + 1]; ++j) {
+            const uint32_t e = ent[j];
+            const double *sl = slots + 12 * (int64_t)(e >> 2) + 3 * (e & 3);
+            acc = acc + v3(sl[0], sl[1], sl[2]);
+        }
         f[3 * p] = (PosT)acc.x;
         f[3 * p + 1] = (PosT)acc.y;
         f[3 * p + 2] = (PosT)acc.z;
     }
-    if (partials) block_reduce_store<2>(ev, partials + 2 * blockIdx.x);
 }
 
 // force.py:855-877 as a gather over the dihedral lists
@@ -369,6 +398,7 @@ __global__ void __launch_bounds__(kBlock) redistribution_kernel(PosT *__restrict
 struct Workspace {
     std::mutex mu;
     Csr csr;
+    DevBuf slots;  // per-term force contributions (dihedrals: 4 roles x 3 doubles)
     DevBuf partials, result;
     HostBuf host;
 };
@@ -472,30 +502,35 @@ extern "C" int hymd_cdf(void *f, const void *r, void *dipoles, void *transfer_ma
     hipStream_t s = (hipStream_t)stream;
     Workspace &w = workspace();
     std::lock_guard<std::mutex> lock(w.mu);
-    const size_t esz = precision == HYMD_F32 ? sizeof(float) : sizeof(double);
-    if (m > 0) {
-        HYMD_HIP(hipMemsetAsync(dipoles, 0, esz * 12 * m, s));
-        HYMD_HIP(hipMemsetAsync(transfer_matrix, 0, esz * 54 * m, s));
-    }
     TermIdx ti{{a, b, c, d}, 4};
     HYMD_TRY(build_csr(w.csr, ti, m, n, s));
-    const int nb = grid_for(n, kBlock, 2048);
+    HYMD_TRY(w.slots.reserve(sizeof(double) * 12 * (size_t)std::max<int64_t>(m, 1)));
+    const int nb = grid_for(m, kBlock, 2048);
     const bool red = energy || n_clamped;
     if (red) HYMD_TRY(w.partials.reserve(sizeof(double) * 2 * nb));
     double *part = red ? (double *)w.partials.ptr : nullptr;
     DihArgs D{a, b, c, d, dtype, bb_index, coeff, dipole_flag};
     Box bx = make_box(box);
-    if (precision == HYMD_F32)
-        dihedral_gather_kernel<float><<<nb, kBlock, 0, s>>>((const float *)r, (float *)f, n, w.csr.off(), w.csr.ent(),
-                                                            D, bx, (float *)dipoles, (float *)transfer_matrix, part);
-    else
-        dihedral_gather_kernel<double><<<nb, kBlock, 0, s>>>((const double *)r, (double *)f, n, w.csr.off(),
-                                                             w.csr.ent(), D, bx, (double *)dipoles,
-                                                             (double *)transfer_matrix, part);
+    double *slots = (double *)w.slots.ptr;
+    if (precision == HYMD_F32) {
+        if (m > 0)
+            dihedral_term_kernel<float><<<nb, kBlock, 0, s>>>((const float *)r, m, D, bx, slots, (float *)dipoles,
+                                                              (float *)transfer_matrix, part);
+        if (n > 0)
+            dihedral_slot_gather_kernel<float><<<grid_for(n), kBlock, 0, s>>>((float *)f, n, w.csr.off(),
+                                                                            w.csr.ent(), slots);
+    } else {
+        if (m > 0)
+            dihedral_term_kernel<double><<<nb, kBlock, 0, s>>>((const double *)r, m, D, bx, slots, (double *)dipoles,
+                                                               (double *)transfer_matrix, part);
+        if (n > 0)
+            dihedral_slot_gather_kernel<double><<<grid_for(n), kBlock, 0, s>>>((double *)f, n, w.csr.off(),
+                                                                             w.csr.ent(), slots);
+    }
     HYMD_HIP(hipGetLastError());
     if (red) {
-        double out[2];
-        HYMD_TRY(finish_reduce(w, nb, 2, out, s));
+        double out[2] = {0.0, 0.0};
+        if (m > 0) HYMD_TRY(finish_reduce(w, nb, 2, out, s));
         if (energy) *energy = out[0];
         if (n_clamped) *n_clamped = (int64_t)out[1];
     }
@@ -624,5 +659,82 @@ extern "C" int hymd_topology_forces(hymd_topology *top, const double *r, float *
                 virials[3 + d] = h[5 + d];
             }
     }
+    return HYMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Persistent dihedral term list: per-particle lists built once, term arrays
+// referenced (not copied), FP64 forces without per-call synchronisation.
+// ---------------------------------------------------------------------------
+struct hymd_dihedrals {
+    int64_t n = 0, m = 0;
+    Csr csr;
+    DihArgs D{};
+    DevBuf slots, partials, result;
+    HostBuf host;
+    int nblocks = 1;
+};
+
+extern "C" int hymd_dihedrals_create(hymd_dihedrals **out, int64_t n, int64_t m, const int32_t *a, const int32_t *b,
+                                     const int32_t *c, const int32_t *d, const float *coeff, const int32_t *dtype,
+                                     const int32_t *last, void *stream) {
+    HYMD_REQUIRE(out && n >= 0 && m >= 0, "bad argument");
+    HYMD_REQUIRE(m == 0 || (a && b && c && d && coeff && dtype && last), "null term array");
+    hipStream_t s = (hipStream_t)stream;
+    std::unique_ptr<hymd_dihedrals> h(new hymd_dihedrals());
+    h->n = n;
+    h->m = m;
+    h->D = DihArgs{a, b, c, d, dtype, last, coeff, 0};
+    TermIdx ti{{a, b, c, d}, 4};
+    HYMD_TRY(build_csr(h->csr, ti, m, n, s));  // validates the indices (synchronises once)
+    h->nblocks = grid_for(m, kBlock, 2048);
+    HYMD_TRY(h->slots.reserve(sizeof(double) * 12 * (size_t)std::max<int64_t>(m, 1)));
+    HYMD_TRY(h->partials.reserve(sizeof(double) * 2 * h->nblocks));
+    HYMD_TRY(h->result.reserve(sizeof(double) * 2));
+    HYMD_TRY(h->host.reserve(sizeof(double) * 2));
+    *out = h.release();
+    return HYMD_OK;
+}
+
+extern "C" int hymd_dihedrals_destroy(hymd_dihedrals *h) {
+    delete h;
+    return HYMD_OK;
+}
+
+extern "C" int hymd_dihedrals_forces(hymd_dihedrals *h, const double *x, double *f, double *dipoles, double *tmat,
+                                     const double box[3], int32_t dipole_flag, double *energy, void *stream) {
+    HYMD_REQUIRE(h && box && f && (h->n == 0 || x) && (h->m == 0 || (dipoles && tmat)), "bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    DihArgs D = h->D;
+    D.dipole_flag = dipole_flag;
+    Box bx = make_box(box);
+    double *part = energy ? (double *)h->partials.ptr : nullptr;
+    if (h->m > 0)
+        dihedral_term_kernel<double><<<h->nblocks, kBlock, 0, s>>>(x, h->m, D, bx, (double *)h->slots.ptr, dipoles,
+                                                                   tmat, part);
+    if (h->n > 0)
+        dihedral_slot_gather_kernel<double><<<grid_for(h->n), kBlock, 0, s>>>(f, h->n, h->csr.off(), h->csr.ent(),
+                                                                            (const double *)h->slots.ptr);
+    HYMD_HIP(hipGetLastError());
+    if (energy) {
+        *energy = 0.0;
+        if (h->m > 0) {
+            HYMD_TRY(reduce_partials((const double *)h->partials.ptr, h->nblocks, 2, (double *)h->result.ptr, s));
+            HYMD_HIP(hipMemcpyAsync(h->host.ptr, h->result.ptr, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+            HYMD_HIP(hipStreamSynchronize(s));
+            *energy = ((const double *)h->host.ptr)[0];
+        }
+    }
+    return HYMD_OK;
+}
+
+extern "C" int hymd_dihedrals_redistribute(hymd_dihedrals *h, const double *f_dipoles, const double *tmat, double *f,
+                                           void *stream) {
+    HYMD_REQUIRE(h && f && (h->m == 0 || (f_dipoles && tmat)), "bad argument");
+    hipStream_t s = (hipStream_t)stream;
+    if (h->n > 0)
+        redistribution_kernel<double><<<grid_for(h->n), kBlock, 0, s>>>(f, h->n, h->csr.off(), h->csr.ent(), f_dipoles,
+                                                                         tmat, h->D.dtype, h->D.last);
+    HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }

@@ -108,6 +108,34 @@ __device__ __forceinline__ double np_mod(double x, double L) {
     return r == 0.0 ? 0.0 : r;
 }
 
+// Rank of this lane's item among the items of the same bin, counted in
+// counts[key] with atomics; key < 0 marks an inactive lane (no count, rank 0).
+// Lanes of a wavefront that share a bin are aggregated: up to four rounds
+// pick the first pending lane's bin and issue one atomic for all lanes in
+// it; a round whose bin holds a single lane ends the aggregation (the input
+// is not bin-ordered there), and the lanes left over use one atomic each.
+// Spatially ordered input has few bins per wavefront, so its atomics (and
+// their contention on one counter) drop by up to 64x.  Every lane of the
+// wavefront must call it.
+__device__ __forceinline__ int rank_in_bin(int key, int *counts) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long pending = __ballot(key >= 0);
+    int rank = 0;
+    for (int round = 0; round < 4 && pending; ++round) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int k = __shfl(key, leader, 64);
+        const unsigned long long match = __ballot(key == k) & pending;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(counts + k, __popcll(match));
+        base = __shfl(base, leader, 64);
+        if ((match >> lane) & 1ull) rank = base + __popcll(match & ((1ull << lane) - 1ull));
+        pending &= ~match;
+        if (__popcll(match) == 1) break;
+    }
+    if ((pending >> lane) & 1ull) rank = atomicAdd(counts + key, 1);
+    return rank;
+}
+
 // mesh geometry passed by value to kernels
 struct Mesh {
     int mx, my, mz, zh;     // real edges, zh = mz/2+1
@@ -202,6 +230,11 @@ int field_key_target(hymd_pm *pm, int64_t n, RowKeyTarget *t);
 int update_field_keyed(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
                        const double *coupling, const double *shift, double constant, const double *mass,
                        double sigma, float *force_mesh, double *energy);
+// force arrays of the outer step: mode 0 out = a + (float)b, 1 out += a,
+// 2 out += (float)b, 3 out = a; and float64 <-> float32 copies
+int force_combine(float *out, const float *a, const double *b, int64_t n3, int mode, hipStream_t s);
+int convert_f64_f32(float *f32, double *f64, int64_t n, bool to_f32, hipStream_t s);
+
 // hymd_fused_inner that also produces the row keys of the final positions
 int fused_inner_impl(hymd_fused *f, double *x, double *v, float *x32, float *f_fast, const float *f_slow, double h_in,
                      double h_out, double dt, int32_t n_inner, int32_t pending, const double box[3], double *energies,

@@ -86,14 +86,20 @@ __device__ __forceinline__ int base_row(const float *__restrict__ x, int64_t i, 
     return cx.i0 * m.my + cy.i0;
 }
 
-// key = mesh row of the particle's base cell; slot = its arrival rank in
-// that row (a returning atomic), so the scatter needs no second atomic.
-__global__ void __launch_bounds__(kBlock) row_key_kernel(const float *__restrict__ x, int64_t n, Mesh m,
-                                                          int2 *__restrict__ keys, int *__restrict__ counts) {
+// key = mesh row of the particle's base cell; slot = its rank among that
+// row's particles (wave-aggregated atomics), so the scatter needs none.
+// Particles of zero weight (w != NULL) deposit nothing and are left out
+// (key -1).  The loop is uniform across the wavefront for the aggregation.
+__global__ void __launch_bounds__(kBlock) row_key_kernel(const float *__restrict__ x, const float *__restrict__ w,
+                                                          int64_t n, Mesh m, int2 *__restrict__ keys,
+                                                          int *__restrict__ counts) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const int key = base_row(x, i, m);
-        keys[i] = make_int2(key, atomicAdd(counts + key, 1));
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        const int64_t i = i0 + threadIdx.x;
+        int key = -1;
+        if (i < n && !(w && w[i] == 0.f)) key = base_row(x, i, m);
+        const int rank = rank_in_bin(key, counts);
+        if (i < n) keys[i] = make_int2(key, rank);
     }
 }
 
@@ -107,6 +113,7 @@ __global__ void __launch_bounds__(kBlock) key_scatter_kernel(const float *__rest
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const int2 kr = keys[i];
+        if (kr.x < 0) continue;  // left out of the binning
         const int pos = start[kr.x] + kr.y;
         perm[pos] = (int)i;
         xs[3 * (int64_t)pos] = x[3 * i];
@@ -193,9 +200,11 @@ __global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict
                                                           const int32_t *__restrict__ types,
                                                           const float *__restrict__ scale_per_particle,
                                                           int64_t n, int n_types, Mesh m,
-                                                          float *__restrict__ out, const int *__restrict__ perm) {
+                                                          float *__restrict__ out, const int *__restrict__ perm,
+                                                          const int *__restrict__ n_dev) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t first = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (n_dev) n = min(n, (int64_t)*n_dev);  // particles kept by the binning
     for (int64_t i = first; i < n; i += stride) {
         const int t = types ? types[i] : 0;
         const int64_t o = perm ? (int64_t)perm[i] : i;
@@ -697,7 +706,7 @@ static int sort_particles(hymd_pm *pm, const float *x, const int32_t *types, con
     HYMD_TRY(pm->ts.reserve(sizeof(int32_t) * nn));
     if (w) HYMD_TRY(pm->ws.reserve(sizeof(float) * nn));
     if (n > 0 && !keys_ready)
-        row_key_kernel<<<grid_for(n), kBlock, 0, s>>>(x, n, m, (int2 *)pm->keys.ptr, (int *)pm->counts.ptr);
+        row_key_kernel<<<grid_for(n), kBlock, 0, s>>>(x, w, n, m, (int2 *)pm->keys.ptr, (int *)pm->counts.ptr);
     HYMD_HIP(hipGetLastError());
     HYMD_TRY(exclusive_scan((int *)pm->counts.ptr, (int *)pm->start.ptr, nrows, (int *)pm->tiles.ptr, nullptr,
                             true, s));
@@ -732,16 +741,24 @@ static int paint_sorted(hymd_pm *pm, int T, const double *mass, float *out, long
 // the particles' original rows (scaled by their weights if the sort had them).
 static int readout_sorted(hymd_pm *pm, int ncomp, int n_types, const float *grids, float *out, bool weighted) {
     const int64_t n = pm->sorted_n;
-    if (n > 0)
-    {
+    if (n > 0) {
         const float *xs = (const float *)pm->xs.ptr;
         const int32_t *ts = (const int32_t *)pm->ts.ptr;
         const float *ws = weighted ? (const float *)pm->ws.ptr : nullptr;
         const int *pp = (const int *)pm->perm.ptr;
+        // a weighted binning leaves zero-weight particles out: their rows get
+        // zero, the kept ones are counted in start[rows]
+        const int *kept = nullptr;
+        if (pm->sorted_weights) {
+            HYMD_TRY(zero_words(out, ncomp * n, pm->stream));
+            kept = (const int *)pm->start.ptr + (int64_t)pm->mesh.mx * pm->mesh.my;
+        }
         if (ncomp == 3)
-            readout_kernel<3><<<grid_for(n), kBlock, 0, pm->stream>>>(grids, xs, ts, ws, n, n_types, pm->mesh, out, pp);
+            readout_kernel<3><<<grid_for(n), kBlock, 0, pm->stream>>>(grids, xs, ts, ws, n, n_types, pm->mesh, out, pp,
+                                                                      kept);
         else
-            readout_kernel<1><<<grid_for(n), kBlock, 0, pm->stream>>>(grids, xs, ts, ws, n, n_types, pm->mesh, out, pp);
+            readout_kernel<1><<<grid_for(n), kBlock, 0, pm->stream>>>(grids, xs, ts, ws, n, n_types, pm->mesh, out, pp,
+                                                                      kept);
     }
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
@@ -809,7 +826,7 @@ extern "C" int hymd_readout(hymd_pm *pm, const float *grid, const float *x, int6
     HYMD_REQUIRE(pm && grid && out && (x || n == 0) && n >= 0, "bad argument");
     if (n > 0)
         readout_kernel<1><<<grid_for(n), kBlock, 0, pm->stream>>>(grid, x, nullptr, nullptr, n, 1, pm->mesh, out,
-                                                                    nullptr);
+                                                                    nullptr, nullptr);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }

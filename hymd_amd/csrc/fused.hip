@@ -206,11 +206,19 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT
             A.x32[g + d] = (float)xi;
             A.f_fast[g + d] = ff[k][d];
         }
-        if (A.key.keys) {  // the field path's row key of the float32 position
-            const CicAxis cx = cic_axis((double)(float)xs[3 * j], A.key.inv_h[0], A.key.mx);
-            const CicAxis cy = cic_axis((double)(float)xs[3 * j + 1], A.key.inv_h[1], A.key.my);
-            const int row = cx.i0 * A.key.my + cy.i0;
-            A.key.keys[p0 + j] = make_int2(row, atomicAdd(A.key.counts + row, 1));
+    }
+    if (A.key.keys) {  // the field path's row keys of the float32 positions
+#pragma unroll
+        for (int k = 0; k < PPT; ++k) {
+            const int j = threadIdx.x + k * kBlock;
+            int row = -1;
+            if (j < np) {
+                const CicAxis cx = cic_axis((double)(float)xs[3 * j], A.key.inv_h[0], A.key.mx);
+                const CicAxis cy = cic_axis((double)(float)xs[3 * j + 1], A.key.inv_h[1], A.key.my);
+                row = cx.i0 * A.key.my + cy.i0;
+            }
+            const int rank = rank_in_bin(row, A.key.counts);
+            if (j < np) A.key.keys[p0 + j] = make_int2(row, rank);
         }
     }
     if (A.partials) {

@@ -49,6 +49,45 @@ __global__ void __launch_bounds__(kBlock) ke_kernel(const T *__restrict__ v, int
     block_reduce_store<1>(acc, partials + blockIdx.x);
 }
 
+// Force-array helpers of the outer step with PME and dihedral terms (each
+// element rounded as the equivalent torch expression rounds it):
+//   mode 0: out = a + (float)b      mode 1: out += a
+//   mode 2: out += (float)b         mode 3: out = a
+__global__ void __launch_bounds__(kBlock) force_combine_kernel(float *__restrict__ out, const float *__restrict__ a,
+                                                                const double *__restrict__ b, int64_t n3, int mode) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += stride) {
+        switch (mode) {
+            case 0: out[i] = a[i] + (float)b[i]; break;
+            case 1: out[i] = out[i] + a[i]; break;
+            case 2: out[i] = out[i] + (float)b[i]; break;
+            default: out[i] = a[i]; break;
+        }
+    }
+}
+
+// float64 <-> float32 copies (round to nearest)
+__global__ void __launch_bounds__(kBlock) convert_kernel(float *__restrict__ f32, double *__restrict__ f64, int64_t n,
+                                                          int to_f32) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (to_f32) f32[i] = (float)f64[i];
+        else f64[i] = (double)f32[i];
+    }
+}
+
+int force_combine(float *out, const float *a, const double *b, int64_t n3, int mode, hipStream_t s) {
+    if (n3 > 0) force_combine_kernel<<<grid_for(n3), kBlock, 0, s>>>(out, a, b, n3, mode);
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
+}
+
+int convert_f64_f32(float *f32, double *f64, int64_t n, bool to_f32, hipStream_t s) {
+    if (n > 0) convert_kernel<<<grid_for(n), kBlock, 0, s>>>(f32, f64, n, to_f32 ? 1 : 0);
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
+}
+
 }  // namespace hymd
 
 using namespace hymd;

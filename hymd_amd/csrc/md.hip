@@ -31,6 +31,17 @@ struct hymd_md {
     int respa_inner = 1;
     hymd_fused *fused = nullptr;  // fused inner loop (null: per-kernel path)
     int pending = 0;              // fused: outer half kick of the last step not yet applied
+    // peptide terms (optional): PME of the charges, dihedrals with dipoles
+    // whose charges go through PME and back onto the backbone
+    const float *charges = nullptr;
+    double elec = 0.0, e_self = 0.0;
+    hymd_dihedrals *dih = nullptr;
+    int64_t m4 = 0;
+    bool protein = false;
+    const float *q_dip = nullptr;
+    float *f_slow = nullptr;  // caller's: summed slow forces
+    DevBuf f_elec, f_ba, f_dih, dipoles, tmat, f_dip, dip32, f_dip64, f_recon;
+    double extra_e[2] = {0.0, 0.0};  // dihedral, field_q of the last energies step
     // optional HIP-event timing of one stage in eager steps
     int timed = 0;  // 0 none, 1 bonded forces, 2 kick+drift, 3 update_field, 4 readout
     std::vector<hipEvent_t> ev;
@@ -42,6 +53,32 @@ static void mark(hymd_md *m, int stage) {
 }
 
 extern "C" int hymd_md_set_fused(hymd_md *m, int32_t enable);
+
+// Slow forces beyond the field ones (main.py:1044-1095): f_slow = f_field
+// + PME forces of the charges + the dipole-charge PME forces redistributed
+// onto the backbone, in the order (and rounding) of the Python step.
+static int peptide_slow_forces(hymd_md *m, bool energy) {
+    hipStream_t s = m->work;
+    const int64_t n3 = 3 * m->n;
+    HYMD_TRY(force_combine(m->f_slow, m->f_field, nullptr, n3, 3, s));
+    if (m->charges) {
+        HYMD_TRY(hymd_update_field_force_q(m->pm, m->x32, m->charges, m->n, m->sigma, m->elec, m->e_self,
+                                           (float *)m->f_elec.ptr, nullptr, nullptr, nullptr,
+                                           energy ? &m->extra_e[1] : nullptr));
+        HYMD_TRY(force_combine(m->f_slow, (const float *)m->f_elec.ptr, nullptr, n3, 1, s));
+    }
+    if (m->protein) {
+        const int64_t nd = 4 * m->m4;
+        HYMD_TRY(convert_f64_f32((float *)m->dip32.ptr, (double *)m->dipoles.ptr, 3 * nd, true, s));
+        HYMD_TRY(hymd_update_field_force_q(m->pm, (const float *)m->dip32.ptr, m->q_dip, nd, m->sigma, m->elec, 0.0,
+                                           (float *)m->f_dip.ptr, nullptr, nullptr, nullptr, nullptr));
+        HYMD_TRY(convert_f64_f32((float *)m->f_dip.ptr, (double *)m->f_dip64.ptr, 3 * nd, false, s));
+        HYMD_TRY(hymd_dihedrals_redistribute(m->dih, (const double *)m->f_dip64.ptr, (const double *)m->tmat.ptr,
+                                             (double *)m->f_recon.ptr, s));
+        HYMD_TRY(force_combine(m->f_slow, nullptr, (const double *)m->f_recon.ptr, n3, 2, s));
+    }
+    return HYMD_OK;
+}
 
 // energies (host, 5): field, bond, angle, kinetic, (unused = 0)
 static int enqueue_step(hymd_md *m, double *energies, int pending) {
@@ -72,16 +109,27 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
         mark(m, 4);
         return HYMD_OK;
     }
+    const bool extra = m->charges || m->dih;
+    float *f_slow = extra ? m->f_slow : m->f_field;
+    const int64_t n3 = 3 * m->n;
     for (int i = 0; i < m->respa_inner; ++i) {
-        const bool first = (i == 0);
+        const bool first = (i == 0), last_inner = (i == m->respa_inner - 1);
         mark(m, 2);
-        HYMD_TRY(hymd_respa_kick_drift(m->x, m->v, m->x32, m->n, m->f_fast, half_in, first ? m->f_field : m->f_fast,
+        HYMD_TRY(hymd_respa_kick_drift(m->x, m->v, m->x32, m->n, m->f_fast, half_in, first ? f_slow : m->f_fast,
                                        first ? half_out : half_in, m->dt, m->box, m->work));
         mark(m, 2);
-        const bool last_e = energies && (i == m->respa_inner - 1);
+        const bool last_e = energies && last_inner;
         double eb[2];
         mark(m, 1);
-        HYMD_TRY(hymd_topology_forces(m->topo, m->x, m->f_fast, m->box, last_e ? eb : nullptr, nullptr, m->work));
+        HYMD_TRY(hymd_topology_forces(m->topo, m->x, m->dih ? (float *)m->f_ba.ptr : m->f_fast, m->box,
+                                      last_e ? eb : nullptr, nullptr, m->work));
+        if (m->dih) {  // dihedrals (dipoles on the last inner step), then f_fast = f_ba + f_dih
+            HYMD_TRY(hymd_dihedrals_forces(m->dih, m->x, (double *)m->f_dih.ptr, (double *)m->dipoles.ptr,
+                                           (double *)m->tmat.ptr, m->box, (last_inner && m->protein) ? 1 : 0,
+                                           last_e ? &m->extra_e[0] : nullptr, m->work));
+            HYMD_TRY(force_combine(m->f_fast, (const float *)m->f_ba.ptr, (const double *)m->f_dih.ptr, n3, 0,
+                                   m->work));
+        }
         mark(m, 1);
         if (last_e) {
             energies[1] = eb[0];
@@ -95,7 +143,8 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
     mark(m, 4);
     HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));
     mark(m, 4);
-    HYMD_TRY(hymd_respa_kick(m->v, m->n, m->f_fast, half_in, m->f_field, half_out, m->work));
+    if (extra) HYMD_TRY(peptide_slow_forces(m, energies != nullptr));
+    HYMD_TRY(hymd_respa_kick(m->v, m->n, m->f_fast, half_in, f_slow, half_out, m->work));
     return HYMD_OK;
 }
 
@@ -236,6 +285,10 @@ extern "C" int hymd_md_step(hymd_md *m, int32_t n_steps, int32_t use_graph, doub
 
 extern "C" int hymd_md_set_fused(hymd_md *m, int32_t enable) {
     HYMD_REQUIRE(m, "null handle");
+    if (enable && (m->charges || m->dih)) {
+        set_error("the fused inner loop has no dihedral or PME terms");
+        return HYMD_ERR_UNSUPPORTED;
+    }
     HYMD_HIP(hipStreamSynchronize(m->work));
     if (m->exec) {
         (void)hipGraphExecDestroy(m->exec);
@@ -288,5 +341,59 @@ extern "C" int hymd_md_stage_time(hymd_md *m, double *total_ms, int32_t *launche
     }
     *total_ms = tot;
     *launches = (int32_t)pairs;
+    return HYMD_OK;
+}
+
+extern "C" int hymd_md_set_peptide(hymd_md *m, const float *charges, double elec_factor, double self_energy,
+                                   hymd_dihedrals *dih, int64_t n_dihedrals, const float *dipole_charges,
+                                   float *f_slow) {
+    HYMD_REQUIRE(m, "null handle");
+    HYMD_REQUIRE(f_slow || (!charges && !dih), "f_slow is required with charges or dihedrals");
+    HYMD_REQUIRE(!dih || n_dihedrals >= 0, "bad dihedral count");
+    HYMD_HIP(hipStreamSynchronize(m->work));
+    if (charges || dih) HYMD_TRY(hymd_md_set_fused(m, 0));  // the fused inner loop has no dihedral terms
+    if (m->exec) {
+        (void)hipGraphExecDestroy(m->exec);
+        m->exec = nullptr;
+    }
+    if (m->graph) {
+        (void)hipGraphDestroy(m->graph);
+        m->graph = nullptr;
+    }
+    m->charges = charges;
+    m->elec = elec_factor;
+    m->e_self = self_energy;
+    m->dih = dih;
+    m->m4 = dih ? n_dihedrals : 0;
+    m->q_dip = dipole_charges;
+    m->protein = dih && dipole_charges;
+    m->f_slow = f_slow;
+    const size_t n3 = 3 * (size_t)std::max<int64_t>(m->n, 1), nd = 4 * (size_t)std::max<int64_t>(m->m4, 1);
+    if (charges) HYMD_TRY(m->f_elec.reserve(sizeof(float) * n3));
+    if (dih) {
+        HYMD_TRY(m->f_ba.reserve(sizeof(float) * n3));
+        HYMD_TRY(m->f_dih.reserve(sizeof(double) * n3));
+        HYMD_TRY(m->dipoles.reserve(sizeof(double) * 3 * nd));
+        HYMD_TRY(m->tmat.reserve(sizeof(double) * 54 * (nd / 4)));
+    }
+    if (m->protein) {
+        HYMD_TRY(m->f_dip.reserve(sizeof(float) * 3 * nd));
+        HYMD_TRY(m->dip32.reserve(sizeof(float) * 3 * nd));
+        HYMD_TRY(m->f_dip64.reserve(sizeof(double) * 3 * nd));
+        HYMD_TRY(m->f_recon.reserve(sizeof(double) * n3));
+    }
+    return HYMD_OK;
+}
+
+extern "C" int hymd_md_peptide_state(hymd_md *m, double *dipoles, double *tmat, double *extra_energies) {
+    HYMD_REQUIRE(m, "null handle");
+    HYMD_HIP(hipStreamSynchronize(m->work));
+    if (dipoles && m->dih)
+        HYMD_HIP(hipMemcpy(dipoles, m->dipoles.ptr, sizeof(double) * 12 * m->m4, hipMemcpyDeviceToDevice));
+    if (tmat && m->dih) HYMD_HIP(hipMemcpy(tmat, m->tmat.ptr, sizeof(double) * 54 * m->m4, hipMemcpyDeviceToDevice));
+    if (extra_energies) {
+        extra_energies[0] = m->extra_e[0];
+        extra_energies[1] = m->extra_e[1];
+    }
     return HYMD_OK;
 }

@@ -232,3 +232,48 @@ class BondedTopology:
             self.close()
         except Exception:
             pass
+
+
+class DihedralTopology:
+    """A fixed dihedral term list on the device (compute_dihedral_forces__double
+    + dipole reconstruction, force.py:855-877 redistribution) with per-particle
+    lists built once; ``forces`` and ``redistribute`` run without per-call
+    list builds or synchronisation (unless an energy is asked for)."""
+
+    def __init__(self, n, a, b, c, d, coeff, dtype, last):
+        _lib.require_gpu()
+        self.n = int(n)
+        self._t = [_to_dev(v, torch.int32) for v in (a, b, c, d)]
+        self.m = self._t[0].numel()
+        self.coeff = _to_dev(coeff, torch.float32).reshape(self.m, 6, 5).contiguous()
+        self.dtype = _to_dev(dtype, torch.int32)
+        self.last = _to_dev(last, torch.int32)
+        h = ctypes.c_void_p()
+        call("hymd_dihedrals_create", ctypes.byref(h), self.n, self.m, *[dptr(t) for t in self._t], dptr(self.coeff),
+             dptr(self.dtype), dptr(self.last), _lib.stream_ptr())
+        self._h = h
+
+    def forces(self, x64, f64, dipoles, tmat, box, dipole_flag, energy=False):
+        """f64 <- dihedral forces at x64; dipoles / tmat written (zero where a
+        term has none).  Returns the energy when ``energy``."""
+        e = ctypes.c_double(0.0)
+        call("hymd_dihedrals_forces", self._h, dptr(x64, torch.float64, "positions"), dptr(f64, torch.float64, "forces"),
+             dptr(dipoles, torch.float64, "dipoles"), dptr(tmat, torch.float64, "transfer_matrix"), _box(box),
+             int(dipole_flag), ctypes.byref(e) if energy else None, _lib.stream_ptr())
+        return e.value if energy else None
+
+    def redistribute(self, f_dipoles, tmat, f64):
+        """f64 <- dipole forces (M,4,3) projected through tmat onto the beads."""
+        call("hymd_dihedrals_redistribute", self._h, dptr(f_dipoles, torch.float64, "f_dipoles"),
+             dptr(tmat, torch.float64, "transfer_matrix"), dptr(f64, torch.float64, "forces"), _lib.stream_ptr())
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _lib.lib().hymd_dihedrals_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

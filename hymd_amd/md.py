@@ -13,6 +13,14 @@ Per outer step, in the reference's order (main.py:802-887, 975-1004,
 Adjacent kicks share one kernel with the drift (each half kick is still its
 own FP64 addition, in the reference's order).  Positions and velocities are
 float64; forces float32; a float32 copy of x feeds paint and readout.
+
+Systems with charges and/or dihedrals (the peptide/solvent workload) add, as
+main.py:508-588, 802-887, 1044-1095 do: the dihedral forces (FP64 kernel,
+dipoles reconstructed on the last inner step) to the fast forces, and the
+PME forces of the charges plus the dipole-charge PME forces redistributed
+onto the backbone (force.py:855-877) to the slow forces, both in ``step()``
+and in the library's captured outer step (``run()``), which then uses the
+per-kernel inner loop.
 """
 import ctypes
 
@@ -20,10 +28,12 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import call, darr, dptr
+from ._lib import HymdError, call, darr, dptr
 from .field import ParticleMesh
-from .force import BondedTopology
+from .force import BondedTopology, DihedralTopology
 from .synth import hamiltonian_form
+
+COULOMB_CONSTANT = 138.935458  # input_parser.py:160
 
 
 class RespaMD:
@@ -53,12 +63,52 @@ class RespaMD:
         self.f_fast = torch.zeros((self.n, 3), dtype=torch.float32, device=dev)
         self._box_c = darr(self.box, 3)
         self.energies = {}
-        # initial forces (main.py:331-371, 508-551)
+        self._setup_peptide_terms(s, dev)
+        # initial forces (main.py:331-371, 508-588): the bonded ones first, the
+        # dihedrals with their dipoles, which the slow dipole PME then uses
+        self.fast_forces(dipole_flag=1)
         self.slow_forces()
-        self.fast_forces()
         self._native = None
 
     # ------------------------------------------------------------------
+    def _setup_peptide_terms(self, s, dev):
+        """Charges (PME), dihedrals and their dipoles, when the system has them."""
+        q = None if s.charges is None else np.asarray(s.charges, dtype=np.float32)
+        self.charges = torch.as_tensor(q, device=dev) if q is not None and np.any(q) else None
+        dih = s.extra.get("dihedrals") if getattr(s, "extra", None) else None
+        self.dihedrals = None
+        self.protein = False
+        if self.charges is not None:
+            self.elec = COULOMB_CONSTANT / float(s.dielectric)
+            self.f_elec = torch.zeros((self.n, 3), dtype=torch.float32, device=dev)
+            # compute_self_energy_q (field.py:231-238), once
+            self.e_self = self.elec * np.sqrt(1.0 / (2.0 * np.pi * self.sigma ** 2)) * float(
+                np.sum(np.asarray(q, dtype=np.float64) ** 2))
+        if dih is not None and len(dih[0]):
+            a, b, c, d, coeff, dtype, last = dih
+            self.dihedrals = DihedralTopology(self.n, a, b, c, d, coeff, dtype, last)
+            m4 = self.dihedrals.m
+            self.f_ba = torch.zeros((self.n, 3), dtype=torch.float32, device=dev)
+            self.f_dih = torch.zeros((self.n, 3), dtype=torch.float64, device=dev)
+            self.dipoles = torch.zeros((m4, 4, 3), dtype=torch.float64, device=dev)
+            self.tmat = torch.zeros((m4, 6, 3, 3), dtype=torch.float64, device=dev)
+            self.protein = bool(np.any(np.asarray(dtype) == 1))
+            if self.protein:
+                qd = np.zeros((m4, 4), dtype=np.float32)
+                cbt = np.asarray(dtype) == 1
+                both = cbt & (np.asarray(last) == 1)
+                qd[cbt, 0], qd[cbt, 1] = 0.25, -0.25      # main.py:475-483
+                qd[both, 2], qd[both, 3] = 0.25, -0.25
+                self.q_dip = torch.as_tensor(qd.ravel(), device=dev)
+                self.f_dip = torch.zeros((4 * m4, 3), dtype=torch.float32, device=dev)
+                self.f_dip64 = torch.zeros((m4, 4, 3), dtype=torch.float64, device=dev)
+                self.dip32 = torch.zeros((4 * m4, 3), dtype=torch.float32, device=dev)
+                self.f_recon = torch.zeros((self.n, 3), dtype=torch.float64, device=dev)
+        self.extra_terms = self.charges is not None or self.dihedrals is not None
+        # slow forces kicked together: the field forces alone, or their sum with
+        # the PME and redistributed dipole forces
+        self.f_slow = self.f_field if not self.extra_terms else torch.zeros_like(self.f_field)
+
     def _native_handle(self):
         if self._native is None:
             h = ctypes.c_void_p()
@@ -67,6 +117,12 @@ class RespaMD:
                  dptr(self.force_mesh), darr(np.ravel(self.A)), darr(self.b), float(self.c0),
                  darr(self.type_mass), self.sigma, self.mass, self.dt, self.n_inner, self._box_c)
             self._native = h
+            if self.extra_terms:
+                call("hymd_md_set_peptide", h, dptr(self.charges), self.elec if self.charges is not None else 0.0,
+                     self.e_self if self.charges is not None else 0.0,
+                     self.dihedrals._h if self.dihedrals is not None else None,
+                     self.dihedrals.m if self.dihedrals is not None else 0,
+                     dptr(self.q_dip) if self.protein else None, dptr(self.f_slow))
         return self._native
 
     @property
@@ -84,9 +140,17 @@ class RespaMD:
         with ``use_graph`` one captured step is replayed).  With ``energies``
         the last step reports field, bond, angle and kinetic energies."""
         e = (ctypes.c_double * 5)() if energies else None
-        call("hymd_md_step", self._native_handle(), int(n_steps), 1 if use_graph else 0, e, _lib.stream_ptr())
+        h = self._native_handle()
+        call("hymd_md_step", h, int(n_steps), 1 if use_graph else 0, e, _lib.stream_ptr())
+        if self.dihedrals is not None:   # the library's dipoles / transfer matrices are the current ones
+            x2 = (ctypes.c_double * 2)()
+            call("hymd_md_peptide_state", h, dptr(self.dipoles), dptr(self.tmat), x2)
         if energies:
             self.energies.update(field=e[0], bond=e[1], angle=e[2], kinetic=e[3])
+            if self.dihedrals is not None:
+                self.energies["dihedral"] = x2[0]
+            if self.charges is not None:
+                self.energies["field_q"] = x2[1]
             return dict(self.energies)
         return None
 
@@ -111,12 +175,34 @@ class RespaMD:
         self.pm.readout_forces(self.x32, self.types, self.T, self.force_mesh, self.f_field)
         if energy:
             self.energies["field"] = e
+        if not self.extra_terms:
+            return
+        self.f_slow.copy_(self.f_field)
+        if self.charges is not None:     # field.py:241-403 on the particle charges
+            eq = self.pm.pme(self.x32, self.charges, self.sigma, self.elec, self.e_self, self.f_elec,
+                             want_energy=energy)
+            self.f_slow += self.f_elec
+            if energy:
+                self.energies["field_q"] = eq
+        if self.protein:                 # main.py:1060-1095: PME of the dipole charges, redistributed
+            self.dip32.copy_(self.dipoles.reshape(-1, 3))
+            self.pm.pme(self.dip32, self.q_dip, self.sigma, self.elec, 0.0, self.f_dip)
+            self.f_dip64.copy_(self.f_dip.reshape(-1, 4, 3))
+            self.dihedrals.redistribute(self.f_dip64, self.tmat, self.f_recon)
+            self.f_slow += self.f_recon.to(torch.float32)
 
-    def fast_forces(self, energy=False):
-        r = self.topo.forces(self.x, self.f_fast, self.box, energies=energy)
+    def fast_forces(self, energy=False, dipole_flag=0):
+        target = self.f_fast if self.dihedrals is None else self.f_ba
+        r = self.topo.forces(self.x, target, self.box, energies=energy)
         if energy:
             (eb, ea), (pb, pa) = r
             self.energies.update(bond=eb, angle=ea, bond_pr=pb, angle_pr=pa)
+        if self.dihedrals is not None:   # compute_dihedral_forces__double.f90 (+ dipoles)
+            ed = self.dihedrals.forces(self.x, self.f_dih, self.dipoles, self.tmat, self.box,
+                                       1 if (dipole_flag and self.protein) else 0, energy=energy)
+            torch.add(self.f_ba, self.f_dih.to(torch.float32), out=self.f_fast)
+            if energy:
+                self.energies["dihedral"] = ed
 
     def kick_drift(self, f_fast, c_fast, f_slow=None, c_slow=0.0):
         call("hymd_respa_kick_drift", dptr(self.x), dptr(self.v), dptr(self.x32), self.n, dptr(f_fast),
@@ -139,16 +225,17 @@ class RespaMD:
         half_out = 0.5 * self.n_inner * self.dt / self.mass
         for i in range(self.n_inner):
             if i == 0:
-                self.kick_drift(self.f_fast, half_in, self.f_field, half_out)
+                self.kick_drift(self.f_fast, half_in, self.f_slow, half_out)
             else:
                 self.kick_drift(self.f_fast, half_in, self.f_fast, half_in)
-            last = energies and i == self.n_inner - 1
+            last_inner = i == self.n_inner - 1
+            last = energies and last_inner
             if timer is None:
-                self.fast_forces(energy=last)
+                self.fast_forces(energy=last, dipole_flag=1 if last_inner else 0)
             else:
-                timer("bonded", lambda: self.fast_forces(energy=last))
+                timer("bonded", lambda: self.fast_forces(energy=last, dipole_flag=1 if last_inner else 0))
         self.slow_forces(energy=energies)
-        self.kick(self.f_fast, half_in, self.f_field, half_out)
+        self.kick(self.f_fast, half_in, self.f_slow, half_out)
         if energies:
             self.energies["kinetic"] = self.kinetic_energy()
         return self.energies if energies else None
@@ -158,4 +245,6 @@ class RespaMD:
             _lib.lib().hymd_md_destroy(self._native)
             self._native = None
         self.topo.close()
+        if self.dihedrals is not None:
+            self.dihedrals.close()
         self.pm.close()

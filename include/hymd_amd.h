@@ -112,6 +112,25 @@ int hymd_field_moments(hymd_pm *pm, const float *x, const int32_t *types, int64_
                        const double *mass, double sigma, double *moments, const double *coupling,
                        const double *shift, float *force_mesh);
 
+/* A fixed dihedral term list (compute_dihedral_forces__double.f90 +
+ * dipole_reconstruction__double.f90, force.py:855-877) held on the device with
+ * its per-particle lists built once: the loop-side form of hymd_cdf (double)
+ * and hymd_dipole_redistribution, without their per-call list build and
+ * synchronisation.  Term arrays are referenced, not copied.  forces: f (N,3)
+ * float64 overwritten; dipoles (M,4,3) and tmat (M,6,3,3) float64 written as
+ * cdf_d does (zero where a term has none); energy (host) optional, its
+ * reduction synchronises.  redistribute: f (N,3) float64 overwritten with the
+ * dipole forces (M,4,3) projected through tmat onto the beads. */
+typedef struct hymd_dihedrals hymd_dihedrals;
+int hymd_dihedrals_create(hymd_dihedrals **out, int64_t n, int64_t m, const int32_t *a, const int32_t *b,
+                          const int32_t *c, const int32_t *d, const float *coeff, const int32_t *dtype,
+                          const int32_t *last, void *stream);
+int hymd_dihedrals_destroy(hymd_dihedrals *h);
+int hymd_dihedrals_forces(hymd_dihedrals *h, const double *x, double *f, double *dipoles, double *tmat,
+                          const double box[3], int32_t dipole_flag, double *energy, void *stream);
+int hymd_dihedrals_redistribute(hymd_dihedrals *h, const double *f_dipoles, const double *tmat, double *f,
+                                void *stream);
+
 /* Batched replicas (no reference counterpart: the reference runs one system
  * per process).  n_replicas independent systems, replica r owning particles
  * [offsets[r], offsets[r+1]) of the caller's arrays, mesh[3r..3r+2],
@@ -236,6 +255,20 @@ int hymd_md_create(hymd_md **md, hymd_pm *pm, hymd_topology *topo, int64_t n, in
  * waits for them; use_graph replays one captured step as a hipGraph;
  * energies host double[5] (field, bond, angle, kinetic, 0) after the last step, or NULL */
 int hymd_md_step(hymd_md *md, int32_t n_steps, int32_t use_graph, double *energies, void *stream);
+/* Adds the peptide terms of main.py:508-588, 802-887, 1044-1095 to the outer
+ * step (either may be NULL): PME of `charges` (device float32 (N,), with
+ * elec_factor = k_e/eps_r and the self energy) and the dihedral term list
+ * `dih` (n_dihedrals terms), whose dipoles, when dipole_charges (device
+ * float32 (4 n_dihedrals,)) is given, are rebuilt on the last inner step and
+ * go through PME with those charges, their forces redistributed onto the
+ * backbone.  f_slow (device float32 (N,3)) receives the summed slow forces
+ * the outer kicks use.  Turns the fused inner loop off (it has no dihedral
+ * terms).  peptide_state copies the dipoles (M,4,3) and transfer matrices
+ * (M,6,3,3) (device float64, optional) and the (dihedral, PME) energies of the
+ * last energies step (host double[2], optional). */
+int hymd_md_set_peptide(hymd_md *md, const float *charges, double elec_factor, double self_energy,
+                        hymd_dihedrals *dih, int64_t n_dihedrals, const float *dipole_charges, float *f_slow);
+int hymd_md_peptide_state(hymd_md *md, double *dipoles, double *tmat, double *extra_energies);
 int hymd_md_destroy(hymd_md *md);
 /* use the fused inner loop (1) or the per-kernel inner loop (0); returns
  * HYMD_ERR_UNSUPPORTED if the topology does not allow fusing */

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 import torch
 
-from tests.conftest import rel_err
+from tests.conftest import rel_err, rms_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -114,6 +114,71 @@ def test_respa_md_matches_oracle_loop(hip, oracle_built):
     assert e["field"] == pytest.approx(ref.field_energy(), rel=1e-5)
     assert e["bond"] == pytest.approx(ref.e_fast["bond"], rel=1e-6)
     assert e["angle"] == pytest.approx(ref.e_fast["angle"], rel=1e-6)
+
+
+def _small_peptide():
+    from hymd_amd import synth
+    s = synth.peptide_solvent(n_chains=24, n_res=12, n_solvent=2500, box=7.5, mesh=24, seed=31)
+    s.respa_inner = 2          # the dipoles are rebuilt on the last inner step only
+    return s
+
+
+def test_peptide_md_matches_oracle_loop(hip, oracle_built):
+    """Peptide/solvent loop (field + PME of the charges + CBT dihedrals with
+    dipoles whose charges go through PME and back onto the backbone), two
+    outer steps of two inner steps, device against the FP64 oracle loop."""
+    from hymd_amd import synth
+    from hymd_amd.md import RespaMD
+    from oracle import field as F
+    from oracle import md as OM
+    s = _small_peptide()
+    A, b, c0 = synth.hamiltonian_form(s)
+    ref = OM.State(s, F.Quadratic(A, b, c0))
+    md = RespaMD(s)
+    assert md.protein and md.charges is not None
+    f_slow = md.f_slow.cpu().numpy()
+    assert rms_rel(f_slow, ref.f_field) < 1e-5
+    assert rel_err(f_slow, ref.f_field) < 1e-4
+    assert rel_err(md.f_fast.cpu().numpy(), ref.f_fast) < 1e-5
+    # the dipole part alone (redistributed dipole PME forces)
+    assert rel_err(md.f_recon.cpu().numpy(), ref.f_slow[2]) < 1e-4
+    for _ in range(2):
+        ref.outer_step()
+        md.step()
+    x = md.x.cpu().numpy()
+    dx = x - ref.x
+    dx -= s.box * np.round(dx / s.box)
+    assert np.abs(dx).max() < 1e-5
+    assert rel_err(md.v.cpu().numpy(), ref.v) < 1e-4
+    np.testing.assert_allclose(md.dipoles.cpu().numpy(), ref.dipoles, atol=1e-6)
+    e = md.step(energies=True)
+    ref.outer_step()
+    assert e["dihedral"] == pytest.approx(ref.e_fast["dihedral"], rel=1e-6)
+    assert e["kinetic"] == pytest.approx(ref.kinetic_energy(), rel=1e-6)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_peptide_native_step_equals_python_step(hip, use_graph):
+    """The library's outer step with PME, dihedrals and dipoles (eager and
+    hipGraph replay) runs the kernels of the Python step in its order:
+    identical results; the fused inner loop is refused for this system."""
+    from hymd_amd._lib import HymdError
+    from hymd_amd.md import RespaMD
+    a = RespaMD(_small_peptide())
+    b = RespaMD(_small_peptide())
+    for _ in range(3):
+        a.step()
+    b.run(3, use_graph=use_graph)
+    torch.cuda.synchronize()
+    assert torch.equal(a.x, b.x) and torch.equal(a.v, b.v)
+    assert torch.equal(a.f_slow, b.f_slow) and torch.equal(a.f_fast, b.f_fast)
+    assert torch.equal(a.dipoles, b.dipoles)
+    ea = a.step(energies=True)
+    eb = b.run(1, energies=True)
+    for k in ("field", "bond", "angle", "kinetic", "dihedral", "field_q"):
+        assert ea[k] == pytest.approx(eb[k], rel=1e-12), k
+    with pytest.raises(HymdError):
+        b.set_fused(True)
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
