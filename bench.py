@@ -79,7 +79,8 @@ def cpu_baseline(system, steps):
     t0 = time.perf_counter()
     st = OM.State(system, OF.Quadratic(*A))
     t_init = time.perf_counter() - t0
-    initial = (st.f_field.copy(), st.f_fast.copy())
+    initial = (st.f_field.copy(), st.f_fast.copy(),
+               {"field": st.field_energy(), "bond": st.e_fast["bond"], "angle": st.e_fast["angle"]})
     t0 = time.perf_counter()
     for _ in range(steps):
         st.outer_step()
@@ -180,6 +181,7 @@ def peptide_solvent_stages(reps, check):
     f_red = torch.zeros((n, 3), dtype=torch.float64, device="cuda")
     e_self = float(ke / s.dielectric * np.sqrt(1.0 / (2.0 * np.pi * sigma * sigma)) * np.sum(s.charges.astype(np.float64) ** 2))
     box = s.box
+    energies = {}
     stages = {
         "field_update": lambda: pm.field_update(x32, types, T, A, b, c0, [1.0] * T, sigma, fm),
         "field_readout": lambda: pm.readout_forces(x32, types, T, fm, f_field),
@@ -189,6 +191,10 @@ def peptide_solvent_stages(reps, check):
             f_dih, x64, dip, tmat, box, *dih, coeff_d, dty_d, last_d, 1),
         "dipole_redistribution": lambda: dipole_forces_redistribution(f_red, f_dip, tmat, *dih, dty_d, last_d),
     }
+    energies["field"] = pm.field_update(x32, types, T, A, b, c0, [1.0] * T, sigma, fm, want_energy=True)
+    energies["field_q"] = pm.pme(x32, q, sigma, elec, e_self, f_q, want_energy=True)
+    energies["dihedral"] = compute_dihedral_forces__fortran__double(f_dih, x64, dip, tmat, box, *dih, coeff_d, dty_d,
+                                                                     last_d, 1)
     ms = {}
     for name, fn in stages.items():
         fn()
@@ -233,15 +239,21 @@ def peptide_solvent_stages(reps, check):
         rq = OF.update_field_force_q(xo, s.charges, s.mesh, s.box, sigma, ke, s.dielectric)
         cpu["pme"] = 1e3 * (time.perf_counter() - t0)
         t0 = time.perf_counter()
-        fdo, _, _, _ = OB.cdf(s.positions, s.box, da, db, dc, dd, coeff, dty, last, 1)
+        fdo, _, _, edo = OB.cdf(s.positions, s.box, da, db, dc, dd, coeff, dty, last, 1)
         cpu["dihedrals_cbt_dipoles"] = 1e3 * (time.perf_counter() - t0)
+        e_field_o = OF.field_energy(ref["phi"], OF.Quadratic(A, b, c0), s.mesh, s.box)
+        e_q_o = OF.field_q_energy(rq["phi_q"], rq["psi"], OF.self_energy_q(s.charges, sigma, ke, s.dielectric),
+                                  s.mesh, s.box)
 
         def err(got, want):
             return {"rel_rms": float(np.sqrt(np.mean((got - want) ** 2)) / np.sqrt(np.mean(want ** 2))),
                     "rel_max": float(np.abs(got - want).max() / np.abs(want).max())}
         out["accuracy_vs_fp64_oracle"] = {"field_force": err(f_field.cpu().numpy(), fr),
                                           "pme_force": err(f_q.cpu().numpy(), rq["forces"]),
-                                          "dihedral_force": err(f_dih.cpu().numpy(), fdo)}
+                                          "dihedral_force": err(f_dih.cpu().numpy(), fdo),
+                                          "field_energy_rel_err": abs(energies["field"] - e_field_o) / abs(e_field_o),
+                                          "pme_energy_rel_err": abs(energies["field_q"] - e_q_o) / abs(e_q_o),
+                                          "dihedral_energy_rel_err": abs(energies["dihedral"] - edo) / abs(edo)}
         out["cpu_oracle_ms"] = cpu
         out["cpu_oracle_cores"] = min(os.cpu_count() or 1, 16)
     pm.close()
@@ -300,8 +312,11 @@ def main():
     log(f"system ready: {s.n} beads, mesh {s.mesh}")
     md = RespaMD(s)
     log("device state and initial forces ready")
-    f_field0 = md.f_field.cpu().numpy()
-    f_fast0 = md.f_fast.cpu().numpy()
+    f_field0 = md.in_input_order(md.f_field).cpu().numpy()
+    f_fast0 = md.in_input_order(md.f_fast).cpu().numpy()
+    md.slow_forces(energy=True)      # energies at the initial configuration (forces recomputed unchanged)
+    md.fast_forces(energy=True)
+    e0 = dict(md.energies)
     md.run(args.warmup, use_graph=not args.no_graph)
     torch.cuda.synchronize()
 
@@ -340,9 +355,14 @@ def main():
         log(f"cpu baseline: {args.cpu_steps} oracle outer steps")
         f0, cpu = cpu_baseline(s, args.cpu_steps)
         log(f"cpu baseline: {cpu['value']:.4g} particle-steps/s")
-        fo_field, fo_fast = f0
+        fo_field, fo_fast, eo = f0
         # FP32 device forces vs the FP64 oracle at the same initial configuration
         accuracy = {
+            # the synthetic chains start with every bond at r0, so E_bond is
+            # rounding-level there: its error is reported with the values
+            "energies": {k: {"device": e0[k], "oracle": eo[k], "abs_err": abs(e0[k] - eo[k]),
+                             "rel_err": abs(e0[k] - eo[k]) / abs(eo[k]) if eo[k] else None}
+                         for k in ("field", "bond", "angle")},
             "field_force_rel_rms": float(np.sqrt(np.mean((f_field0 - fo_field) ** 2)) / np.sqrt(np.mean(fo_field ** 2))),
             "field_force_rel_max": float(np.abs(f_field0 - fo_field).max() / np.abs(fo_field).max()),
             "bonded_force_rel_rms": float(np.sqrt(np.mean((f_fast0 - fo_fast) ** 2)) / np.sqrt(np.mean(fo_fast ** 2))),

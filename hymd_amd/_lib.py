@@ -72,12 +72,14 @@ SIGNATURES = {
     "hymd_md_time_stage": [_p, _i32, _i32],
     "hymd_md_set_fused": [_p, _i32],
     "hymd_md_is_fused": [_p],
+    "hymd_md_set_reorder": [_p, _i32, _p, _p, _p],
+    "hymd_md_single_tail": [_p],
     "hymd_fused_create": [ctypes.POINTER(_p), _i64, _i64, _p, _p, _p, _p, _i64, _p, _p, _p, _p, _p],
     "hymd_fused_destroy": [_p],
     "hymd_fused_inner": [_p, _p, _p, _p, _p, _p, _d, _d, _d, _i32, _i32, _pd, _pd, _pd, _p],
     "hymd_md_stage_time": [_p, _pd, _pi32],
 }
-_RESTYPES = {"hymd_last_error": ctypes.c_char_p}
+_RESTYPES = {"hymd_last_error": ctypes.c_char_p, "hymd_md_single_tail": ctypes.c_int64}
 
 
 class HymdError(RuntimeError):

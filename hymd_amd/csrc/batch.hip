@@ -68,7 +68,8 @@ __global__ void __launch_bounds__(kBlock) batch_key_kernel(const float *__restri
                                                             const RepGeom *__restrict__ geo, int2 *__restrict__ keys,
                                                             int *__restrict__ counts) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {  // uniform per wavefront
+    for (int64_t i0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x; i0 < n;
+         i0 += stride) {  // uniform per wavefront
         const int64_t i = i0 + threadIdx.x;
         int key = -1;
         if (i < n) {
@@ -85,8 +86,7 @@ __global__ void __launch_bounds__(kBlock) batch_key_kernel(const float *__restri
 
 // One workgroup per global mesh row (all replicas): the row paint of the
 // single-mesh path with the replica's geometry from the table.
-__global__ void __launch_bounds__(kBlock) batch_paint_rows_kernel(const float *__restrict__ xs,
-                                                                   const int32_t *__restrict__ ts,
+__global__ void __launch_bounds__(kBlock) batch_paint_rows_kernel(const float4 *__restrict__ rec,
                                                                    const int *__restrict__ start,
                                                                    const RepGeom *__restrict__ geo, int S,
                                                                    PaintArgs pa, float *__restrict__ out) {
@@ -118,10 +118,10 @@ __global__ void __launch_bounds__(kBlock) batch_paint_rows_kernel(const float *_
             ++q;
         }
         const int p = rs[q] + off;
-        const int t = ts[p];
+        const float4 r = rec[p];
+        const int t = __float_as_int(r.w);
         if ((unsigned)t >= (unsigned)T) continue;
-        row_deposit(xs + 3 * (int64_t)p, q >> 1, q & 1, pa.mass[t], pa.scale, G.inv_h, G.mx, G.my, G.mz,
-                    acc + t * G.mz);
+        row_deposit(r, q >> 1, q & 1, pa.mass[t], pa.scale, G.inv_h, G.mx, G.my, G.mz, acc + t * G.mz);
     }
     __syncthreads();
     float *o = out + G.grid_off + (int64_t)row * G.mz;
@@ -166,7 +166,7 @@ struct hymd_batch {
     std::vector<int64_t> poff;          // caller particle offsets (R + 1)
     std::vector<std::unique_ptr<RepGroup>> groups;
     DevBuf d_geo, d_slot_of, d_poff, d_sigma2, d_nb, d_row0;
-    DevBuf keys, perm, xs, ts, counts, start, tiles;
+    DevBuf keys, perm, xs, counts, start, tiles;  // xs: binned records (x, y, z, type bits)
     DevBuf rho, spec, partials, result;
     HostBuf h_sigma2, h_result;
     // captured sequence
@@ -194,11 +194,11 @@ static int batch_enqueue(hymd_batch *b, const float *x, const int32_t *types) {
     HYMD_TRY(exclusive_scan((int *)b->counts.ptr, (int *)b->start.ptr, b->rows, (int *)b->tiles.ptr, nullptr,
                             true, s));
     HYMD_TRY(launch_key_scatter(x, types, nullptr, b->n, (const int2 *)b->keys.ptr, (const int *)b->start.ptr,
-                                (int *)b->perm.ptr, (float *)b->xs.ptr, (int32_t *)b->ts.ptr, nullptr, s));
+                                (int *)b->perm.ptr, (float4 *)b->xs.ptr, nullptr, s));
     int max_mz = 0;
     for (const RepGeom &G : b->geo) max_mz = std::max(max_mz, G.mz);
     batch_paint_rows_kernel<<<b->rows, kBlock, sizeof(long long) * b->T * max_mz, s>>>(
-        (const float *)b->xs.ptr, (const int32_t *)b->ts.ptr, (const int *)b->start.ptr,
+        (const float4 *)b->xs.ptr, (const int *)b->start.ptr,
         (const RepGeom *)b->d_geo.ptr, S, b->pa, (float *)b->rho.ptr);
     HYMD_HIP(hipGetLastError());
     for (const auto &gp : b->groups) {
@@ -336,8 +336,7 @@ extern "C" int hymd_batch_create(hymd_batch **out, int32_t n_replicas, const int
     HYMD_TRY(b->d_row0.reserve(sizeof(int64_t) * R));
     HYMD_TRY(b->keys.reserve(sizeof(int2) * nn));
     HYMD_TRY(b->perm.reserve(sizeof(int) * nn));
-    HYMD_TRY(b->xs.reserve(sizeof(float) * 3 * nn));
-    HYMD_TRY(b->ts.reserve(sizeof(int32_t) * nn));
+    HYMD_TRY(b->xs.reserve(sizeof(float4) * nn));
     HYMD_TRY(b->counts.reserve(sizeof(int) * (rows + 2)));
     HYMD_TRY(b->start.reserve(sizeof(int) * (rows + 1)));
     HYMD_TRY(b->tiles.reserve(sizeof(int) * (scan_tiles(rows) + 1)));

@@ -136,6 +136,27 @@ __device__ __forceinline__ int rank_in_bin(int key, int *counts) {
     return rank;
 }
 
+// Division by a divisor fixed at launch, as a multiply-high and a shift
+// (Granlund & Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, round-up multiplier): with s = ceil(log2 d) and
+// m = floor(2^32 (2^s - d) / d) + 1, n / d = (umulhi(n, m) + n) >> s for every
+// 0 <= n < 2^31 (the sum cannot overflow since umulhi(n, m) <= n).  The FFT
+// stages use it where an integer divide would cost ~20 instructions per lane.
+struct FastDiv {
+    uint32_t d, m, s;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+    uint32_t s = 0;
+    while ((1ull << s) < d) ++s;
+    const uint64_t m = (((1ull << s) - d) << 32) / d + 1;
+    return FastDiv{d, (uint32_t)m, s};
+}
+
+__device__ __forceinline__ int fdiv(int n, const FastDiv &f) {
+    return (int)((__umulhi((uint32_t)n, f.m) + (uint32_t)n) >> f.s);
+}
+
 // mesh geometry passed by value to kernels
 struct Mesh {
     int mx, my, mz, zh;     // real edges, zh = mz/2+1
@@ -157,10 +178,10 @@ __device__ __forceinline__ float2 cmul_i(float2 a, float s) { return make_float2
 
 // Deterministic block reduction of doubles (fixed order: a shuffle tree
 // inside each 64-wide wavefront, then the wavefront sums in order).  All
-// kBlock threads of the block must call it.
-template <int NV>
+// NT (the block size) threads of the block must call it.
+template <int NV, int NT = kBlock>
 __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *out_block) {
-    constexpr int kWaves = kBlock / 64;
+    constexpr int kWaves = NT / 64;
     __shared__ double red[NV][kWaves];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int j = 0; j < NV; ++j) {
@@ -225,6 +246,11 @@ struct RowKeyTarget {
     int mx, my;
 };
 int field_key_target(hymd_pm *pm, int64_t n, RowKeyTarget *t);
+// First row of the trailing run of one-particle molecules of a fused handle
+// (their rows carry no bonded terms, so they may be permuted among themselves).
+int64_t fused_single_tail(const hymd_fused *f);
+// order[slot] = i for the n particles at x in mesh-row order (between updates).
+int row_order(hymd_pm *pm, const float *x, int64_t n, int *order, hipStream_t s);
 // hymd_update_field for force grids only, with the keys of the float32
 // positions x already produced into field_key_target()'s arrays
 int update_field_keyed(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,

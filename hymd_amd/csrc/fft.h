@@ -26,6 +26,13 @@ struct FftAxis {
     int ld;       // LDS row stride (complex elements)
     int radix[kMaxStages];
     int twoff[kMaxStages];  // offset of the stage's twiddles in the table
+    FastDiv ndiv;               // n
+    FastDiv pdiv[kMaxStages];   // p, the sub-transform size before the stage
+    FastDiv nrdiv[kMaxStages];  // n / radix, the butterflies per line
+};
+
+struct PlaneDivs {
+    FastDiv my, zh, nh, pairs;  // My, Mz/2+1, Mz/2, Mz/4+1
 };
 
 struct FftPlan {
@@ -36,6 +43,7 @@ struct FftPlan {
     DevBuf post;                 // exp(-2 pi i k / Mz), k = 0..Mz/2
     size_t lds_x, lds_y, lds_z;  // dynamic LDS bytes per kernel
     size_t lds_plane;            // y-z plane kernels (0: planes do not fit, separate passes)
+    PlaneDivs pdiv;              // divisors of the plane kernels' index maps
 };
 
 int fft_plan_init(FftPlan &plan, const Mesh &mesh);
@@ -52,6 +60,13 @@ int fft_r2c_real(const FftPlan &plan, const float *in, float2 *out, int batch, h
 int fft_inverse_x(const FftPlan &plan, float2 *data, int batch, hipStream_t s);
 int fft_c2r_yz(const FftPlan &plan, float2 *in, float *out, int batch, float scale, hipStream_t s);
 int fft_c2r(const FftPlan &plan, float2 *data, float *out, int batch, float scale, hipStream_t s);
+// The 3T gradient grids out[3t + d] = c2r(-i k_d V_t) from the x-inverse
+// transformed spectra in[t] = IFFT_x(-i k_x V_t) and in[T + t] = IFFT_x(V_t)
+// (k_y = kfy n_y, k_z = kfz n_z, zero on Nyquist planes); needs plan.lds_plane.
+int fft_c2r_yz_grad(const FftPlan &plan, const float2 *in, float *out, int T, double kfy, double kfz, float scale,
+                    hipStream_t s);
+// Whether fft_inplace<elems> can transform nlines lines of this axis with `threads` threads.
+bool fft_inplace_fits(const FftAxis &ax, int nlines, int threads, int elems);
 
 // ---------------------------------------------------------------------------
 // device building blocks
@@ -129,13 +144,15 @@ __device__ __forceinline__ void small_dft<8>(float2 (&u)[8], float sgn) {
 
 template <int R>
 __device__ __forceinline__ void stockham_stage(const float2 *__restrict__ src, float2 *__restrict__ dst, int W,
-                                               int ld, int N, int p, const float2 *__restrict__ tw, float sgn) {
+                                               int ld, int N, int p, const FastDiv &fp, const FastDiv &fnr,
+                                               const float2 *__restrict__ tw, float sgn) {
     const int NR = N / R;
     const int total = W * NR;
     for (int b = threadIdx.x; b < total; b += blockDim.x) {
-        const int l = b / NR;
+        const int l = fdiv(b, fnr);
         const int i = b - l * NR;
-        const int k = i % p;
+        const int q = fdiv(i, fp);
+        const int k = i - q * p;
         const float2 *s = src + l * ld;
         float2 u[8];
 #pragma unroll
@@ -150,7 +167,7 @@ __device__ __forceinline__ void stockham_stage(const float2 *__restrict__ src, f
         }
         small_dft<R>(u, sgn);
         float2 *d = dst + l * ld;
-        const int ob = (i / p) * p * R + k;
+        const int ob = q * p * R + k;
 #pragma unroll
         for (int j = 0; j < R; ++j) d[ob + j * p] = u[j];
     }
@@ -164,12 +181,13 @@ __device__ __forceinline__ float2 *fft_lds(float2 *a, float2 *b, int W, const Ff
     for (int s = 0; s < ax.nstages; ++s) {
         const int R = ax.radix[s];
         const float2 *tws = tw + ax.twoff[s];
+        const FastDiv &fp = ax.pdiv[s], &fnr = ax.nrdiv[s];
         switch (R) {
-            case 2: stockham_stage<2>(a, b, W, ax.ld, ax.n, p, tws, sgn); break;
-            case 3: stockham_stage<3>(a, b, W, ax.ld, ax.n, p, tws, sgn); break;
-            case 4: stockham_stage<4>(a, b, W, ax.ld, ax.n, p, tws, sgn); break;
-            case 5: stockham_stage<5>(a, b, W, ax.ld, ax.n, p, tws, sgn); break;
-            default: stockham_stage<8>(a, b, W, ax.ld, ax.n, p, tws, sgn); break;
+            case 2: stockham_stage<2>(a, b, W, ax.ld, ax.n, p, fp, fnr, tws, sgn); break;
+            case 3: stockham_stage<3>(a, b, W, ax.ld, ax.n, p, fp, fnr, tws, sgn); break;
+            case 4: stockham_stage<4>(a, b, W, ax.ld, ax.n, p, fp, fnr, tws, sgn); break;
+            case 5: stockham_stage<5>(a, b, W, ax.ld, ax.n, p, fp, fnr, tws, sgn); break;
+            default: stockham_stage<8>(a, b, W, ax.ld, ax.n, p, fp, fnr, tws, sgn); break;
         }
         __syncthreads();
         float2 *t = a;
@@ -188,12 +206,13 @@ __device__ __forceinline__ float2 *fft_lds(float2 *a, float2 *b, int W, const Ff
 // nlines * N <= kPlaneThreads * KB * R.
 // ---------------------------------------------------------------------------
 constexpr int kPlaneThreads = 512;
+constexpr int kPlaneLoadBatch = 4;  // loads in flight per thread in the plane kernels (64-VGPR budget)
 constexpr size_t kPlaneLdsBytes = 150 * 1024;  // of the 160 KiB a gfx950 workgroup may use
 constexpr int kPlaneElems = 32;  // complex values per thread per stage (KB * R <= 32)
 
 template <int R, int KB>
-__device__ __forceinline__ void stockham_inplace(float2 *buf, int se, int sl, int nlines, int N, int p,
-                                                 const float2 *__restrict__ tw, float sgn) {
+__device__ __forceinline__ void stockham_inplace(float2 *buf, int se, int sl, int nlines, const FastDiv &fl, int N,
+                                                 int p, const FastDiv &fp, const float2 *__restrict__ tw, float sgn) {
     const int NR = N / R;
     const int total = nlines * NR;
     float2 u[KB][8];
@@ -201,7 +220,7 @@ __device__ __forceinline__ void stockham_inplace(float2 *buf, int se, int sl, in
     for (int kb = 0; kb < KB; ++kb) {
         const int b = threadIdx.x + kb * blockDim.x;
         if (b >= total) continue;
-        const int i = b / nlines, l = b - i * nlines, k = i % p;
+        const int i = fdiv(b, fl), l = b - i * nlines, k = i - fdiv(i, fp) * p;
         const float2 *s = buf + l * sl;
 #pragma unroll
         for (int j = 0; j < R; ++j) u[kb][j] = s[(i + j * NR) * se];
@@ -220,29 +239,32 @@ __device__ __forceinline__ void stockham_inplace(float2 *buf, int se, int sl, in
     for (int kb = 0; kb < KB; ++kb) {
         const int b = threadIdx.x + kb * blockDim.x;
         if (b >= total) continue;
-        const int i = b / nlines, l = b - i * nlines, k = i % p;
+        const int i = fdiv(b, fl), l = b - i * nlines, q = fdiv(i, fp), k = i - q * p;
         float2 *d = buf + l * sl;
-        const int ob = (i / p) * p * R + k;
+        const int ob = q * p * R + k;
 #pragma unroll
         for (int j = 0; j < R; ++j) d[(ob + j * p) * se] = u[kb][j];
     }
     __syncthreads();
 }
 
-// In-place transform of nlines lines (strides as above); the caller barriers
-// before (data ready) and may read the result right after.
-__device__ __forceinline__ void fft_inplace(float2 *buf, int se, int sl, int nlines, const FftAxis &ax,
-                                            const float2 *__restrict__ tw, float sgn) {
+// In-place transform of nlines lines (strides as above) with at most ELEMS
+// values per thread and stage; the caller barriers before (data ready) and
+// may read the result right after.
+template <int ELEMS = kPlaneElems>
+__device__ __forceinline__ void fft_inplace(float2 *buf, int se, int sl, int nlines, const FastDiv &fl,
+                                            const FftAxis &ax, const float2 *__restrict__ tw, float sgn) {
     int p = 1;
     for (int s = 0; s < ax.nstages; ++s) {
         const int R = ax.radix[s];
         const float2 *tws = tw + ax.twoff[s];
+        const FastDiv &fp = ax.pdiv[s];
         switch (R) {
-            case 2: stockham_inplace<2, kPlaneElems / 2>(buf, se, sl, nlines, ax.n, p, tws, sgn); break;
-            case 3: stockham_inplace<3, kPlaneElems / 3>(buf, se, sl, nlines, ax.n, p, tws, sgn); break;
-            case 4: stockham_inplace<4, kPlaneElems / 4>(buf, se, sl, nlines, ax.n, p, tws, sgn); break;
-            case 5: stockham_inplace<5, kPlaneElems / 5>(buf, se, sl, nlines, ax.n, p, tws, sgn); break;
-            default: stockham_inplace<8, kPlaneElems / 8>(buf, se, sl, nlines, ax.n, p, tws, sgn); break;
+            case 2: stockham_inplace<2, ELEMS / 2>(buf, se, sl, nlines, fl, ax.n, p, fp, tws, sgn); break;
+            case 3: stockham_inplace<3, ELEMS / 3>(buf, se, sl, nlines, fl, ax.n, p, fp, tws, sgn); break;
+            case 4: stockham_inplace<4, ELEMS / 4>(buf, se, sl, nlines, fl, ax.n, p, fp, tws, sgn); break;
+            case 5: stockham_inplace<5, ELEMS / 5>(buf, se, sl, nlines, fl, ax.n, p, fp, tws, sgn); break;
+            default: stockham_inplace<8, ELEMS / 8>(buf, se, sl, nlines, fl, ax.n, p, fp, tws, sgn); break;
         }
         p *= R;
     }

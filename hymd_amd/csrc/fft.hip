@@ -29,6 +29,7 @@ static int axis_init(FftAxis &ax, DevBuf &tw, int n, int pad) {
         return HYMD_ERR_UNSUPPORTED;
     }
     ax.n = n;
+    ax.ndiv = make_fastdiv((uint32_t)n);
     ax.nstages = (int)rs.size();
     ax.ld = n + pad;
     std::vector<float2> host(n > 1 ? n - 1 : 1, make_float2(0.f, 0.f));
@@ -37,6 +38,8 @@ static int axis_init(FftAxis &ax, DevBuf &tw, int n, int pad) {
         int R = rs[s];
         ax.radix[s] = R;
         ax.twoff[s] = off;
+        ax.pdiv[s] = make_fastdiv((uint32_t)p);
+        ax.nrdiv[s] = make_fastdiv((uint32_t)(n / R));
         for (int k = 0; k < p; ++k)
             for (int j = 1; j < R; ++j) {
                 double ang = -2.0 * M_PI * (double)(j * k) / (double)(p * R);
@@ -60,6 +63,8 @@ int fft_plan_init(FftPlan &plan, const Mesh &mesh) {
     HYMD_TRY(axis_init(plan.ay, plan.tw_y, mesh.my, 1));
     HYMD_TRY(axis_init(plan.azh, plan.tw_zh, mesh.mz / 2, 0));
     plan.azh.ld = mesh.zh;  // rows hold Mz/2+1 spectra in c2r
+    plan.pdiv = PlaneDivs{make_fastdiv(mesh.my), make_fastdiv(mesh.zh), make_fastdiv(mesh.mz / 2),
+                          make_fastdiv(mesh.mz / 4 + 1)};
     std::vector<float2> post(mesh.zh);
     for (int k = 0; k < mesh.zh; ++k) {
         double ang = -2.0 * M_PI * (double)k / (double)mesh.mz;
@@ -71,22 +76,17 @@ int fft_plan_init(FftPlan &plan, const Mesh &mesh) {
     plan.wx = lines_for(mesh.mx);
     plan.wy = lines_for(mesh.my);
     plan.wz = std::max(1, 2048 / (mesh.mz / 2));
-    // strided passes: twiddles, two line buffers, W int64 line offsets
-    plan.lds_x = sizeof(float2) * (size_t)(plan.ax.n + 2 * plan.wx * plan.ax.ld) + sizeof(int64_t) * plan.wx;
-    plan.lds_y = sizeof(float2) * (size_t)(plan.ay.n + 2 * plan.wy * plan.ay.ld) + sizeof(int64_t) * plan.wy;
+    // strided passes: twiddles, two line buffers
+    plan.lds_x = sizeof(float2) * (size_t)(plan.ax.n + 2 * plan.wx * plan.ax.ld);
+    plan.lds_y = sizeof(float2) * (size_t)(plan.ay.n + 2 * plan.wy * plan.ay.ld);
     plan.lds_z = sizeof(float2) * (size_t)(plan.azh.n + mesh.zh + 2 * plan.wz * plan.azh.ld);
     // y-z plane kernels: twiddles (y, z half), post twiddles, one My x zh plane;
     // every in-place stage must fit kPlaneElems values per thread
     plan.lds_plane = sizeof(float2) * ((size_t)mesh.my + (size_t)plan.azh.n + 2 * (size_t)mesh.zh +
                                        (size_t)mesh.my * mesh.zh);
-    auto fits = [](const FftAxis &ax, int nlines) {
-        for (int st = 0; st < ax.nstages; ++st) {
-            const int R = ax.radix[st];
-            if ((int64_t)nlines * ax.n > (int64_t)kPlaneThreads * (kPlaneElems / R) * R) return false;
-        }
-        return true;
-    };
-    if (plan.lds_plane > kPlaneLdsBytes || !fits(plan.ay, mesh.zh) || !fits(plan.azh, mesh.my)) plan.lds_plane = 0;
+    if (plan.lds_plane > kPlaneLdsBytes || !fft_inplace_fits(plan.ay, mesh.zh, kPlaneThreads, kPlaneElems) ||
+        !fft_inplace_fits(plan.azh, mesh.my, kPlaneThreads, kPlaneElems))
+        plan.lds_plane = 0;
     const size_t lds_max = 64 * 1024;  // the line kernels stay within the default dynamic-LDS limit
     if (plan.lds_x > lds_max || plan.lds_y > lds_max || plan.lds_z > lds_max) {
         set_error("mesh too large for the LDS-staged FFT");
@@ -219,17 +219,15 @@ __global__ void __launch_bounds__(kBlock) fft_strided_kernel(float2 *__restrict_
     float2 *tw = smem;
     float2 *A = tw + n;
     float2 *B = A + W * ax.ld;
-    int64_t *lbase = reinterpret_cast<int64_t *>(B + W * ax.ld);  // W line offsets (-1: past the end)
-    for (int i = threadIdx.x; i < n - 1; i += blockDim.x) tw[i] = tw_glob[i];
     const unsigned flat = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
     const unsigned bx = flat % gridDim.x, by = flat / gridDim.x;
     float2 *d = data + (int64_t)by * batch_stride;
     const int line0 = (int)bx * W;
-    if ((int)threadIdx.x < W) {
-        const int line = line0 + (int)threadIdx.x;
-        lbase[threadIdx.x] = line < nlines ? (int64_t)(line / inner) * outer + (line % inner) : -1;
-    }
-    __syncthreads();
+    // element idx of the block is line idx & (W - 1), element idx >> wshift;
+    // kBlock is a multiple of W, so a thread only ever touches line tid & (W-1)
+    const int myl = (int)threadIdx.x & (W - 1);
+    const int myline = line0 + myl;
+    const int64_t mybase = myline < nlines ? (int64_t)(myline / inner) * outer + (myline % inner) : -1;
     const int nel = W * n;
     for (int i0 = threadIdx.x; i0 < nel; i0 += kBlock * kLoadBatch) {
         float2 v[kLoadBatch];
@@ -237,24 +235,19 @@ __global__ void __launch_bounds__(kBlock) fft_strided_kernel(float2 *__restrict_
         for (int j = 0; j < kLoadBatch; ++j) {
             const int idx = i0 + j * kBlock;
             v[j] = make_float2(0.f, 0.f);
-            if (idx < nel) {
-                const int64_t b = lbase[idx & (W - 1)];
-                if (b >= 0) v[j] = d[b + (int64_t)(idx >> wshift) * es];
-            }
+            if (idx < nel && mybase >= 0) v[j] = d[mybase + (int64_t)(idx >> wshift) * es];
         }
 #pragma unroll
         for (int j = 0; j < kLoadBatch; ++j) {
             const int idx = i0 + j * kBlock;
-            if (idx < nel) A[(idx & (W - 1)) * ax.ld + (idx >> wshift)] = v[j];
+            if (idx < nel) A[myl * ax.ld + (idx >> wshift)] = v[j];
         }
     }
+    for (int i = threadIdx.x; i < n - 1; i += blockDim.x) tw[i] = tw_glob[i];
     __syncthreads();
     float2 *r = fft_lds(A, B, W, ax, tw, sgn);
-    for (int idx = threadIdx.x; idx < nel; idx += blockDim.x) {
-        const int l = idx & (W - 1), e = idx >> wshift;
-        const int64_t b = lbase[l];
-        if (b >= 0) d[b + (int64_t)e * es] = r[l * ax.ld + e];
-    }
+    if (mybase >= 0)
+        for (int idx = threadIdx.x; idx < nel; idx += blockDim.x) d[mybase + (int64_t)(idx >> wshift) * es] = r[myl * ax.ld + (idx >> wshift)];
 }
 
 // ---------------------------------------------------------------------------
@@ -270,7 +263,8 @@ __global__ void __launch_bounds__(kPlaneThreads) r2c_zy_plane_kernel(const float
                                                                       int mz, FftAxis ay, FftAxis azh,
                                                                       const float2 *__restrict__ twy_g,
                                                                       const float2 *__restrict__ twz_g,
-                                                                      const float2 *__restrict__ post_g) {
+                                                                      const float2 *__restrict__ post_g,
+                                                                      PlaneDivs dv) {
     extern __shared__ float2 smem[];
     const int nh = azh.n, zh = nh + 1;
     float2 *twy = smem;
@@ -285,29 +279,29 @@ __global__ void __launch_bounds__(kPlaneThreads) r2c_zy_plane_kernel(const float
     // rows of Mz reals as nh complex, row stride zh in D
     const float2 *src = reinterpret_cast<const float2 *>(in) + plane * my * nh;
     const int nel = my * nh;
-    for (int i0 = threadIdx.x; i0 < nel; i0 += kPlaneThreads * kLoadBatch) {
-        float2 v[kLoadBatch];
+    for (int i0 = threadIdx.x; i0 < nel; i0 += kPlaneThreads * kPlaneLoadBatch) {
+        float2 v[kPlaneLoadBatch];
 #pragma unroll
-        for (int j = 0; j < kLoadBatch; ++j) {
+        for (int j = 0; j < kPlaneLoadBatch; ++j) {
             const int idx = i0 + j * kPlaneThreads;
             v[j] = idx < nel ? src[idx] : make_float2(0.f, 0.f);
         }
 #pragma unroll
-        for (int j = 0; j < kLoadBatch; ++j) {
+        for (int j = 0; j < kPlaneLoadBatch; ++j) {
             const int idx = i0 + j * kPlaneThreads;
             if (idx < nel) {
-                const int y = idx / nh;
+                const int y = fdiv(idx, dv.nh);
                 D[y * zh + (idx - y * nh)] = v[j];
             }
         }
     }
     __syncthreads();
-    fft_inplace(D, 1, zh, my, azh, twz, -1.f);
+    fft_inplace(D, 1, zh, my, dv.my, azh, twz, -1.f);
     // split in place: the thread of pair (k, nh - k) writes X[k] and X[nh - k]
     // (k = 0 writes X[0] and X[nh]; k = nh/2 is its own pair)
     const int npairs = nh / 2 + 1;
     for (int idx = threadIdx.x; idx < my * npairs; idx += blockDim.x) {
-        const int y = idx / npairs, k = idx - y * npairs;
+        const int y = fdiv(idx, dv.pairs), k = idx - y * npairs;
         float2 *row = D + y * zh;
         const int kc = k == 0 ? 0 : nh - k;
         const float2 zk = row[k], zc = row[kc];
@@ -323,21 +317,34 @@ __global__ void __launch_bounds__(kPlaneThreads) r2c_zy_plane_kernel(const float
         if (km != k) row[km] = xm;
     }
     __syncthreads();
-    fft_inplace(D, zh, 1, zh, ay, twy, -1.f);
+    fft_inplace(D, zh, 1, zh, dv.zh, ay, twy, -1.f);
     float2 *dst = out + plane * my * zh;
     for (int idx = threadIdx.x; idx < my * zh; idx += blockDim.x) dst[idx] = D[idx];
 }
 
 // inverse: FFT along y of the Mz/2+1 columns -> per row Z'[k] = (X[k] + X*[nh-k])
 // + i e^{2 pi i k/Mz} (X[k] - X*[nh-k]) (as c2r_z_kernel) -> inverse FFT of
-// Mz/2 -> My rows of Mz reals, times `scale`
+// Mz/2 -> My rows of Mz reals, times `scale`.
+// GRAD: the 3T gradient grids of fft_c2r_yz_grad.  Logical block
+// (t Mx + x) 3 + d (the two planes that read in[T + t] are neighbours, so the
+// second read is served by L2); d = 0 transforms in[t], d = 1 multiplies
+// in[T + t] by -i k_y before the y pass, d = 2 by -i k_z after it; out grid 3t + d.
+struct GradK {
+    int T;
+    double kfy, kfz;  // 2 pi / L_y, 2 pi / L_z
+};
+
+// -i kd v for a derivative factor kd
+__device__ __forceinline__ float2 mul_minus_i(float2 v, float kd) { return make_float2(kd * v.y, -kd * v.x); }
+
+template <bool GRAD>
 __global__ void __launch_bounds__(kPlaneThreads) c2r_yz_plane_kernel(const float2 *__restrict__ in,
                                                                       float *__restrict__ out, int mx, int my,
                                                                       int mz, FftAxis ay, FftAxis azh,
                                                                       const float2 *__restrict__ twy_g,
                                                                       const float2 *__restrict__ twz_g,
                                                                       const float2 *__restrict__ post_g,
-                                                                      float scale) {
+                                                                      float scale, GradK gk, PlaneDivs dv) {
     extern __shared__ float2 smem[];
     const int nh = azh.n, zh = nh + 1;
     float2 *twy = smem;
@@ -348,31 +355,52 @@ __global__ void __launch_bounds__(kPlaneThreads) c2r_yz_plane_kernel(const float
     for (int i = threadIdx.x; i < nh - 1; i += blockDim.x) twz[i] = twz_g[i];
     for (int i = threadIdx.x; i < zh; i += blockDim.x) post[i] = post_g[i];
     const unsigned flat = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
-    const int64_t plane = (int64_t)(flat / gridDim.x) * mx + (flat % gridDim.x);
-    const float2 *src = in + plane * my * zh;
+    int64_t in_plane, out_plane;
+    int d = 0;
+    if (GRAD) {
+        d = (int)(flat % 3u);
+        const int rest = (int)(flat / 3u), x = rest % mx, t = rest / mx;
+        in_plane = (int64_t)(d == 0 ? t : gk.T + t) * mx + x;
+        out_plane = (int64_t)(3 * t + d) * mx + x;
+    } else {
+        in_plane = out_plane = (int64_t)(flat / gridDim.x) * mx + (flat % gridDim.x);
+    }
+    const float2 *src = in + in_plane * my * zh;
     const int nel = my * zh;
-    for (int i0 = threadIdx.x; i0 < nel; i0 += kPlaneThreads * kLoadBatch) {
-        float2 v[kLoadBatch];
+    for (int i0 = threadIdx.x; i0 < nel; i0 += kPlaneThreads * kPlaneLoadBatch) {
+        float2 v[kPlaneLoadBatch];
 #pragma unroll
-        for (int j = 0; j < kLoadBatch; ++j) {
+        for (int j = 0; j < kPlaneLoadBatch; ++j) {
             const int idx = i0 + j * kPlaneThreads;
             v[j] = idx < nel ? src[idx] : make_float2(0.f, 0.f);
         }
 #pragma unroll
-        for (int j = 0; j < kLoadBatch; ++j) {
+        for (int j = 0; j < kPlaneLoadBatch; ++j) {
             const int idx = i0 + j * kPlaneThreads;
-            if (idx < nel) D[idx] = v[j];
+            if (idx < nel) {
+                float2 w = v[j];
+                if (GRAD && d == 1) {  // row y: k_y, zero on the Nyquist row
+                    const int y = fdiv(idx, dv.zh);
+                    const int ny = 2 * y < my ? y : y - my;
+                    w = mul_minus_i(w, 2 * y == my ? 0.f : (float)(gk.kfy * ny));
+                }
+                D[idx] = w;
+            }
         }
     }
     __syncthreads();
-    fft_inplace(D, zh, 1, zh, ay, twy, 1.f);
+    fft_inplace(D, zh, 1, zh, dv.zh, ay, twy, 1.f);
     // pre-process in place, pairs (k, nh - k) per thread; k = 0 reads X[nh]
     const int npairs = nh / 2 + 1;
     for (int idx = threadIdx.x; idx < my * npairs; idx += blockDim.x) {
-        const int y = idx / npairs, k = idx - y * npairs;
+        const int y = fdiv(idx, dv.pairs), k = idx - y * npairs;
         float2 *row = D + y * zh;
         const int kc = nh - k;  // k = 0 -> X[nh]
-        const float2 xk = row[k], xc = row[kc];
+        float2 xk = row[k], xc = row[kc];
+        if (GRAD && d == 2) {  // column kz: k_z, zero on the Nyquist column
+            xk = mul_minus_i(xk, 2 * k == mz ? 0.f : (float)(gk.kfz * k));
+            xc = mul_minus_i(xc, 2 * kc == mz ? 0.f : (float)(gk.kfz * kc));
+        }
         const float2 s1 = cadd(xk, cconj(xc));
         const float2 d1 = cmul(cconj(post[k]), csub(xk, cconj(xc)));
         const float2 zk = cadd(s1, cmul_i(d1, 1.f));
@@ -384,10 +412,10 @@ __global__ void __launch_bounds__(kPlaneThreads) c2r_yz_plane_kernel(const float
         row[k] = zk;
     }
     __syncthreads();
-    fft_inplace(D, 1, zh, my, azh, twz, 1.f);
-    float2 *dst = reinterpret_cast<float2 *>(out) + plane * my * nh;
+    fft_inplace(D, 1, zh, my, dv.my, azh, twz, 1.f);
+    float2 *dst = reinterpret_cast<float2 *>(out) + out_plane * my * nh;
     for (int idx = threadIdx.x; idx < my * nh; idx += blockDim.x) {
-        const int y = idx / nh;
+        const int y = fdiv(idx, dv.nh);
         const float2 v = D[y * zh + (idx - y * nh)];
         dst[idx] = make_float2(v.x * scale, v.y * scale);
     }
@@ -424,7 +452,9 @@ static int plane_lds_opt_in() {
     if (!done) {
         HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&r2c_zy_plane_kernel),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLdsBytes));
-        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&c2r_yz_plane_kernel),
+        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&c2r_yz_plane_kernel<false>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLdsBytes));
+        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&c2r_yz_plane_kernel<true>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLdsBytes));
         done = true;
     }
@@ -446,7 +476,7 @@ int fft_r2c_zy(const FftPlan &plan, const float *in, float2 *out, int batch, hip
         dim3 grid((unsigned)m.mx, (unsigned)batch);
         r2c_zy_plane_kernel<<<grid, kPlaneThreads, plan.lds_plane, s>>>(
             in, out, m.mx, m.my, m.mz, plan.ay, plan.azh, (const float2 *)plan.tw_y.ptr,
-            (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr);
+            (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr, plan.pdiv);
         HYMD_HIP(hipGetLastError());
         return HYMD_OK;
     }
@@ -469,9 +499,9 @@ int fft_c2r_yz(const FftPlan &plan, float2 *in, float *out, int batch, float sca
     if (plan.lds_plane) {
         HYMD_TRY(plane_lds_opt_in());
         dim3 grid((unsigned)m.mx, (unsigned)batch);
-        c2r_yz_plane_kernel<<<grid, kPlaneThreads, plan.lds_plane, s>>>(
+        c2r_yz_plane_kernel<false><<<grid, kPlaneThreads, plan.lds_plane, s>>>(
             in, out, m.mx, m.my, m.mz, plan.ay, plan.azh, (const float2 *)plan.tw_y.ptr,
-            (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr, scale);
+            (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr, scale, GradK{}, plan.pdiv);
         HYMD_HIP(hipGetLastError());
         return HYMD_OK;
     }
@@ -483,6 +513,30 @@ int fft_c2r_yz(const FftPlan &plan, float2 *in, float *out, int batch, float sca
                                                    scale, plan.wz);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
+}
+
+int fft_c2r_yz_grad(const FftPlan &plan, const float2 *in, float *out, int T, double kfy, double kfz, float scale,
+                    hipStream_t s) {
+    const Mesh &m = plan.mesh;
+    if (!plan.lds_plane) {
+        set_error("the gradient plane pass needs a y-z plane that fits in LDS");
+        return HYMD_ERR_UNSUPPORTED;
+    }
+    HYMD_TRY(plane_lds_opt_in());
+    dim3 grid((unsigned)m.mx, (unsigned)(3 * T));
+    c2r_yz_plane_kernel<true><<<grid, kPlaneThreads, plan.lds_plane, s>>>(
+        in, out, m.mx, m.my, m.mz, plan.ay, plan.azh, (const float2 *)plan.tw_y.ptr, (const float2 *)plan.tw_zh.ptr,
+        (const float2 *)plan.post.ptr, scale, GradK{T, kfy, kfz}, plan.pdiv);
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
+}
+
+bool fft_inplace_fits(const FftAxis &ax, int nlines, int threads, int elems) {
+    for (int st = 0; st < ax.nstages; ++st) {
+        const int R = ax.radix[st];
+        if ((int64_t)nlines * ax.n > (int64_t)threads * (elems / R) * R) return false;
+    }
+    return true;
 }
 
 int fft_c2r(const FftPlan &plan, float2 *data, float *out, int batch, float scale, hipStream_t s) {

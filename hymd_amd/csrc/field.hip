@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(kBlock) row_key_kernel(const float *__restrict
                                                           int64_t n, Mesh m, int2 *__restrict__ keys,
                                                           int *__restrict__ counts) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+    for (int64_t i0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x; i0 < n; i0 += stride) {
         const int64_t i = i0 + threadIdx.x;
         int key = -1;
         if (i < n && !(w && w[i] == 0.f)) key = base_row(x, i, m);
@@ -108,25 +108,23 @@ __global__ void __launch_bounds__(kBlock) key_scatter_kernel(const float *__rest
                                                               const float *__restrict__ w, int64_t n,
                                                               const int2 *__restrict__ keys,
                                                               const int *__restrict__ start, int *__restrict__ perm,
-                                                              float *__restrict__ xs, int32_t *__restrict__ ts,
-                                                              float *__restrict__ ws) {
+                                                              float4 *__restrict__ rec, float *__restrict__ ws) {
+    // consecutive particles land in neighbouring rows: blocks are XCD-remapped
+    // so that the partial cache lines of one output range fill in one L2
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    for (int64_t i = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
         const int2 kr = keys[i];
         if (kr.x < 0) continue;  // left out of the binning
         const int pos = start[kr.x] + kr.y;
         perm[pos] = (int)i;
-        xs[3 * (int64_t)pos] = x[3 * i];
-        xs[3 * (int64_t)pos + 1] = x[3 * i + 1];
-        xs[3 * (int64_t)pos + 2] = x[3 * i + 2];
-        ts[pos] = types ? types[i] : 0;
+        rec[pos] = make_float4(x[3 * i], x[3 * i + 1], x[3 * i + 2], __int_as_float(types ? types[i] : 0));
         if (ws) ws[pos] = w[i];
     }
 }
 
 int launch_key_scatter(const float *x, const int32_t *types, const float *w, int64_t n, const int2 *keys,
-                       const int *start, int *perm, float *xs, int32_t *ts, float *ws, hipStream_t s) {
-    if (n > 0) key_scatter_kernel<<<grid_for(n), kBlock, 0, s>>>(x, types, w, n, keys, start, perm, xs, ts, ws);
+                       const int *start, int *perm, float4 *rec, float *ws, hipStream_t s) {
+    if (n > 0) key_scatter_kernel<<<grid_for(n), kBlock, 0, s>>>(x, types, w, n, keys, start, perm, rec, ws);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }
@@ -139,8 +137,7 @@ int launch_key_scatter(const float *x, const int32_t *types, const float *w, int
 // particle deposits exactly round(m 2^32)), and integer addition makes the
 // sums independent of order.  The row is then written once per type as
 // rho = sum * out_scale (and the int64 sums if out_fixed != NULL).
-__global__ void __launch_bounds__(kBlock) paint_rows_kernel(const float *__restrict__ xs,
-                                                             const int32_t *__restrict__ ts,
+__global__ void __launch_bounds__(kBlock) paint_rows_kernel(const float4 *__restrict__ rec,
                                                              const float *__restrict__ ws,
                                                              const int *__restrict__ start, Mesh m, PaintArgs pa,
                                                              int tc, double out_scale, float *__restrict__ out,
@@ -171,11 +168,12 @@ __global__ void __launch_bounds__(kBlock) paint_rows_kernel(const float *__restr
             ++q;
         }
         const int p = rs[q] + off;
-        const int t = ts[p] - t0;
+        const float4 r = rec[p];
+        const int t = __float_as_int(r.w) - t0;
         if ((unsigned)t >= (unsigned)nt) continue;
         const int a = q >> 1, b = q & 1;
         const double mass = ws ? (double)ws[p] : pa.mass[t0 + t];
-        row_deposit(xs + 3 * (int64_t)p, a, b, mass, pa.scale, m.inv_h, m.mx, m.my, m.mz, acc + t * m.mz);
+        row_deposit(r, a, b, mass, pa.scale, m.inv_h, m.mx, m.my, m.mz, acc + t * m.mz);
     }
     __syncthreads();
     const int64_t rbase = (int64_t)row * m.mz;
@@ -194,10 +192,12 @@ __global__ void __launch_bounds__(kBlock) paint_rows_kernel(const float *__restr
 // perm != NULL: particle i of the (sorted) inputs is written to row perm[i].
 // Blocks are XCD-remapped so that consecutive (cell-sorted) particle ranges,
 // which read the same grid rows, share one L2.
+// Particles come as binned records (x, y, z, type bits) or, with rec == NULL,
+// as plain xyz rows of type 0.
 template <int NCOMP>
 __global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict__ grids,
-                                                          const float *__restrict__ x,
-                                                          const int32_t *__restrict__ types,
+                                                          const float4 *__restrict__ rec,
+                                                          const float *__restrict__ xyz,
                                                           const float *__restrict__ scale_per_particle,
                                                           int64_t n, int n_types, Mesh m,
                                                           float *__restrict__ out, const int *__restrict__ perm,
@@ -206,16 +206,17 @@ __global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict
     const int64_t first = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (n_dev) n = min(n, (int64_t)*n_dev);  // particles kept by the binning
     for (int64_t i = first; i < n; i += stride) {
-        const int t = types ? types[i] : 0;
+        const float4 r = rec ? rec[i] : make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.f);
+        const int t = __float_as_int(r.w);
         const int64_t o = perm ? (int64_t)perm[i] : i;
         if ((unsigned)t >= (unsigned)n_types) {
 #pragma unroll
             for (int c = 0; c < NCOMP; ++c) out[NCOMP * o + c] = 0.f;
             continue;
         }
-        const CicAxis cx = cic_axis((double)x[3 * i + 0], m.inv_h[0], m.mx);
-        const CicAxis cy = cic_axis((double)x[3 * i + 1], m.inv_h[1], m.my);
-        const CicAxis cz = cic_axis((double)x[3 * i + 2], m.inv_h[2], m.mz);
+        const CicAxis cx = cic_axis((double)r.x, m.inv_h[0], m.mx);
+        const CicAxis cy = cic_axis((double)r.y, m.inv_h[1], m.my);
+        const CicAxis cz = cic_axis((double)r.z, m.inv_h[2], m.mz);
         const float *gt = grids + (int64_t)t * NCOMP * m.g;
         float v[8][NCOMP];
 #pragma unroll
@@ -352,7 +353,7 @@ __global__ void __launch_bounds__(kBlock) field_xline_kernel(const float2 *__res
             const int idx = i0 + j * kBlock;
             v[j] = make_float2(0.f, 0.f);
             if (idx < nel) {
-                const int l = idx & (W - 1), te = idx >> wshift, t = te / n, e = te - t * n;
+                const int l = idx & (W - 1), te = idx >> wshift, t = fdiv(te, ax.ndiv), e = te - t * n;
                 if (l < wv) v[j] = spec[(int64_t)t * g.gc + (int64_t)e * plane + line0 + l];
             }
         }
@@ -360,7 +361,7 @@ __global__ void __launch_bounds__(kBlock) field_xline_kernel(const float2 *__res
         for (int j = 0; j < kLoadBatch; ++j) {
             const int idx = i0 + j * kBlock;
             if (idx < nel) {
-                const int l = idx & (W - 1), te = idx >> wshift, t = te / n, e = te - t * n;
+                const int l = idx & (W - 1), te = idx >> wshift, t = fdiv(te, ax.ndiv), e = te - t * n;
                 A[(t * W + l) * ax.ld + e] = v[j];
             }
         }
@@ -396,7 +397,9 @@ __global__ void __launch_bounds__(kBlock) field_xline_kernel(const float2 *__res
             }
             en *= wgt;
             if (kk == 0) {
-                for (int t = 0; t < T; ++t) en += p.b[t] * (double)ph[t].x;
+#pragma unroll
+                for (int t = 0; t < MAXT; ++t)
+                    if (t < T) en += p.b[t] * (double)ph[t].x;
                 en += p.c0;
             }
             e_acc[0] += en;
@@ -423,6 +426,140 @@ __global__ void __launch_bounds__(kBlock) field_xline_kernel(const float2 *__res
     if (partials) block_reduce_store<1>(e_acc, partials + blockIdx.x);
 }
 
+// Forward x pass, k-space pass and the inverse x passes in one workgroup.
+// k_y and k_z are constant along an x line, so the x-inverse of -i k_d V_t
+// for d = y, z is -i k_d times the x-inverse of V_t: per line and type only
+// two inverse transforms are needed, IFFT_x(-i k_x V_t) -> out[t] and
+// IFFT_x(V_t) -> out[T + t]; fft_c2r_yz_grad applies -i k_y / -i k_z in its
+// y-z plane pass.  The T spectra are transformed in place in one LDS buffer
+// of T W lines; V_t waits in registers (PMAX positions per thread) while the
+// x-gradient family is transformed and stored.
+template <int MAXT, int NT, int ELEMS>
+__global__ void __launch_bounds__(NT) field_xline_grad_kernel(const float2 *__restrict__ spec,
+                                                              float2 *__restrict__ out, KGeom g, FieldK p,
+                                                              FftAxis ax, const float2 *__restrict__ tw_glob,
+                                                              int wshift, FastDiv flines,
+                                                              double *__restrict__ partials) {
+    constexpr int PMAX = 16 / MAXT;  // W n <= NT PMAX (checked on the host)
+    extern __shared__ float2 smem[];
+    const int n = ax.n;  // Mx
+    const int W = 1 << wshift;
+    const int T = p.T;
+    const int ld = ax.ld;
+    float2 *tw = smem;
+    float2 *D = tw + n;  // line (t W + l) at D + (t W + l) ld
+    const int plane = g.my * g.zh;
+    const int line0 = (int)xcd_block(blockIdx.x, gridDim.x) * W;
+    const int wv = min(W, plane - line0);
+    for (int i = threadIdx.x; i < n - 1; i += NT) tw[i] = tw_glob[i];
+    const int nel = T * W * n;
+    for (int i0 = threadIdx.x; i0 < nel; i0 += NT * kLoadBatch) {
+        float2 v[kLoadBatch];
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * NT;
+            v[j] = make_float2(0.f, 0.f);
+            if (idx < nel) {
+                const int l = idx & (W - 1), te = idx >> wshift, t = fdiv(te, ax.ndiv), e = te - t * n;
+                if (l < wv) v[j] = spec[(int64_t)t * g.gc + (int64_t)e * plane + line0 + l];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * NT;
+            if (idx < nel) {
+                const int l = idx & (W - 1), te = idx >> wshift, t = fdiv(te, ax.ndiv), e = te - t * n;
+                D[(t * W + l) * ld + e] = v[j];
+            }
+        }
+    }
+    __syncthreads();
+    fft_inplace<ELEMS>(D, 1, ld, T * W, flines, ax, tw, -1.f);
+    // k-space pass (the arithmetic of field_kspace_kernel): V_t kept in
+    // registers, -i k_x V_t written back in place
+    double e_acc[1] = {0.0};
+    float2 hold[PMAX][MAXT];
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j) {
+        const int pos = threadIdx.x + j * NT;
+        const int l = pos & (W - 1), e = pos >> wshift;
+        if (pos >= W * n || l >= wv) {
+#pragma unroll
+            for (int t = 0; t < MAXT; ++t) hold[j][t] = make_float2(0.f, 0.f);
+            continue;
+        }
+        const int64_t kk = (int64_t)e * plane + line0 + l;
+        double k[3], kd[3], wgt;
+        kvec(g, kk, k, kd, wgt);
+        const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        const float H = (float)exp(-0.5 * p.sigma2 * k2);
+        const float hn = (float)((double)H * p.inv_g);
+        float2 ph[MAXT];
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t) ph[t] = t < T ? cscale(D[(t * W + l) * ld + e], hn) : make_float2(0.f, 0.f);
+        if (p.want_energy) {
+            double en = 0.0;
+#pragma unroll
+            for (int t = 0; t < MAXT; ++t) {
+                if (t >= T) continue;
+#pragma unroll
+                for (int u = 0; u < MAXT; ++u) {
+                    if (u >= T) continue;
+                    en += 0.5 * p.Ad[t * HYMD_MAX_TYPES + u] *
+                          ((double)ph[t].x * (double)ph[u].x + (double)ph[t].y * (double)ph[u].y);
+                }
+            }
+            en *= wgt;
+            if (kk == 0) {
+#pragma unroll
+                for (int t = 0; t < MAXT; ++t)
+                    if (t < T) en += p.b[t] * (double)ph[t].x;
+                en += p.c0;
+            }
+            e_acc[0] += en;
+        }
+        const float kx = (float)kd[0];
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t) {
+            float2 v = make_float2(0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < MAXT; ++u)
+                if (u < T) {
+                    const float a = (float)p.Ad[t * HYMD_MAX_TYPES + u];
+                    v.x += a * ph[u].x;
+                    v.y += a * ph[u].y;
+                }
+            v = cscale(v, H);
+            if (kk == 0) v.x += (float)p.b[t];
+            hold[j][t] = v;
+            if (t < T) D[(t * W + l) * ld + e] = make_float2(kx * v.y, -kx * v.x);
+        }
+    }
+    __syncthreads();
+    fft_inplace<ELEMS>(D, 1, ld, T * W, flines, ax, tw, 1.f);
+    for (int idx = threadIdx.x; idx < nel; idx += NT) {
+        const int l = idx & (W - 1), te = idx >> wshift, t = fdiv(te, ax.ndiv), e = te - t * n;
+        if (l < wv) out[(int64_t)t * g.gc + (int64_t)e * plane + line0 + l] = D[(t * W + l) * ld + e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PMAX; ++j) {
+        const int pos = threadIdx.x + j * NT;
+        if (pos >= W * n) continue;
+        const int l = pos & (W - 1), e = pos >> wshift;
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t)
+            if (t < T) D[(t * W + l) * ld + e] = hold[j][t];
+    }
+    __syncthreads();
+    fft_inplace<ELEMS>(D, 1, ld, T * W, flines, ax, tw, 1.f);
+    for (int idx = threadIdx.x; idx < nel; idx += NT) {
+        const int l = idx & (W - 1), te = idx >> wshift, t = fdiv(te, ax.ndiv), e = te - t * n;
+        if (l < wv) out[(int64_t)(T + t) * g.gc + (int64_t)e * plane + line0 + l] = D[(t * W + l) * ld + e];
+    }
+    if (partials) block_reduce_store<1, NT>(e_acc, partials + blockIdx.x);
+}
+
 // Spectral moments (P, Q, phi~(0)) of the filtered densities, one partial
 // row per block.
 template <int MAXT>
@@ -441,9 +578,13 @@ struct PmeK {
 
 // PME (field.py:363-397): rho~ = H rho/G; psi = 4 pi elec rho~/|k|^2 (|k|^2 := 1
 // at k = 0); E_d = -i k_d psi.  Energy partials: 1/2 wgt Re(rho^* psi).
+// grad_form: out[0] = -i k_x psi, out[1] = psi (the inputs of the x-inverse
+// and fft_c2r_yz_grad, which applies -i k_y and -i k_z); otherwise out[d] =
+// E_d and, with write_psi, out[3] = psi.
 __global__ void __launch_bounds__(kBlock) pme_kspace_kernel(const float2 *__restrict__ spec,
-                                                             float2 *__restrict__ out, int write_psi, KGeom g,
-                                                             PmeK p, double *__restrict__ partials) {
+                                                             float2 *__restrict__ out, int write_psi,
+                                                             int grad_form, KGeom g, PmeK p,
+                                                             double *__restrict__ partials) {
     double e_acc[1] = {0.0};
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; kk < g.gc; kk += stride) {
@@ -458,6 +599,10 @@ __global__ void __launch_bounds__(kBlock) pme_kspace_kernel(const float2 *__rest
         const double px = f * rx, py = f * ry;                                // psi
         e_acc[0] += 0.5 * wgt * (rx * px + ry * py);
         out[kk] = make_float2((float)(kd[0] * py), (float)(-kd[0] * px));
+        if (grad_form) {
+            out[g.gc + kk] = make_float2((float)px, (float)py);
+            continue;
+        }
         out[g.gc + kk] = make_float2((float)(kd[1] * py), (float)(-kd[1] * px));
         out[2 * g.gc + kk] = make_float2((float)(kd[2] * py), (float)(-kd[2] * px));
         if (write_psi) out[3 * g.gc + kk] = make_float2((float)px, (float)py);
@@ -531,7 +676,7 @@ struct hymd_pm {
     int kblocks;      // blocks of the k-space passes
     DevBuf mpart;     // moments partial rows
     // spatial sort of the last painted particle set (row-binned order)
-    DevBuf keys, perm, xs, ts, ws;  // per particle
+    DevBuf keys, perm, xs, ws;  // per particle (xs: binned records x, y, z, type bits)
     DevBuf counts, start, tiles;  // per mesh row (counts kept zero between sorts)
     int64_t sorted_n = 0;
     bool sorted_weights = false;
@@ -568,6 +713,46 @@ static int launch_xline(hymd_pm *pm, const float2 *spec, float2 *grad, const KGe
     while ((1 << wshift) < W) ++wshift;
     field_xline_kernel<MAXT><<<nb, kBlock, lds, pm->stream>>>(spec, grad, g, p, f.ax, (const float2 *)f.tw_x.ptr,
                                                               wshift, partials);
+    HYMD_HIP(hipGetLastError());
+    *nblocks = nb;
+    return HYMD_OK;
+}
+
+constexpr int kXlineGradThreads = 512;
+constexpr int kXlineGradElems = 16;  // values per thread and in-place stage
+
+// Lines per workgroup of field_xline_grad_kernel for T types (0: it does not
+// fit, or the y-z plane pass it feeds is unavailable).
+static int xline_grad_width(const FftPlan &f, int T) {
+    if (!f.lds_plane) return 0;
+    const int pmax = 16 / moments_maxt(T);
+    for (int W = 16; W >= 2; W >>= 1) {
+        if ((int64_t)W * f.ax.n > (int64_t)kXlineGradThreads * pmax) continue;
+        if (sizeof(float2) * ((size_t)f.ax.n + (size_t)T * W * f.ax.ld) > kXlineLdsBytes) continue;
+        if (!fft_inplace_fits(f.ax, T * W, kXlineGradThreads, kXlineGradElems)) continue;
+        return W;
+    }
+    return 0;
+}
+
+template <int MAXT>
+static int launch_xline_grad(hymd_pm *pm, const float2 *spec, float2 *out, const KGeom &g, const FieldK &p, int W,
+                             double *partials, int *nblocks) {
+    const FftPlan &f = pm->fft;
+    const int plane = g.my * g.zh;
+    const int nb = (plane + W - 1) / W;
+    const size_t lds = sizeof(float2) * ((size_t)f.ax.n + (size_t)p.T * W * f.ax.ld);
+    static bool attr_set = false;  // the opt-in is per kernel and process
+    if (!attr_set && lds > 64 * 1024) {
+        HYMD_HIP(hipFuncSetAttribute(
+            reinterpret_cast<const void *>(&field_xline_grad_kernel<MAXT, kXlineGradThreads, kXlineGradElems>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kXlineLdsBytes));
+        attr_set = true;
+    }
+    int wshift = 0;
+    while ((1 << wshift) < W) ++wshift;
+    field_xline_grad_kernel<MAXT, kXlineGradThreads, kXlineGradElems><<<nb, kXlineGradThreads, lds, pm->stream>>>(
+        spec, out, g, p, f.ax, (const float2 *)f.tw_x.ptr, wshift, make_fastdiv((uint32_t)(p.T * W)), partials);
     HYMD_HIP(hipGetLastError());
     *nblocks = nb;
     return HYMD_OK;
@@ -702,8 +887,7 @@ static int sort_particles(hymd_pm *pm, const float *x, const int32_t *types, con
     const size_t nn = (size_t)std::max<int64_t>(n, 1);
     HYMD_TRY(pm->keys.reserve(sizeof(int2) * nn));
     HYMD_TRY(pm->perm.reserve(sizeof(int) * nn));
-    HYMD_TRY(pm->xs.reserve(sizeof(float) * 3 * nn));
-    HYMD_TRY(pm->ts.reserve(sizeof(int32_t) * nn));
+    HYMD_TRY(pm->xs.reserve(sizeof(float4) * nn));
     if (w) HYMD_TRY(pm->ws.reserve(sizeof(float) * nn));
     if (n > 0 && !keys_ready)
         row_key_kernel<<<grid_for(n), kBlock, 0, s>>>(x, w, n, m, (int2 *)pm->keys.ptr, (int *)pm->counts.ptr);
@@ -711,8 +895,7 @@ static int sort_particles(hymd_pm *pm, const float *x, const int32_t *types, con
     HYMD_TRY(exclusive_scan((int *)pm->counts.ptr, (int *)pm->start.ptr, nrows, (int *)pm->tiles.ptr, nullptr,
                             true, s));
     HYMD_TRY(launch_key_scatter(x, types, w, n, (const int2 *)pm->keys.ptr, (const int *)pm->start.ptr,
-                                (int *)pm->perm.ptr, (float *)pm->xs.ptr, (int32_t *)pm->ts.ptr,
-                                w ? (float *)pm->ws.ptr : nullptr, s));
+                                (int *)pm->perm.ptr, (float4 *)pm->xs.ptr, w ? (float *)pm->ws.ptr : nullptr, s));
     pm->sorted_n = n;
     pm->sorted_weights = (w != nullptr);
     return HYMD_OK;
@@ -730,8 +913,7 @@ static int paint_sorted(hymd_pm *pm, int T, const double *mass, float *out, long
     const int tc = std::max(1, std::min(T, (int)(kRowLdsBytes / (sizeof(long long) * m.mz))));
     dim3 grid((unsigned)(m.mx * m.my), (unsigned)((T + tc - 1) / tc));
     paint_rows_kernel<<<grid, kBlock, sizeof(long long) * tc * m.mz, pm->stream>>>(
-        (const float *)pm->xs.ptr, (const int32_t *)pm->ts.ptr,
-        pm->sorted_weights ? (const float *)pm->ws.ptr : nullptr, (const int *)pm->start.ptr, m, pa, tc, out_scale,
+        (const float4 *)pm->xs.ptr, pm->sorted_weights ? (const float *)pm->ws.ptr : nullptr, (const int *)pm->start.ptr, m, pa, tc, out_scale,
         out, out_fixed);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
@@ -742,8 +924,7 @@ static int paint_sorted(hymd_pm *pm, int T, const double *mass, float *out, long
 static int readout_sorted(hymd_pm *pm, int ncomp, int n_types, const float *grids, float *out, bool weighted) {
     const int64_t n = pm->sorted_n;
     if (n > 0) {
-        const float *xs = (const float *)pm->xs.ptr;
-        const int32_t *ts = (const int32_t *)pm->ts.ptr;
+        const float4 *rec = (const float4 *)pm->xs.ptr;
         const float *ws = weighted ? (const float *)pm->ws.ptr : nullptr;
         const int *pp = (const int *)pm->perm.ptr;
         // a weighted binning leaves zero-weight particles out: their rows get
@@ -754,11 +935,11 @@ static int readout_sorted(hymd_pm *pm, int ncomp, int n_types, const float *grid
             kept = (const int *)pm->start.ptr + (int64_t)pm->mesh.mx * pm->mesh.my;
         }
         if (ncomp == 3)
-            readout_kernel<3><<<grid_for(n), kBlock, 0, pm->stream>>>(grids, xs, ts, ws, n, n_types, pm->mesh, out, pp,
-                                                                      kept);
+            readout_kernel<3><<<grid_for(n), kBlock, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh,
+                                                                      out, pp, kept);
         else
-            readout_kernel<1><<<grid_for(n), kBlock, 0, pm->stream>>>(grids, xs, ts, ws, n, n_types, pm->mesh, out, pp,
-                                                                      kept);
+            readout_kernel<1><<<grid_for(n), kBlock, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh,
+                                                                      out, pp, kept);
     }
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
@@ -825,7 +1006,7 @@ extern "C" int hymd_fft_c2r(hymd_pm *pm, const void *in, float *out, int32_t bat
 extern "C" int hymd_readout(hymd_pm *pm, const float *grid, const float *x, int64_t n, float *out) {
     HYMD_REQUIRE(pm && grid && out && (x || n == 0) && n >= 0, "bad argument");
     if (n > 0)
-        readout_kernel<1><<<grid_for(n), kBlock, 0, pm->stream>>>(grid, x, nullptr, nullptr, n, 1, pm->mesh, out,
+        readout_kernel<1><<<grid_for(n), kBlock, 0, pm->stream>>>(grid, nullptr, x, nullptr, n, 1, pm->mesh, out,
                                                                     nullptr, nullptr);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
@@ -854,6 +1035,25 @@ static int update_field_impl(hymd_pm *pm, const float *x, const int32_t *types, 
     FieldK p = field_params(T, coupling, shift, constant, sigma, m);
     p.want_energy = energy ? 1 : 0;
     KGeom g0 = kgeom(pm);
+    const int W2 = T <= 8 ? xline_grad_width(pm->fft, T) : 0;
+    if (!phi && !phi_fourier && !v_ext && W2 > 0) {
+        // force grids only: z + y forward passes; one pass for the forward x
+        // transform, the k-space pass and the 2T inverse x transforms; the
+        // y-z plane pass writes the 3T force grids
+        HYMD_TRY(fft_r2c_zy(pm->fft, rho, spec, T, s));
+        double *part = energy ? (double *)pm->partials.ptr : nullptr;
+        int nb = 0;
+        if (T <= 2) HYMD_TRY(launch_xline_grad<2>(pm, spec, grad, g0, p, W2, part, &nb));
+        else if (T <= 4) HYMD_TRY(launch_xline_grad<4>(pm, spec, grad, g0, p, W2, part, &nb));
+        else HYMD_TRY(launch_xline_grad<8>(pm, spec, grad, g0, p, W2, part, &nb));
+        HYMD_TRY(fft_c2r_yz_grad(pm->fft, grad, force_mesh, T, g0.kfac[1], g0.kfac[2], 1.f, s));
+        if (energy) {
+            double e = 0.0;
+            HYMD_TRY(read_result(pm, nb, 1, &e));
+            *energy = e * (m.box[0] * m.box[1] * m.box[2]);
+        }
+        return HYMD_OK;
+    }
     const int W = xline_width(pm->fft, T);
     if (!phi && !phi_fourier && !v_ext && W > 0) {
         // force grids only: z + y forward passes, the forward x pass fused
@@ -943,6 +1143,35 @@ int update_field_keyed(hymd_pm *pm, const float *x, const int32_t *types, int64_
 }
 }  // namespace hymd
 
+__global__ void __launch_bounds__(kBlock) row_order_kernel(const int2 *__restrict__ keys, int64_t n,
+                                                           const int *__restrict__ start, int *__restrict__ order) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int2 kr = keys[i];
+        order[start[kr.x] + kr.y] = (int)i;
+    }
+}
+
+namespace hymd {
+// order[slot] = i: the n particles at x (float32 rows) in mesh-row order (the
+// binning of update_field, without its records).  Uses the handle's binning
+// buffers, so it runs between field updates.
+int row_order(hymd_pm *pm, const float *x, int64_t n, int *order, hipStream_t s) {
+    HYMD_REQUIRE(pm && (x || n == 0) && order && n >= 0 && n < (int64_t(1) << 31), "bad argument");
+    if (n == 0) return HYMD_OK;
+    const Mesh &m = pm->mesh;
+    HYMD_TRY(pm->keys.reserve(sizeof(int2) * (size_t)n));
+    row_key_kernel<<<grid_for(n), kBlock, 0, s>>>(x, nullptr, n, m, (int2 *)pm->keys.ptr, (int *)pm->counts.ptr);
+    HYMD_HIP(hipGetLastError());
+    HYMD_TRY(exclusive_scan((int *)pm->counts.ptr, (int *)pm->start.ptr, (int64_t)m.mx * m.my, (int *)pm->tiles.ptr,
+                            nullptr, true, s));
+    row_order_kernel<<<grid_for(n), kBlock, 0, s>>>((const int2 *)pm->keys.ptr, n, (const int *)pm->start.ptr, order);
+    HYMD_HIP(hipGetLastError());
+    pm->sorted_n = 0;  // the last binned set is gone
+    return HYMD_OK;
+}
+}  // namespace hymd
+
 // paint + r2c + moments (and, if force_mesh != NULL, the force grids of the
 // quadratic Hamiltonian (coupling, shift) as in hymd_update_field)
 extern "C" int hymd_field_moments(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
@@ -1022,16 +1251,24 @@ extern "C" int hymd_update_field_force_q(hymd_pm *pm, const float *x, const floa
     p.four_pi_elec = 4.0 * M_PI * elec_factor;
     KGeom g = kgeom(pm);
     double *partials = energy_q ? (double *)pm->partials.ptr : nullptr;
-    pme_kspace_kernel<<<pm->kblocks, kBlock, 0, s>>>(spec, out, psi ? 1 : 0, g, p, partials);
+    const int grad_form = pm->fft.lds_plane ? 1 : 0;
+    pme_kspace_kernel<<<pm->kblocks, kBlock, 0, s>>>(spec, out, psi ? 1 : 0, grad_form, g, p, partials);
     HYMD_HIP(hipGetLastError());
     float *field = elec_field;
     if (!field) {
         HYMD_TRY(pm->rscratch.reserve(sizeof(float) * 3 * m.g));
         field = (float *)pm->rscratch.ptr;
     }
-    HYMD_TRY(fft_inverse_x(pm->fft, out, psi ? 4 : 3, s));
-    HYMD_TRY(fft_c2r_yz(pm->fft, out, field, 3, 1.f, s));
-    if (psi) HYMD_TRY(fft_c2r_yz(pm->fft, out + 3 * m.gc, psi, 1, 1.f, s));
+    if (grad_form) {
+        // x-inverse of -i k_x psi and psi; the plane pass forms E_y, E_z
+        HYMD_TRY(fft_inverse_x(pm->fft, out, 2, s));
+        HYMD_TRY(fft_c2r_yz_grad(pm->fft, out, field, 1, g.kfac[1], g.kfac[2], 1.f, s));
+        if (psi) HYMD_TRY(fft_c2r_yz(pm->fft, out + m.gc, psi, 1, 1.f, s));
+    } else {
+        HYMD_TRY(fft_inverse_x(pm->fft, out, psi ? 4 : 3, s));
+        HYMD_TRY(fft_c2r_yz(pm->fft, out, field, 3, 1.f, s));
+        if (psi) HYMD_TRY(fft_c2r_yz(pm->fft, out + 3 * m.gc, psi, 1, 1.f, s));
+    }
     HYMD_TRY(readout_sorted(pm, 3, 1, field, elec_forces, true));
     if (energy_q) {
         double e = 0.0;

@@ -11,8 +11,11 @@ namespace hymd {
 // among the particles of that key), start = exclusive scan of the key
 // counts; perm[pos] = i, xs/ts/ws the sorted positions, types (0 when
 // types == NULL) and weights (if ws != NULL).
+// Particle i (key (row, rank), key.x < 0: left out) goes to slot start[row] +
+// rank: perm[slot] = i, rec[slot] = (x, y, z, type bits; type 0 without
+// types), ws[slot] = w[i] when ws is given.
 int launch_key_scatter(const float *x, const int32_t *types, const float *w, int64_t n, const int2 *keys,
-                       const int *start, int *perm, float *xs, int32_t *ts, float *ws, hipStream_t s);
+                       const int *start, int *perm, float4 *rec, float *ws, hipStream_t s);
 
 struct PaintArgs {
     int n_types;
@@ -26,13 +29,13 @@ struct PaintArgs {
 // off); the 8th corner (1, 1, 1) takes the remainder, so every particle
 // deposits exactly round(m 2^32).  acc_row: the row's int64 accumulators in
 // LDS (integer adds: order-independent).
-__device__ __forceinline__ void row_deposit(const float *__restrict__ xp, int a, int b, double mass, double scale,
+__device__ __forceinline__ void row_deposit(const float4 &xp, int a, int b, double mass, double scale,
                                             const double inv_h[3], int mx, int my, int mz,
                                             unsigned long long *acc_row) {
 #pragma clang fp contract(off)
-    const CicAxis cx = cic_axis((double)xp[0], inv_h[0], mx);
-    const CicAxis cy = cic_axis((double)xp[1], inv_h[1], my);
-    const CicAxis cz = cic_axis((double)xp[2], inv_h[2], mz);
+    const CicAxis cx = cic_axis((double)xp.x, inv_h[0], mx);
+    const CicAxis cy = cic_axis((double)xp.y, inv_h[1], my);
+    const CicAxis cz = cic_axis((double)xp.z, inv_h[2], mz);
     const double wxy = (a ? cx.w1 : cx.w0) * (b ? cy.w1 : cy.w0);
     const long long q0 = __double2ll_rn(((wxy * cz.w0) * mass) * scale);
     long long q1;

@@ -267,6 +267,7 @@ struct hymd_fused {
     int64_t n = 0;
     int nblocks = 0, ppt = 1;
     int max_bonds = 0, max_angles = 0, max_entries = 0, max_particles = 0;
+    int64_t single_tail = 0;  // rows [single_tail, n) are molecules of one particle (no terms)
     size_t lds = 0;
     DevBuf blk_start, blk_bond_off, bond_ab, bond_par, blk_ang_off, ang_abc, ang_par, csr_off, csr_ent;
     DevBuf partials, result;
@@ -308,6 +309,10 @@ int download(std::vector<T> &h, const void *d, int64_t n) {
 // Builds the blocks from device term arrays.  Returns HYMD_ERR_UNSUPPORTED
 // (with a message) when a molecule is not a contiguous index range or does
 // not fit one workgroup; the caller then keeps the unfused kernels.
+namespace hymd {
+int64_t fused_single_tail(const hymd_fused *f) { return f ? f->single_tail : 0; }
+}  // namespace hymd
+
 extern "C" int hymd_fused_create(hymd_fused **out, int64_t n, int64_t m2, const int32_t *a2, const int32_t *b2,
                                  const double *r0, const double *k2, int64_t m3, const int32_t *a3,
                                  const int32_t *b3, const int32_t *c3, const double *t0, const double *k3) {
@@ -447,6 +452,8 @@ extern "C" int hymd_fused_create(hymd_fused **out, int64_t n, int64_t m2, const 
     for (int b = 0; b < nblk; ++b) max_e = std::max(max_e, off[blk_start[b + 1]] - off[blk_start[b]]);
     hymd_fused *f = new hymd_fused();
     f->n = n;
+    f->single_tail = n;
+    while (f->single_tail > 0 && size[uf.find((int)(f->single_tail - 1))] == 1) --f->single_tail;
     f->nblocks = nblk;
     f->ppt = ppt;
     f->max_bonds = std::max(max_b, 1);

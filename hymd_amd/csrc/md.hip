@@ -42,11 +42,110 @@ struct hymd_md {
     float *f_slow = nullptr;  // caller's: summed slow forces
     DevBuf f_elec, f_ba, f_dih, dipoles, tmat, f_dip, dip32, f_dip64, f_recon;
     double extra_e[2] = {0.0, 0.0};  // dihedral, field_q of the last energies step
+    // periodic reorder of the one-particle molecules (fused path): every
+    // reorder_every outer steps the rows [single0, n) are put in the mesh-row
+    // order of their current positions, so the binning scatter and the
+    // readout's writes stay local as the solvent diffuses
+    int reorder_every = 0, since_reorder = 0;
+    int64_t single0 = 0;
+    int32_t *types_rw = nullptr;
+    int64_t *order = nullptr, *where = nullptr;  // caller's: state row -> input row, input row -> state row
+    DevBuf rperm, rtmp;
     // optional HIP-event timing of one stage in eager steps
     int timed = 0;  // 0 none, 1 bonded forces, 2 kick+drift, 3 update_field, 4 readout
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;
 };
+
+// rows s0 + j <- rows s0 + perm[j] (j < ns) of the fused-path state, through
+// tmp: gather, then store back and point where[] at the new rows.
+struct ReorderArrays {
+    double *x, *v;
+    float *x32, *ff, *fs;
+    int32_t *types;
+    int64_t *order, *where;
+};
+
+// One row of the reordered fields, loaded whole before it is stored (the
+// arrays are distinct, but the compiler cannot know it: loads first keep all
+// of a row's loads in flight together).
+struct ReorderRow {
+    double x[3], v[3];
+    float x32[3], ff[3], fs[3];
+    int32_t type;
+    int64_t order;
+};
+
+__device__ __forceinline__ ReorderRow load_row(const ReorderArrays &a, int64_t i) {
+    ReorderRow r;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        r.x[d] = a.x[3 * i + d];
+        r.v[d] = a.v[3 * i + d];
+        r.x32[d] = a.x32[3 * i + d];
+        r.ff[d] = a.ff[3 * i + d];
+        r.fs[d] = a.fs[3 * i + d];
+    }
+    r.type = a.types[i];
+    r.order = a.order[i];
+    return r;
+}
+
+__device__ __forceinline__ void store_row(const ReorderArrays &a, int64_t i, const ReorderRow &r) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        a.x[3 * i + d] = r.x[d];
+        a.v[3 * i + d] = r.v[d];
+        a.x32[3 * i + d] = r.x32[d];
+        a.ff[3 * i + d] = r.ff[d];
+        a.fs[3 * i + d] = r.fs[d];
+    }
+    a.types[i] = r.type;
+    a.order[i] = r.order;
+}
+
+__global__ void __launch_bounds__(kBlock) reorder_gather_kernel(int64_t ns, int64_t s0, const int *__restrict__ perm,
+                                                                ReorderArrays a, ReorderArrays t) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < ns; j += stride)
+        store_row(t, j, load_row(a, s0 + perm[j]));
+}
+
+__global__ void __launch_bounds__(kBlock) reorder_store_kernel(int64_t ns, int64_t s0, ReorderArrays t,
+                                                               ReorderArrays a) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < ns; j += stride) {
+        const ReorderRow r = load_row(t, j);
+        store_row(a, s0 + j, r);
+        a.where[r.order] = s0 + j;
+    }
+}
+
+static int reorder_singles(hymd_md *m) {
+    const int64_t s0 = m->single0, ns = m->n - s0;
+    if (ns <= 1) return HYMD_OK;
+    hipStream_t s = m->work;
+    HYMD_TRY(m->rperm.reserve(sizeof(int) * ns));
+    HYMD_TRY(m->rtmp.reserve((sizeof(double) * 6 + sizeof(float) * 9 + sizeof(int32_t) + sizeof(int64_t)) * ns));
+    HYMD_TRY(hymd_pm_set_stream(m->pm, s));
+    HYMD_TRY(row_order(m->pm, m->x32 + 3 * s0, ns, (int *)m->rperm.ptr, s));
+    char *p = (char *)m->rtmp.ptr;
+    ReorderArrays t;
+    t.x = (double *)p;
+    t.v = t.x + 3 * ns;
+    t.order = (int64_t *)(t.v + 3 * ns);
+    t.x32 = (float *)(t.order + ns);
+    t.ff = t.x32 + 3 * ns;
+    t.fs = t.ff + 3 * ns;
+    t.types = (int32_t *)(t.fs + 3 * ns);
+    t.where = nullptr;
+    ReorderArrays a{m->x, m->v, m->x32, m->f_field, m->f_fast, m->types_rw, m->order, m->where};
+    reorder_gather_kernel<<<grid_for(ns), kBlock, 0, s>>>(ns, s0, (const int *)m->rperm.ptr, a, t);
+    HYMD_HIP(hipGetLastError());
+    reorder_store_kernel<<<grid_for(ns), kBlock, 0, s>>>(ns, s0, t, a);
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
+}
 
 static void mark(hymd_md *m, int stage) {
     if (m->timed == stage && m->ev_used < m->ev.size()) (void)hipEventRecord(m->ev[m->ev_used++], m->work);
@@ -269,6 +368,10 @@ extern "C" int hymd_md_step(hymd_md *m, int32_t n_steps, int32_t use_graph, doub
             HYMD_TRY(enqueue_step(m, nullptr, m->pending));
         }
         if (m->fused) m->pending = 1;
+        if (m->fused && m->reorder_every > 0 && ++m->since_reorder >= m->reorder_every) {
+            HYMD_TRY(reorder_singles(m));
+            m->since_reorder = 0;
+        }
     }
     if (energies && n_steps > 0) {
         std::memset(energies, 0, 5 * sizeof(double));
@@ -310,6 +413,34 @@ extern "C" int hymd_md_set_fused(hymd_md *m, int32_t enable) {
 }
 
 extern "C" int hymd_md_is_fused(hymd_md *m) { return (m && m->fused) ? 1 : 0; }
+
+// Every `every` outer steps (0: never) the trailing one-particle molecules
+// are put in the mesh-row order of their current positions.  types (the
+// array the handle was created with), order (state row -> input row) and
+// where (input row -> state row) are the caller's int arrays, permuted in
+// place with the state.  Needs the fused path.
+extern "C" int hymd_md_set_reorder(hymd_md *m, int32_t every, int32_t *types, int64_t *order, int64_t *where) {
+    HYMD_REQUIRE(m && every >= 0, "bad argument");
+    if (every == 0) {
+        m->reorder_every = 0;
+        return HYMD_OK;
+    }
+    if (!m->fused) {
+        set_error("reordering needs the fused inner loop (one-particle molecules without terms)");
+        return HYMD_ERR_UNSUPPORTED;
+    }
+    HYMD_REQUIRE(types == m->types && order && where, "types must be the handle's array; order and where non-null");
+    m->types_rw = types;
+    m->order = order;
+    m->where = where;
+    m->single0 = fused_single_tail(m->fused);
+    m->reorder_every = every;
+    m->since_reorder = 0;
+    return HYMD_OK;
+}
+
+// First row of the reordered one-particle tail (n when there is none).
+extern "C" int64_t hymd_md_single_tail(hymd_md *m) { return (m && m->fused) ? fused_single_tail(m->fused) : 0; }
 
 // Time one stage of the eager steps that follow with HIP events recorded on
 // the launch stream around every launch of that stage (graph replays are not

@@ -27,7 +27,7 @@ import ctypes
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, layout
 from ._lib import HymdError, call, darr, dptr
 from .field import ParticleMesh
 from .force import BondedTopology, DihedralTopology
@@ -37,10 +37,30 @@ COULOMB_CONSTANT = 138.935458  # input_parser.py:160
 
 
 class RespaMD:
-    def __init__(self, system, stream=None):
+    """rRESPA MD of `system` on the device.
+
+    With ``spatial_order`` (default) the state arrays hold the particles in
+    the spatial layout of ``hymd_amd.layout`` (whole molecules ordered by mesh
+    cell, one-particle molecules last); ``order`` maps state rows to input
+    rows and :meth:`in_input_order` returns a per-particle array in the
+    input's order.  With ``reorder_every`` > 0 (and the fused inner loop) the
+    library puts the one-particle molecules back in mesh-row order every that
+    many outer steps, as they diffuse; ``x``, ``v``, ``types``, the forces and
+    the order maps are permuted in place, the physics is unchanged (the field
+    path's sums are order-independent and those rows carry no bonded terms).
+    """
+
+    def __init__(self, system, stream=None, spatial_order=True, reorder_every=25):
         _lib.require_gpu()
-        s = system
-        self.system = s
+        self.system = system
+        if spatial_order:
+            order = layout.spatial_order(system.positions, system.box, system.mesh, system.bonds,
+                                         system.angles, system.extra.get("dihedrals"))
+            s, inverse = layout.permute_system(system, order)
+        else:
+            order = np.arange(int(system.n))
+            s, inverse = system, order
+        self.state_system = s
         self.n = int(s.n)
         self.T = int(s.n_types)
         self.box = np.asarray(s.box, dtype=np.float64)
@@ -51,6 +71,9 @@ class RespaMD:
         self.A, self.b, self.c0 = hamiltonian_form(s)
         self.type_mass = [1.0] * self.T
         dev = torch.device("cuda", torch.cuda.current_device())
+        self._order = torch.as_tensor(np.asarray(order, dtype=np.int64), device=dev)
+        self._inverse = torch.as_tensor(np.asarray(inverse, dtype=np.int64), device=dev)
+        self.reorder_every = int(reorder_every) if spatial_order else 0
         self.x = torch.as_tensor(np.ascontiguousarray(s.positions, dtype=np.float64), device=dev)
         self.x = torch.remainder(self.x, torch.as_tensor(self.box, device=dev)).contiguous()
         self.v = torch.as_tensor(np.ascontiguousarray(s.velocities, dtype=np.float64), device=dev)
@@ -69,6 +92,15 @@ class RespaMD:
         self.fast_forces(dipole_flag=1)
         self.slow_forces()
         self._native = None
+
+    @property
+    def order(self):
+        """State row -> input row (numpy), as of the last step."""
+        return self._order.cpu().numpy()
+
+    def in_input_order(self, t):
+        """Rows of a per-particle state array (x, v, forces) in the input's order."""
+        return t[self._inverse]
 
     # ------------------------------------------------------------------
     def _setup_peptide_terms(self, s, dev):
@@ -117,6 +149,9 @@ class RespaMD:
                  dptr(self.force_mesh), darr(np.ravel(self.A)), darr(self.b), float(self.c0),
                  darr(self.type_mass), self.sigma, self.mass, self.dt, self.n_inner, self._box_c)
             self._native = h
+            if self.reorder_every > 0 and _lib.lib().hymd_md_is_fused(h):
+                call("hymd_md_set_reorder", h, self.reorder_every, dptr(self.types, torch.int32, "types"),
+                     dptr(self._order, torch.int64, "order"), dptr(self._inverse, torch.int64, "where"))
             if self.extra_terms:
                 call("hymd_md_set_peptide", h, dptr(self.charges), self.elec if self.charges is not None else 0.0,
                      self.e_self if self.charges is not None else 0.0,

@@ -275,6 +275,18 @@ int hymd_md_destroy(hymd_md *md);
 int hymd_md_set_fused(hymd_md *md, int32_t enable);
 /* 1 if the fused inner loop is active */
 int hymd_md_is_fused(hymd_md *md);
+/* Every `every` outer steps (0: never), put the trailing one-particle
+ * molecules (rows [hymd_md_single_tail, n), which carry no bonded terms) in
+ * the mesh-row order of their current positions, so that the field path's
+ * binning writes and force writes stay local as they diffuse.  types must be
+ * the array given to hymd_md_create; order (state row -> input row) and where
+ * (input row -> state row) are the caller's int64 device arrays; all three
+ * are permuted in place with x, v, x32 and the forces.  Fused path only
+ * (HYMD_ERR_UNSUPPORTED otherwise).  No counterpart in the reference, which
+ * keeps each rank's particles in input order (file_io.py:781-874). */
+int hymd_md_set_reorder(hymd_md *md, int32_t every, int32_t *types, int64_t *order, int64_t *where);
+/* first row of the one-particle tail that hymd_md_set_reorder permutes */
+int64_t hymd_md_single_tail(hymd_md *md);
 /* HIP-event timing of one stage (1 bonded forces, 2 kick+drift, 3 update_field,
  * 4 readout; 0 off) in the eager steps that follow, recorded on the launch
  * stream around each launch; at most max_launches are recorded */

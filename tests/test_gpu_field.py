@@ -220,12 +220,15 @@ def test_drop_in_field_surface(hip):
         assert rms_rel(phi[t].cpu().numpy(), ref["phi"][t]) < 1e-6
 
 
-@pytest.mark.parametrize("mesh,T", [((32, 48, 40), 3), ((24, 24, 24), 2), ((16, 20, 12), 5)])
+@pytest.mark.parametrize("mesh,T", [((32, 48, 40), 3), ((24, 24, 24), 2), ((16, 20, 12), 5), ((48, 40, 64), 4),
+                                    ((128, 96, 80), 6), ((64, 64, 64), 1), ((40, 30, 18), 8)])
 def test_fused_xline_pass_equals_unfused(hip, mesh, T):
-    """The force-grid-only update (forward x pass fused with the k-space pass)
-    against the unfused passes, which run when v_ext is requested.  Same
-    formulas in both; the compiler may contract different products into FMAs
-    in the two kernels, so the grids agree to FP32 rounding, not bit for bit."""
+    """The force-grid-only update (forward x, k-space and inverse x passes in
+    one kernel, -i k_y and -i k_z applied in the y-z plane pass) against the
+    unfused passes (-i k_d V_t formed in k-space, then three full inverse
+    transforms per type), which run when v_ext is requested.  The same
+    operator, applied in a different order, so the grids agree to FP32
+    rounding, not bit for bit."""
     rng = np.random.default_rng(5)
     box = (6.0, 7.5, 5.0)
     n = 6000

@@ -98,16 +98,17 @@ def test_respa_md_matches_oracle_loop(hip, oracle_built):
     A, b, c0 = synth.hamiltonian_form(s)
     ref = OM.State(s, F.Quadratic(A, b, c0))
     md = RespaMD(s)
-    assert rel_err(md.f_field.cpu().numpy(), ref.f_field) < 1e-4
-    assert rel_err(md.f_fast.cpu().numpy(), ref.f_fast) < 1e-5
+    io = md.in_input_order
+    assert rel_err(io(md.f_field).cpu().numpy(), ref.f_field) < 1e-4
+    assert rel_err(io(md.f_fast).cpu().numpy(), ref.f_fast) < 1e-5
     for _ in range(2):
         ref.outer_step()
         md.step()
-    x = md.x.cpu().numpy()
+    x = io(md.x).cpu().numpy()
     dx = x - ref.x
     dx -= s.box * np.round(dx / s.box)
     assert np.abs(dx).max() < 1e-5
-    assert rel_err(md.v.cpu().numpy(), ref.v) < 1e-4
+    assert rel_err(io(md.v).cpu().numpy(), ref.v) < 1e-4
     e = md.step(energies=True)
     ref.outer_step()
     assert e["kinetic"] == pytest.approx(ref.kinetic_energy(), rel=1e-6)
@@ -135,21 +136,22 @@ def test_peptide_md_matches_oracle_loop(hip, oracle_built):
     A, b, c0 = synth.hamiltonian_form(s)
     ref = OM.State(s, F.Quadratic(A, b, c0))
     md = RespaMD(s)
+    io = md.in_input_order
     assert md.protein and md.charges is not None
-    f_slow = md.f_slow.cpu().numpy()
+    f_slow = io(md.f_slow).cpu().numpy()
     assert rms_rel(f_slow, ref.f_field) < 1e-5
     assert rel_err(f_slow, ref.f_field) < 1e-4
-    assert rel_err(md.f_fast.cpu().numpy(), ref.f_fast) < 1e-5
+    assert rel_err(io(md.f_fast).cpu().numpy(), ref.f_fast) < 1e-5
     # the dipole part alone (redistributed dipole PME forces)
-    assert rel_err(md.f_recon.cpu().numpy(), ref.f_slow[2]) < 1e-4
+    assert rel_err(io(md.f_recon).cpu().numpy(), ref.f_slow[2]) < 1e-4
     for _ in range(2):
         ref.outer_step()
         md.step()
-    x = md.x.cpu().numpy()
+    x = io(md.x).cpu().numpy()
     dx = x - ref.x
     dx -= s.box * np.round(dx / s.box)
     assert np.abs(dx).max() < 1e-5
-    assert rel_err(md.v.cpu().numpy(), ref.v) < 1e-4
+    assert rel_err(io(md.v).cpu().numpy(), ref.v) < 1e-4
     np.testing.assert_allclose(md.dipoles.cpu().numpy(), ref.dipoles, atol=1e-6)
     e = md.step(energies=True)
     ref.outer_step()
@@ -276,9 +278,57 @@ def test_fused_rejects_non_contiguous_molecules(hip):
     a, b = s.bonds[0].copy(), s.bonds[1].copy()
     b[0] = 25  # bead 0 bonded into the third chain: a molecule with a gap
     s.bonds = (a, b, s.bonds[2], s.bonds[3])
-    md = RespaMD(s)
+    md = RespaMD(s, spatial_order=False)
     assert not md.fused
     with pytest.raises(HymdError, match="contiguous"):
         md.set_fused(True)
     md.run(2)  # the per-kernel path still runs
+    # the spatial layout keeps every molecule whole and contiguous, so the
+    # same input runs fused, with the same trajectory in input order
+    sp = RespaMD(s)
+    assert sp.fused
+    sp.run(2)
+    torch.cuda.synchronize()
+    dx = sp.in_input_order(sp.x).cpu().numpy() - md.x.cpu().numpy()
+    dx -= s.box * np.round(dx / s.box)
+    assert np.abs(dx).max() < 1e-9
     md.close()
+    sp.close()
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_solvent_reorder_keeps_the_trajectory(hip, use_graph):
+    """Putting the one-particle molecules back in mesh-row order every step
+    changes only the row order of the state: their rows carry no bonded terms
+    and the field path is order-independent (integer paint, per-particle
+    readout), so the trajectory in input order is bit-identical to the run
+    without reordering; the order maps stay inverse permutations and the
+    types follow their particles."""
+    from hymd_amd import _lib, synth
+    from hymd_amd.md import RespaMD
+    s = synth.lipid_water(n_chains=3000, n_water=20000, box=12.0, mesh=32, seed=7)
+    a = RespaMD(s, reorder_every=0)
+    b = RespaMD(s, reorder_every=1)
+    assert a.fused and b.fused
+    order0 = b.order.copy()
+    for _ in range(3):
+        a.run(2, use_graph=use_graph)
+        b.run(2, use_graph=use_graph)
+    torch.cuda.synchronize()
+    for name in ("x", "v", "f_field", "f_fast", "x32"):
+        assert torch.equal(a.in_input_order(getattr(a, name)), b.in_input_order(getattr(b, name))), name
+    order = b.order
+    assert np.array_equal(np.sort(order), np.arange(b.n))
+    assert torch.equal(b._order[b._inverse], torch.arange(b.n, device=b._order.device))
+    assert np.array_equal(b.in_input_order(b.types).cpu().numpy(), np.asarray(s.types, dtype=np.int32))
+    tail = int(_lib.lib().hymd_md_single_tail(b._native_handle()))
+    assert 0 < tail < b.n
+    assert np.array_equal(order[:tail], order0[:tail])          # molecules keep their rows
+    assert not np.array_equal(order[tail:], order0[tail:])      # the solvent was permuted
+    ea = a.run(1, energies=True)
+    eb = b.run(1, energies=True)
+    for k in ("field", "bond", "angle"):
+        assert ea[k] == eb[k], k
+    assert eb["kinetic"] == pytest.approx(ea["kinetic"], rel=1e-13)   # an FP64 sum in another order
+    a.close()
+    b.close()
