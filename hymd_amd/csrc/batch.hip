@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(kBlock) batch_key_kernel(const float *__restri
             const CicAxis cy = cic_axis((double)x[3 * i + 1], G.inv_h[1], G.my);
             key = G.row_off + cx.i0 * G.my + cy.i0;
         }
-        const int rank = rank_in_bin(key, counts);
+        const int rank = rank_in_bin_all(key, counts);
         if (i < n) keys[i] = make_int2(key, rank);
     }
 }

@@ -136,6 +136,34 @@ __device__ __forceinline__ int rank_in_bin(int key, int *counts) {
     return rank;
 }
 
+// rank_in_bin with every bin of the wavefront aggregated: the lanes are
+// grouped first (one ballot per distinct bin, no memory traffic), then each
+// group's lowest lane reserves the group's ranks with one atomic, all groups'
+// atomics in flight together.  At most one atomic per distinct bin per
+// wavefront, for ~10 instructions per distinct bin: the choice for key
+// kernels whose waves do little else.  Every lane of the wavefront must call it.
+__device__ __forceinline__ int rank_in_bin_all(int key, int *counts) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long pending = __ballot(key >= 0);
+    int leader = lane, offset = 0, size = 0;
+    while (pending) {  // uniform: one pass per distinct bin of the wavefront
+        const int first = __ffsll((long long)pending) - 1;
+        const int k = __builtin_amdgcn_readlane(key, first);
+        const unsigned long long match = __ballot(key == k) & pending;
+        if ((match >> lane) & 1ull) {
+            leader = first;
+            offset = __popcll(match & below);
+            if (lane == first) size = __popcll(match);
+        }
+        pending &= ~match;
+    }
+    int base = 0;
+    if (size) base = atomicAdd(counts + key, size);
+    base = __shfl(base, leader, 64);
+    return key >= 0 ? base + offset : 0;
+}
+
 // Division by a divisor fixed at launch, as a multiply-high and a shift
 // (Granlund & Montgomery, "Division by invariant integers using
 // multiplication", PLDI 1994, round-up multiplier): with s = ceil(log2 d) and

@@ -270,4 +270,111 @@ __device__ __forceinline__ void fft_inplace(float2 *buf, int se, int sl, int nli
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same in-place stages with the transform length, the line count and the
+// strides fixed at compile time: every index map is a multiply-shift or a
+// shift, the strides fold into the LDS offsets, and each thread holds only
+// the butterflies it owns.  The stage plan is the one fft_plan_init builds
+// (StagePlan, from the same factorisation), so the twiddle tables are shared.
+// ---------------------------------------------------------------------------
+struct StagePlan {
+    int ns;
+    int r[kMaxStages], p[kMaxStages], twoff[kMaxStages];
+};
+
+// radices largest first: 8s, then one 4 / 4+4 / 2 for the power-of-two
+// remainder, then 3s and 5s (ns = 0 if n is not 2^a 3^b 5^c)
+constexpr StagePlan make_stage_plan(int n) {
+    StagePlan sp{};
+    int rs[kMaxStages] = {};
+    int cnt = 0, a = 0;
+    while (n % 2 == 0) {
+        n /= 2;
+        ++a;
+    }
+    int eights = a / 3;
+    const int rem = a % 3;
+    int tail[2] = {0, 0}, nt = 0;
+    if (rem == 1 && eights > 0) {
+        --eights;
+        tail[nt++] = 4;
+        tail[nt++] = 4;
+    } else if (rem == 1) {
+        tail[nt++] = 2;
+    } else if (rem == 2) {
+        tail[nt++] = 4;
+    }
+    for (int i = 0; i < eights; ++i) rs[cnt++] = 8;
+    for (int i = 0; i < nt; ++i) rs[cnt++] = tail[i];
+    while (n % 3 == 0) {
+        n /= 3;
+        rs[cnt++] = 3;
+    }
+    while (n % 5 == 0) {
+        n /= 5;
+        rs[cnt++] = 5;
+    }
+    if (n != 1 || cnt > kMaxStages) return sp;
+    int p = 1, off = 0;
+    for (int i = 0; i < cnt; ++i) {
+        sp.r[i] = rs[i];
+        sp.p[i] = p;
+        sp.twoff[i] = off;
+        off += p * (rs[i] - 1);
+        p *= rs[i];
+    }
+    sp.ns = cnt;
+    return sp;
+}
+
+template <int N>
+struct CtPlan {
+    static constexpr StagePlan v = make_stage_plan(N);
+};
+
+template <int R, int P, int N, int NLINES, int SE, int SL, int NT, int SGN>
+__device__ __forceinline__ void stockham_inplace_ct(float2 *buf, const float2 *__restrict__ tw) {
+    constexpr int NR = N / R, TOTAL = NLINES * NR, KB = (TOTAL + NT - 1) / NT;
+    constexpr float sgn = (float)SGN;
+    float2 u[KB][8];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+        const int b = threadIdx.x + kb * NT;
+        if (TOTAL % NT != 0 && b >= TOTAL) continue;
+        const int i = b / NLINES, l = b - i * NLINES, k = i % P;
+        const float2 *src = buf + l * SL + i * SE;
+#pragma unroll
+        for (int j = 0; j < R; ++j) u[kb][j] = src[j * NR * SE];
+        if (P > 1) {
+#pragma unroll
+            for (int j = 1; j < R; ++j) {
+                float2 w = tw[k * (R - 1) + j - 1];
+                if (SGN > 0) w.y = -w.y;
+                u[kb][j] = cmul(u[kb][j], w);
+            }
+        }
+        small_dft<R>(u[kb], sgn);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+        const int b = threadIdx.x + kb * NT;
+        if (TOTAL % NT != 0 && b >= TOTAL) continue;
+        const int i = b / NLINES, l = b - i * NLINES, q = i / P, k = i - q * P;
+        float2 *dst = buf + l * SL + (q * P * R + k) * SE;
+#pragma unroll
+        for (int j = 0; j < R; ++j) dst[j * P * SE] = u[kb][j];
+    }
+    __syncthreads();
+}
+
+template <int N, int NLINES, int SE, int SL, int NT, int SGN, int S = 0>
+__device__ __forceinline__ void fft_inplace_ct(float2 *buf, const float2 *__restrict__ tw) {
+    constexpr StagePlan sp = CtPlan<N>::v;
+    if constexpr (S < sp.ns) {
+        stockham_inplace_ct<sp.r[S], sp.p[S], N, NLINES, SE, SL, NT, SGN>(buf, tw + sp.twoff[S]);
+        fft_inplace_ct<N, NLINES, SE, SL, NT, SGN, S + 1>(buf, tw);
+    }
+}
+
 }  // namespace hymd

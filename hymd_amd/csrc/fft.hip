@@ -256,17 +256,54 @@ __global__ void __launch_bounds__(kBlock) fft_strided_kernel(float2 *__restrict_
 // plane crosses HBM once per direction instead of twice.
 // ---------------------------------------------------------------------------
 
+// Plane geometry: any mesh whose plane fits (runtime sizes, FastDiv index
+// maps, runtime stage plans) or a fixed (My, Mz) with everything known at
+// compile time.
+struct PlaneRt {
+    int my, mz, nh, zh, npairs;
+    PlaneDivs dv;
+    FftAxis ay, azh;
+    __device__ int div_nh(int i) const { return fdiv(i, dv.nh); }
+    __device__ int div_zh(int i) const { return fdiv(i, dv.zh); }
+    __device__ int div_pairs(int i) const { return fdiv(i, dv.pairs); }
+    template <int SGN>
+    __device__ void fft_z(float2 *D, const float2 *tw) const {
+        fft_inplace(D, 1, zh, my, dv.my, azh, tw, (float)SGN);
+    }
+    template <int SGN>
+    __device__ void fft_y(float2 *D, const float2 *tw) const {
+        fft_inplace(D, zh, 1, zh, dv.zh, ay, tw, (float)SGN);
+    }
+};
+
+template <int MY, int MZ>
+struct PlaneCt {
+    static constexpr int my = MY, mz = MZ, nh = MZ / 2, zh = MZ / 2 + 1, npairs = MZ / 4 + 1;
+    __device__ int div_nh(int i) const { return i / nh; }
+    __device__ int div_zh(int i) const { return i / zh; }
+    __device__ int div_pairs(int i) const { return i / npairs; }
+    // rows: lines of nh complex, element stride 1, line stride zh
+    template <int SGN>
+    __device__ void fft_z(float2 *D, const float2 *tw) const {
+        fft_inplace_ct<nh, MY, 1, zh, kPlaneThreads, SGN>(D, tw);
+    }
+    // columns: lines of My, element stride zh, line stride 1
+    template <int SGN>
+    __device__ void fft_y(float2 *D, const float2 *tw) const {
+        fft_inplace_ct<MY, zh, zh, 1, kPlaneThreads, SGN>(D, tw);
+    }
+};
+
 // forward: My rows of Mz reals -> packed complex FFT of Mz/2 per row + split
 // (as r2c_z_kernel) -> FFT along y of the Mz/2+1 columns -> out plane
+template <class G>
 __global__ void __launch_bounds__(kPlaneThreads) r2c_zy_plane_kernel(const float *__restrict__ in,
-                                                                      float2 *__restrict__ out, int mx, int my,
-                                                                      int mz, FftAxis ay, FftAxis azh,
+                                                                      float2 *__restrict__ out, int mx, G g,
                                                                       const float2 *__restrict__ twy_g,
                                                                       const float2 *__restrict__ twz_g,
-                                                                      const float2 *__restrict__ post_g,
-                                                                      PlaneDivs dv) {
+                                                                      const float2 *__restrict__ post_g) {
     extern __shared__ float2 smem[];
-    const int nh = azh.n, zh = nh + 1;
+    const int my = g.my, nh = g.nh, zh = g.zh;
     float2 *twy = smem;
     float2 *twz = twy + my;
     float2 *post = twz + nh;
@@ -290,18 +327,18 @@ __global__ void __launch_bounds__(kPlaneThreads) r2c_zy_plane_kernel(const float
         for (int j = 0; j < kPlaneLoadBatch; ++j) {
             const int idx = i0 + j * kPlaneThreads;
             if (idx < nel) {
-                const int y = fdiv(idx, dv.nh);
+                const int y = g.div_nh(idx);
                 D[y * zh + (idx - y * nh)] = v[j];
             }
         }
     }
     __syncthreads();
-    fft_inplace(D, 1, zh, my, dv.my, azh, twz, -1.f);
+    g.template fft_z<-1>(D, twz);
     // split in place: the thread of pair (k, nh - k) writes X[k] and X[nh - k]
     // (k = 0 writes X[0] and X[nh]; k = nh/2 is its own pair)
-    const int npairs = nh / 2 + 1;
+    const int npairs = g.npairs;
     for (int idx = threadIdx.x; idx < my * npairs; idx += blockDim.x) {
-        const int y = fdiv(idx, dv.pairs), k = idx - y * npairs;
+        const int y = g.div_pairs(idx), k = idx - y * npairs;
         float2 *row = D + y * zh;
         const int kc = k == 0 ? 0 : nh - k;
         const float2 zk = row[k], zc = row[kc];
@@ -317,7 +354,7 @@ __global__ void __launch_bounds__(kPlaneThreads) r2c_zy_plane_kernel(const float
         if (km != k) row[km] = xm;
     }
     __syncthreads();
-    fft_inplace(D, zh, 1, zh, dv.zh, ay, twy, -1.f);
+    g.template fft_y<-1>(D, twy);
     float2 *dst = out + plane * my * zh;
     for (int idx = threadIdx.x; idx < my * zh; idx += blockDim.x) dst[idx] = D[idx];
 }
@@ -337,16 +374,15 @@ struct GradK {
 // -i kd v for a derivative factor kd
 __device__ __forceinline__ float2 mul_minus_i(float2 v, float kd) { return make_float2(kd * v.y, -kd * v.x); }
 
-template <bool GRAD>
+template <class G, bool GRAD>
 __global__ void __launch_bounds__(kPlaneThreads) c2r_yz_plane_kernel(const float2 *__restrict__ in,
-                                                                      float *__restrict__ out, int mx, int my,
-                                                                      int mz, FftAxis ay, FftAxis azh,
+                                                                      float *__restrict__ out, int mx, G g,
                                                                       const float2 *__restrict__ twy_g,
                                                                       const float2 *__restrict__ twz_g,
                                                                       const float2 *__restrict__ post_g,
-                                                                      float scale, GradK gk, PlaneDivs dv) {
+                                                                      float scale, GradK gk) {
     extern __shared__ float2 smem[];
-    const int nh = azh.n, zh = nh + 1;
+    const int my = g.my, mz = g.mz, nh = g.nh, zh = g.zh;
     float2 *twy = smem;
     float2 *twz = twy + my;
     float2 *post = twz + nh;
@@ -380,7 +416,7 @@ __global__ void __launch_bounds__(kPlaneThreads) c2r_yz_plane_kernel(const float
             if (idx < nel) {
                 float2 w = v[j];
                 if (GRAD && d == 1) {  // row y: k_y, zero on the Nyquist row
-                    const int y = fdiv(idx, dv.zh);
+                    const int y = g.div_zh(idx);
                     const int ny = 2 * y < my ? y : y - my;
                     w = mul_minus_i(w, 2 * y == my ? 0.f : (float)(gk.kfy * ny));
                 }
@@ -389,11 +425,11 @@ __global__ void __launch_bounds__(kPlaneThreads) c2r_yz_plane_kernel(const float
         }
     }
     __syncthreads();
-    fft_inplace(D, zh, 1, zh, dv.zh, ay, twy, 1.f);
+    g.template fft_y<1>(D, twy);
     // pre-process in place, pairs (k, nh - k) per thread; k = 0 reads X[nh]
-    const int npairs = nh / 2 + 1;
+    const int npairs = g.npairs;
     for (int idx = threadIdx.x; idx < my * npairs; idx += blockDim.x) {
-        const int y = fdiv(idx, dv.pairs), k = idx - y * npairs;
+        const int y = g.div_pairs(idx), k = idx - y * npairs;
         float2 *row = D + y * zh;
         const int kc = nh - k;  // k = 0 -> X[nh]
         float2 xk = row[k], xc = row[kc];
@@ -412,10 +448,10 @@ __global__ void __launch_bounds__(kPlaneThreads) c2r_yz_plane_kernel(const float
         row[k] = zk;
     }
     __syncthreads();
-    fft_inplace(D, 1, zh, my, dv.my, azh, twz, 1.f);
+    g.template fft_z<1>(D, twz);
     float2 *dst = reinterpret_cast<float2 *>(out) + out_plane * my * nh;
     for (int idx = threadIdx.x; idx < my * nh; idx += blockDim.x) {
-        const int y = fdiv(idx, dv.nh);
+        const int y = g.div_nh(idx);
         const float2 v = D[y * zh + (idx - y * nh)];
         dst[idx] = make_float2(v.x * scale, v.y * scale);
     }
@@ -446,19 +482,45 @@ static int launch_strided(const FftPlan &plan, float2 *data, int batch, int axis
     return HYMD_OK;
 }
 
-// Opt in to more than 64 KiB of dynamic LDS for the plane kernels (once).
+// Opt in to more than 64 KiB of dynamic LDS for the plane kernels of one
+// geometry (once per instantiation).
+template <class G>
 static int plane_lds_opt_in() {
     static bool done = false;
     if (!done) {
-        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&r2c_zy_plane_kernel),
+        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&r2c_zy_plane_kernel<G>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLdsBytes));
-        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&c2r_yz_plane_kernel<false>),
+        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&c2r_yz_plane_kernel<G, false>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLdsBytes));
-        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&c2r_yz_plane_kernel<true>),
+        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&c2r_yz_plane_kernel<G, true>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPlaneLdsBytes));
         done = true;
     }
     return HYMD_OK;
+}
+
+static bool same_plan(const FftAxis &ax, const StagePlan &sp) {
+    if (ax.nstages != sp.ns) return false;
+    for (int i = 0; i < sp.ns; ++i)
+        if (ax.radix[i] != sp.r[i] || ax.twoff[i] != sp.twoff[i]) return false;
+    return true;
+}
+
+// Calls f(geometry) with the compile-time geometry of the plan's (My, Mz) when
+// one is built for it, the runtime one otherwise.
+template <class F>
+static int with_plane_geom(const FftPlan &plan, F &&f) {
+    const Mesh &m = plan.mesh;
+#define HYMD_PLANE_CT(MY, MZ)                                                                         \
+    if (m.my == MY && m.mz == MZ && same_plan(plan.ay, CtPlan<MY>::v) && same_plan(plan.azh, CtPlan<MZ / 2>::v)) \
+        return f(PlaneCt<MY, MZ>{});
+    HYMD_PLANE_CT(48, 48)
+    HYMD_PLANE_CT(64, 64)
+    HYMD_PLANE_CT(96, 96)
+    HYMD_PLANE_CT(128, 128)
+    HYMD_PLANE_CT(160, 160)
+#undef HYMD_PLANE_CT
+    return f(PlaneRt{m.my, m.mz, m.mz / 2, m.zh, m.mz / 4 + 1, plan.pdiv, plan.ay, plan.azh});
 }
 
 int fft_forward_x(const FftPlan &plan, float2 *data, int batch, hipStream_t s) {
@@ -472,13 +534,16 @@ int fft_inverse_x(const FftPlan &plan, float2 *data, int batch, hipStream_t s) {
 int fft_r2c_zy(const FftPlan &plan, const float *in, float2 *out, int batch, hipStream_t s) {
     const Mesh &m = plan.mesh;
     if (plan.lds_plane) {
-        HYMD_TRY(plane_lds_opt_in());
-        dim3 grid((unsigned)m.mx, (unsigned)batch);
-        r2c_zy_plane_kernel<<<grid, kPlaneThreads, plan.lds_plane, s>>>(
-            in, out, m.mx, m.my, m.mz, plan.ay, plan.azh, (const float2 *)plan.tw_y.ptr,
-            (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr, plan.pdiv);
-        HYMD_HIP(hipGetLastError());
-        return HYMD_OK;
+        return with_plane_geom(plan, [&](auto g) -> int {
+            using G = decltype(g);
+            HYMD_TRY(plane_lds_opt_in<G>());
+            dim3 grid((unsigned)m.mx, (unsigned)batch);
+            r2c_zy_plane_kernel<G><<<grid, kPlaneThreads, plan.lds_plane, s>>>(
+                in, out, m.mx, g, (const float2 *)plan.tw_y.ptr, (const float2 *)plan.tw_zh.ptr,
+                (const float2 *)plan.post.ptr);
+            HYMD_HIP(hipGetLastError());
+            return HYMD_OK;
+        });
     }
     const int64_t rows = (int64_t)m.mx * m.my;
     dim3 grid((unsigned)((rows + plan.wz - 1) / plan.wz), (unsigned)batch);
@@ -497,13 +562,16 @@ int fft_r2c_real(const FftPlan &plan, const float *in, float2 *out, int batch, h
 int fft_c2r_yz(const FftPlan &plan, float2 *in, float *out, int batch, float scale, hipStream_t s) {
     const Mesh &m = plan.mesh;
     if (plan.lds_plane) {
-        HYMD_TRY(plane_lds_opt_in());
-        dim3 grid((unsigned)m.mx, (unsigned)batch);
-        c2r_yz_plane_kernel<false><<<grid, kPlaneThreads, plan.lds_plane, s>>>(
-            in, out, m.mx, m.my, m.mz, plan.ay, plan.azh, (const float2 *)plan.tw_y.ptr,
-            (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr, scale, GradK{}, plan.pdiv);
-        HYMD_HIP(hipGetLastError());
-        return HYMD_OK;
+        return with_plane_geom(plan, [&](auto g) -> int {
+            using G = decltype(g);
+            HYMD_TRY(plane_lds_opt_in<G>());
+            dim3 grid((unsigned)m.mx, (unsigned)batch);
+            c2r_yz_plane_kernel<G, false><<<grid, kPlaneThreads, plan.lds_plane, s>>>(
+                in, out, m.mx, g, (const float2 *)plan.tw_y.ptr, (const float2 *)plan.tw_zh.ptr,
+                (const float2 *)plan.post.ptr, scale, GradK{});
+            HYMD_HIP(hipGetLastError());
+            return HYMD_OK;
+        });
     }
     HYMD_TRY(launch_strided(plan, in, batch, 1, 1.f, s));
     const int64_t rows = (int64_t)m.mx * m.my;
@@ -522,13 +590,16 @@ int fft_c2r_yz_grad(const FftPlan &plan, const float2 *in, float *out, int T, do
         set_error("the gradient plane pass needs a y-z plane that fits in LDS");
         return HYMD_ERR_UNSUPPORTED;
     }
-    HYMD_TRY(plane_lds_opt_in());
-    dim3 grid((unsigned)m.mx, (unsigned)(3 * T));
-    c2r_yz_plane_kernel<true><<<grid, kPlaneThreads, plan.lds_plane, s>>>(
-        in, out, m.mx, m.my, m.mz, plan.ay, plan.azh, (const float2 *)plan.tw_y.ptr, (const float2 *)plan.tw_zh.ptr,
-        (const float2 *)plan.post.ptr, scale, GradK{T, kfy, kfz}, plan.pdiv);
-    HYMD_HIP(hipGetLastError());
-    return HYMD_OK;
+    return with_plane_geom(plan, [&](auto g) -> int {
+        using G = decltype(g);
+        HYMD_TRY(plane_lds_opt_in<G>());
+        dim3 grid((unsigned)m.mx, (unsigned)(3 * T));
+        c2r_yz_plane_kernel<G, true><<<grid, kPlaneThreads, plan.lds_plane, s>>>(
+            in, out, m.mx, g, (const float2 *)plan.tw_y.ptr, (const float2 *)plan.tw_zh.ptr,
+            (const float2 *)plan.post.ptr, scale, GradK{T, kfy, kfz});
+        HYMD_HIP(hipGetLastError());
+        return HYMD_OK;
+    });
 }
 
 bool fft_inplace_fits(const FftAxis &ax, int nlines, int threads, int elems) {

@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(kBlock) row_key_kernel(const float *__restrict
         const int64_t i = i0 + threadIdx.x;
         int key = -1;
         if (i < n && !(w && w[i] == 0.f)) key = base_row(x, i, m);
-        const int rank = rank_in_bin(key, counts);
+        const int rank = rank_in_bin_all(key, counts);
         if (i < n) keys[i] = make_int2(key, rank);
     }
 }
@@ -434,18 +434,43 @@ __global__ void __launch_bounds__(kBlock) field_xline_kernel(const float2 *__res
 // y-z plane pass.  The T spectra are transformed in place in one LDS buffer
 // of T W lines; V_t waits in registers (PMAX positions per thread) while the
 // x-gradient family is transformed and stored.
-template <int MAXT, int NT, int ELEMS>
+// x-line geometry: runtime (Mx, W, T from the plan) or compile-time (Mx = N,
+// T types, W lines per workgroup), the latter with every index map a constant
+// multiply-shift and the Stockham stages of fft_inplace_ct.
+template <int ELEMS>
+struct XlineRt {
+    int n, W, wshift, T, ld;
+    FastDiv ndiv, flines;
+    FftAxis ax;
+    __device__ int div_n(int i) const { return fdiv(i, ndiv); }
+    template <int SGN, int NT>
+    __device__ void fft(float2 *D, const float2 *tw) const {
+        fft_inplace<ELEMS>(D, 1, ld, T * W, flines, ax, tw, (float)SGN);
+    }
+};
+
+template <int T_, int N, int W_>
+struct XlineCt {
+    static constexpr int n = N, W = W_, T = T_, ld = N + 1;
+    static constexpr int wshift = W_ == 1 ? 0 : W_ == 2 ? 1 : W_ == 4 ? 2 : W_ == 8 ? 3 : 4;
+    __device__ int div_n(int i) const { return i / N; }
+    template <int SGN, int NT>
+    __device__ void fft(float2 *D, const float2 *tw) const {
+        fft_inplace_ct<N, T_ * W_, 1, N + 1, NT, SGN>(D, tw);
+    }
+};
+
+template <int MAXT, int NT, class G>
 __global__ void __launch_bounds__(NT) field_xline_grad_kernel(const float2 *__restrict__ spec,
-                                                              float2 *__restrict__ out, KGeom g, FieldK p,
-                                                              FftAxis ax, const float2 *__restrict__ tw_glob,
-                                                              int wshift, FastDiv flines,
+                                                              float2 *__restrict__ out, KGeom g, FieldK p, G geo,
+                                                              const float2 *__restrict__ tw_glob,
                                                               double *__restrict__ partials) {
     constexpr int PMAX = 16 / MAXT;  // W n <= NT PMAX (checked on the host)
     extern __shared__ float2 smem[];
-    const int n = ax.n;  // Mx
-    const int W = 1 << wshift;
-    const int T = p.T;
-    const int ld = ax.ld;
+    const int n = geo.n;  // Mx
+    const int W = geo.W, wshift = geo.wshift;
+    const int T = geo.T;
+    const int ld = geo.ld;
     float2 *tw = smem;
     float2 *D = tw + n;  // line (t W + l) at D + (t W + l) ld
     const int plane = g.my * g.zh;
@@ -460,7 +485,7 @@ __global__ void __launch_bounds__(NT) field_xline_grad_kernel(const float2 *__re
             const int idx = i0 + j * NT;
             v[j] = make_float2(0.f, 0.f);
             if (idx < nel) {
-                const int l = idx & (W - 1), te = idx >> wshift, t = fdiv(te, ax.ndiv), e = te - t * n;
+                const int l = idx & (W - 1), te = idx >> wshift, t = geo.div_n(te), e = te - t * n;
                 if (l < wv) v[j] = spec[(int64_t)t * g.gc + (int64_t)e * plane + line0 + l];
             }
         }
@@ -468,13 +493,13 @@ __global__ void __launch_bounds__(NT) field_xline_grad_kernel(const float2 *__re
         for (int j = 0; j < kLoadBatch; ++j) {
             const int idx = i0 + j * NT;
             if (idx < nel) {
-                const int l = idx & (W - 1), te = idx >> wshift, t = fdiv(te, ax.ndiv), e = te - t * n;
+                const int l = idx & (W - 1), te = idx >> wshift, t = geo.div_n(te), e = te - t * n;
                 D[(t * W + l) * ld + e] = v[j];
             }
         }
     }
     __syncthreads();
-    fft_inplace<ELEMS>(D, 1, ld, T * W, flines, ax, tw, -1.f);
+    geo.template fft<-1, NT>(D, tw);
     // k-space pass (the arithmetic of field_kspace_kernel): V_t kept in
     // registers, -i k_x V_t written back in place
     double e_acc[1] = {0.0};
@@ -536,9 +561,9 @@ __global__ void __launch_bounds__(NT) field_xline_grad_kernel(const float2 *__re
         }
     }
     __syncthreads();
-    fft_inplace<ELEMS>(D, 1, ld, T * W, flines, ax, tw, 1.f);
+    geo.template fft<1, NT>(D, tw);
     for (int idx = threadIdx.x; idx < nel; idx += NT) {
-        const int l = idx & (W - 1), te = idx >> wshift, t = fdiv(te, ax.ndiv), e = te - t * n;
+        const int l = idx & (W - 1), te = idx >> wshift, t = geo.div_n(te), e = te - t * n;
         if (l < wv) out[(int64_t)t * g.gc + (int64_t)e * plane + line0 + l] = D[(t * W + l) * ld + e];
     }
     __syncthreads();
@@ -552,9 +577,9 @@ __global__ void __launch_bounds__(NT) field_xline_grad_kernel(const float2 *__re
             if (t < T) D[(t * W + l) * ld + e] = hold[j][t];
     }
     __syncthreads();
-    fft_inplace<ELEMS>(D, 1, ld, T * W, flines, ax, tw, 1.f);
+    geo.template fft<1, NT>(D, tw);
     for (int idx = threadIdx.x; idx < nel; idx += NT) {
-        const int l = idx & (W - 1), te = idx >> wshift, t = fdiv(te, ax.ndiv), e = te - t * n;
+        const int l = idx & (W - 1), te = idx >> wshift, t = geo.div_n(te), e = te - t * n;
         if (l < wv) out[(int64_t)(T + t) * g.gc + (int64_t)e * plane + line0 + l] = D[(t * W + l) * ld + e];
     }
     if (partials) block_reduce_store<1, NT>(e_acc, partials + blockIdx.x);
@@ -718,7 +743,7 @@ static int launch_xline(hymd_pm *pm, const float2 *spec, float2 *grad, const KGe
     return HYMD_OK;
 }
 
-constexpr int kXlineGradThreads = 512;
+constexpr int kXlineGradThreads = 256;
 constexpr int kXlineGradElems = 16;  // values per thread and in-place stage
 
 // Lines per workgroup of field_xline_grad_kernel for T types (0: it does not
@@ -735,27 +760,60 @@ static int xline_grad_width(const FftPlan &f, int T) {
     return 0;
 }
 
-template <int MAXT>
-static int launch_xline_grad(hymd_pm *pm, const float2 *spec, float2 *out, const KGeom &g, const FieldK &p, int W,
-                             double *partials, int *nblocks) {
+template <int MAXT, class G>
+static int launch_xline_grad_g(hymd_pm *pm, const float2 *spec, float2 *out, const KGeom &g, const FieldK &p, int W,
+                               const G &geo, double *partials, int *nblocks) {
     const FftPlan &f = pm->fft;
     const int plane = g.my * g.zh;
     const int nb = (plane + W - 1) / W;
     const size_t lds = sizeof(float2) * ((size_t)f.ax.n + (size_t)p.T * W * f.ax.ld);
     static bool attr_set = false;  // the opt-in is per kernel and process
     if (!attr_set && lds > 64 * 1024) {
-        HYMD_HIP(hipFuncSetAttribute(
-            reinterpret_cast<const void *>(&field_xline_grad_kernel<MAXT, kXlineGradThreads, kXlineGradElems>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kXlineLdsBytes));
+        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&field_xline_grad_kernel<MAXT, kXlineGradThreads, G>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kXlineLdsBytes));
         attr_set = true;
     }
-    int wshift = 0;
-    while ((1 << wshift) < W) ++wshift;
-    field_xline_grad_kernel<MAXT, kXlineGradThreads, kXlineGradElems><<<nb, kXlineGradThreads, lds, pm->stream>>>(
-        spec, out, g, p, f.ax, (const float2 *)f.tw_x.ptr, wshift, make_fastdiv((uint32_t)(p.T * W)), partials);
+    field_xline_grad_kernel<MAXT, kXlineGradThreads, G><<<nb, kXlineGradThreads, lds, pm->stream>>>(
+        spec, out, g, p, geo, (const float2 *)f.tw_x.ptr, partials);
     HYMD_HIP(hipGetLastError());
     *nblocks = nb;
     return HYMD_OK;
+}
+
+template <int N>
+static bool xline_ct_plan(const FftPlan &f) {
+    const StagePlan &sp = CtPlan<N>::v;
+    if (f.ax.n != N || f.ax.ld != N + 1 || f.ax.nstages != sp.ns) return false;
+    for (int i = 0; i < sp.ns; ++i)
+        if (f.ax.radix[i] != sp.r[i] || f.ax.twoff[i] != sp.twoff[i]) return false;
+    return true;
+}
+
+// The x-line pass for T types with W lines per workgroup: a compile-time
+// geometry for Mx = 128 or 160 at the W that xline_grad_width picks there,
+// the runtime one otherwise.
+static int launch_xline_grad(hymd_pm *pm, const float2 *spec, float2 *out, const KGeom &g, const FieldK &p, int W,
+                             double *partials, int *nblocks) {
+    const FftPlan &f = pm->fft;
+    const int T = p.T;
+#define HYMD_XLINE_CT(NX, TT, WW, MT)                                                                      \
+    if (f.ax.n == NX && T == TT && W == WW && xline_ct_plan<NX>(f))                                       \
+        return launch_xline_grad_g<MT>(pm, spec, out, g, p, W, XlineCt<TT, NX, WW>{}, partials, nblocks);
+    HYMD_XLINE_CT(128, 1, 16, 2)
+    HYMD_XLINE_CT(128, 2, 16, 2)
+    HYMD_XLINE_CT(128, 3, 8, 4)
+    HYMD_XLINE_CT(128, 4, 8, 4)
+    HYMD_XLINE_CT(160, 1, 8, 2)
+    HYMD_XLINE_CT(160, 2, 8, 2)
+    HYMD_XLINE_CT(160, 3, 4, 4)
+    HYMD_XLINE_CT(160, 4, 4, 4)
+#undef HYMD_XLINE_CT
+    int wshift = 0;
+    while ((1 << wshift) < W) ++wshift;
+    const XlineRt<kXlineGradElems> geo{f.ax.n, W, wshift, T, f.ax.ld, f.ax.ndiv, make_fastdiv((uint32_t)(T * W)), f.ax};
+    if (T <= 2) return launch_xline_grad_g<2>(pm, spec, out, g, p, W, geo, partials, nblocks);
+    if (T <= 4) return launch_xline_grad_g<4>(pm, spec, out, g, p, W, geo, partials, nblocks);
+    return launch_xline_grad_g<8>(pm, spec, out, g, p, W, geo, partials, nblocks);
 }
 
 // LDS budget of one row-paint workgroup (int64 accumulators of tc types x Mz)
@@ -878,6 +936,8 @@ static FieldK field_params(int T, const double *coupling, const double *shift, d
 // Bin particles by the mesh row of their CIC base cell: perm[pos] = original
 // index, xs/ts/ws the sorted positions, types (0 when types == NULL) and
 // weights (if w != NULL), start[r] .. start[r+1] the particles based in row r.
+// Bins the n particles by mesh row.  keys_ready: the keys (and counts) were
+// produced by field_key_target()'s producer.
 static int sort_particles(hymd_pm *pm, const float *x, const int32_t *types, const float *w, int64_t n,
                           bool keys_ready = false) {
     const Mesh &m = pm->mesh;
@@ -913,7 +973,8 @@ static int paint_sorted(hymd_pm *pm, int T, const double *mass, float *out, long
     const int tc = std::max(1, std::min(T, (int)(kRowLdsBytes / (sizeof(long long) * m.mz))));
     dim3 grid((unsigned)(m.mx * m.my), (unsigned)((T + tc - 1) / tc));
     paint_rows_kernel<<<grid, kBlock, sizeof(long long) * tc * m.mz, pm->stream>>>(
-        (const float4 *)pm->xs.ptr, pm->sorted_weights ? (const float *)pm->ws.ptr : nullptr, (const int *)pm->start.ptr, m, pa, tc, out_scale,
+        (const float4 *)pm->xs.ptr, pm->sorted_weights ? (const float *)pm->ws.ptr : nullptr,
+        (const int *)pm->start.ptr, m, pa, tc, out_scale,
         out, out_fixed);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
@@ -1043,9 +1104,7 @@ static int update_field_impl(hymd_pm *pm, const float *x, const int32_t *types, 
         HYMD_TRY(fft_r2c_zy(pm->fft, rho, spec, T, s));
         double *part = energy ? (double *)pm->partials.ptr : nullptr;
         int nb = 0;
-        if (T <= 2) HYMD_TRY(launch_xline_grad<2>(pm, spec, grad, g0, p, W2, part, &nb));
-        else if (T <= 4) HYMD_TRY(launch_xline_grad<4>(pm, spec, grad, g0, p, W2, part, &nb));
-        else HYMD_TRY(launch_xline_grad<8>(pm, spec, grad, g0, p, W2, part, &nb));
+        HYMD_TRY(launch_xline_grad(pm, spec, grad, g0, p, W2, part, &nb));
         HYMD_TRY(fft_c2r_yz_grad(pm->fft, grad, force_mesh, T, g0.kfac[1], g0.kfac[2], 1.f, s));
         if (energy) {
             double e = 0.0;

@@ -55,9 +55,11 @@ def test_paint_empty(hip):
 
 
 # (16, 200, 200): its y-z plane exceeds the LDS budget, so the separate line
-# passes run; the other meshes use the plane kernels
+# passes run; the other meshes use the plane kernels, with the runtime plane
+# geometry or, for y-z planes of 48^2, 64^2, 96^2, 128^2, 160^2, the
+# compile-time one
 @pytest.mark.parametrize("mesh", [(8, 8, 8), (32, 32, 32), (48, 64, 96), (128, 128, 128), (160, 160, 160),
-                                  (20, 30, 36), (16, 200, 200)])
+                                  (20, 30, 36), (16, 200, 200), (12, 48, 48), (10, 64, 64), (6, 96, 96)])
 def test_fft_vs_numpy(hip, mesh):
     rng = np.random.default_rng(sum(mesh))
     pm = _pm(mesh, (1.0, 1.0, 1.0), 1)
@@ -221,7 +223,10 @@ def test_drop_in_field_surface(hip):
 
 
 @pytest.mark.parametrize("mesh,T", [((32, 48, 40), 3), ((24, 24, 24), 2), ((16, 20, 12), 5), ((48, 40, 64), 4),
-                                    ((128, 96, 80), 6), ((64, 64, 64), 1), ((40, 30, 18), 8)])
+                                    ((128, 96, 80), 6), ((64, 64, 64), 1), ((40, 30, 18), 8),
+                                    # compile-time x-line (Mx 128, 160) and plane (128^2, 160^2) geometries
+                                    ((128, 40, 24), 4), ((128, 24, 20), 2), ((160, 20, 16), 3), ((160, 16, 24), 1),
+                                    ((16, 128, 128), 4), ((8, 160, 160), 2)])
 def test_fused_xline_pass_equals_unfused(hip, mesh, T):
     """The force-grid-only update (forward x, k-space and inverse x passes in
     one kernel, -i k_y and -i k_z applied in the y-z plane pass) against the
