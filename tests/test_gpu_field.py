@@ -23,7 +23,12 @@ def _dev(a, dtype):
     return torch.as_tensor(np.ascontiguousarray(a)).to(dtype).cuda().contiguous()
 
 
-@pytest.mark.parametrize("mesh,box", [((32, 48, 40), (7.0, 9.5, 8.25)), ((16, 16, 16), (4.0, 4.0, 4.0))])
+# the row paint owns 4 mesh rows along y per workgroup (fewer in the last
+# block when My % 4 != 0; one when My < 4 cannot happen, edges being >= 4):
+# My = 4, 6, 10 cover the wrap of the source rows and the partial blocks
+@pytest.mark.parametrize("mesh,box", [((32, 48, 40), (7.0, 9.5, 8.25)), ((16, 16, 16), (4.0, 4.0, 4.0)),
+                                      ((8, 4, 12), (2.0, 1.0, 3.0)), ((6, 6, 8), (1.5, 1.5, 2.0)),
+                                      ((10, 10, 6), (2.5, 2.5, 1.5))])
 def test_paint_fixed_bit_exact(hip, mesh, box):
     rng = np.random.default_rng(11)
     n, T = 20000, 3
@@ -253,3 +258,30 @@ def test_fused_xline_pass_equals_unfused(hip, mesh, T):
     assert np.abs(a1 - a2).max() <= 1e-6 * np.abs(a2).max()
     assert rms_rel(a1, a2) < 2e-7
     assert e1 == pytest.approx(e2, rel=1e-9)
+
+
+@pytest.mark.parametrize("mesh,T", [((20, 24, 18), 3), ((16, 16, 16), 2), ((12, 10, 8), 5), ((24, 20, 30), 8),
+                                    ((6, 6, 144), 8)])
+def test_readout_matches_oracle(hip, mesh, T):
+    """compute_field_force on arbitrary grids against the oracle's CIC readout
+    (field.py:317-330), with types out of range giving zero: the row-staged
+    readout (4-aligned Mz with vector loads, other Mz with scalar loads) and
+    the per-particle one (T * Mz past the row kernel's LDS budget) alike."""
+    rng = np.random.default_rng(sum(mesh) + T)
+    box = (3.0, 4.5, 2.5)
+    n = 5000
+    x = (rng.uniform(-0.3, 1.3, (n, 3)) * np.array(box)).astype(np.float32)   # some outside the box
+    types = rng.integers(0, T, n).astype(np.int32)
+    types[::97] = T + 1                                                        # out of range: zero force
+    grids = rng.normal(size=(T, 3) + mesh).astype(np.float32)
+    pm = _pm(mesh, box, T)
+    f = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    pm.readout_forces(_dev(x, torch.float32), _dev(types, torch.int32), T, _dev(grids, torch.float32), f)
+    xr = np.mod(x.astype(np.float64), box)
+    ok = types < T
+    ref = np.zeros((n, 3))
+    ref[ok] = F.compute_field_force(xr[ok], types[ok], [[grids[t, d].astype(np.float64) for d in range(3)]
+                                                         for t in range(T)], mesh, box)
+    got = f.cpu().numpy()
+    np.testing.assert_array_equal(got[~ok], 0.0)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=2e-6 * np.abs(ref).max())
