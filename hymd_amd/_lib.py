@@ -78,8 +78,13 @@ SIGNATURES = {
     "hymd_fused_destroy": [_p],
     "hymd_fused_inner": [_p, _p, _p, _p, _p, _p, _d, _d, _d, _i32, _i32, _pd, _pd, _pd, _p],
     "hymd_md_stage_time": [_p, _pd, _pi32],
+    "hymd_thermostat_workspace_bytes": [],
+    "hymd_csvr_thermostat": [_p, _p, _i64, _i32, _pi32, _i32, _pi64, _pd, _pd, _d, _d, _d, _d, _i32, _p, _p, _i64,
+                             _p],
+    "hymd_cancel_com_momentum": [_p, _i64, _i64, _p, _i64, _p],
 }
-_RESTYPES = {"hymd_last_error": ctypes.c_char_p, "hymd_md_single_tail": ctypes.c_int64}
+_RESTYPES = {"hymd_last_error": ctypes.c_char_p, "hymd_md_single_tail": ctypes.c_int64,
+             "hymd_thermostat_workspace_bytes": ctypes.c_int64}
 
 
 class HymdError(RuntimeError):

@@ -51,6 +51,11 @@ class Config:
     m: Optional[list] = None                 # per-type paint mass (input_parser.py:1138-1141)
     target_temperature: Optional[float] = None
     tau: Optional[float] = None
+    start_temperature: Optional[float] = None
+    thermostat_work: float = 0.0             # input_parser.py:189
+    thermostat_coupling_groups: List[List[str]] = field(default_factory=list)
+    cancel_com_momentum: object = False      # int period or False (input_parser.py:192)
+    n_b: Optional[int] = None                # thermostat period; check_n_b defaults it to 1
 
     def __post_init__(self):
         if self.box_size is not None:

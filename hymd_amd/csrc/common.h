@@ -212,6 +212,7 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *out_
     constexpr int kWaves = NT / 64;
     __shared__ double red[NV][kWaves];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
     for (int j = 0; j < NV; ++j) {
         double s = v[j];
         for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
@@ -224,6 +225,23 @@ __device__ __forceinline__ void block_reduce_store(double (&v)[NV], double *out_
             for (int w = 0; w < kWaves; ++w) t += red[j][w];
             out_block[j] = t;
         }
+}
+
+// Fixed-order sum over i = lane, lane + 64, ... < n of p[i * stride] (the
+// order of a plain strided loop), with up to 16 loads in flight per lane.
+__device__ __forceinline__ double lane_strided_sum(const double *__restrict__ p, int n, int64_t stride, int lane) {
+    double t = 0.0;
+    for (int b0 = lane; b0 < n; b0 += 64 * 16) {
+        double x[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int b = b0 + 64 * q;
+            x[q] = b < n ? p[(int64_t)b * stride] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) t += x[q];
+    }
+    return t;
 }
 
 // Reduce `nblocks` partial rows of NV doubles (row-major) in fixed order into

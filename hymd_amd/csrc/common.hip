@@ -62,8 +62,7 @@ void HostBuf::release() {
 __device__ void reduce_rows(const double *__restrict__ partials, int nblocks, int nv, double *__restrict__ out) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
     for (int j = wid; j < nv; j += nw) {
-        double acc = 0.0;
-        for (int b = lane; b < nblocks; b += 64) acc += partials[(int64_t)b * nv + j];
+        double acc = lane_strided_sum(partials + j, nblocks, nv, lane);
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
         if (lane == 0) out[j] = acc;
     }

@@ -1,4 +1,5 @@
-"""rRESPA molecular dynamics on one GPU (main.py:764-1169 without IO/thermostat).
+"""rRESPA molecular dynamics on one GPU (main.py:764-1169 without IO; the
+thermostat and COM removal of main.py:1314-1321 in ``simulate``).
 
 Per outer step, in the reference's order (main.py:802-887, 975-1004,
 1143-1169):
@@ -92,6 +93,7 @@ class RespaMD:
         self.fast_forces(dipole_flag=1)
         self.slow_forces()
         self._native = None
+        self._thermostat = None
 
     @property
     def order(self):
@@ -188,6 +190,37 @@ class RespaMD:
                 self.energies["field_q"] = x2[1]
             return dict(self.energies)
         return None
+
+    def simulate(self, n_steps, config, prng=None, use_graph=True, step0=0):
+        """``n_steps`` outer steps with the coupling of main.py:1314-1321: after
+        outer step ``step`` (counted from ``step0`` + 1), the CSVR thermostat
+        when ``config.target_temperature`` is set and step % config.n_b == 0,
+        then COM momentum removal when step % config.cancel_com_momentum == 0.
+        The thermostat work accumulates on the device in ``thermostat_work``."""
+        from .thermostat import CsvrThermostat, cancel_com_momentum
+        n_b = int(config.n_b or 1)
+        n_com = int(config.cancel_com_momentum or 0)
+        if config.target_temperature and self._thermostat is None:
+            self._thermostat = CsvrThermostat(self.types, config, n_types=self.T)
+        step, end = int(step0), int(step0) + int(n_steps)
+        while step < end:
+            nxt = end
+            if config.target_temperature:
+                nxt = min(nxt, (step // n_b + 1) * n_b)
+            if n_com:
+                nxt = min(nxt, (step // n_com + 1) * n_com)
+            self.run(nxt - step, use_graph=use_graph)
+            step = nxt
+            if config.target_temperature and step % n_b == 0:
+                self._thermostat(self.v, prng)
+            if n_com and step % n_com == 0:
+                cancel_com_momentum(self.v, config)
+        return step
+
+    @property
+    def thermostat_work(self):
+        """Thermostat work of ``simulate`` so far (config.thermostat_work)."""
+        return 0.0 if self._thermostat is None else float(self._thermostat.work.item())
 
     STAGES = {"bonded": 1, "kick_drift": 2, "field": 3, "readout": 4}
 

@@ -241,6 +241,35 @@ int hymd_respa_kick(double *v, int64_t n, const float *f1, double c1, const floa
 int hymd_kinetic_energy(const void *v, int64_t n, int32_t precision, double mass, double *out, void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Thermostat and momentum removal on device FP64 velocities v (n,3).        */
+/* Both take a caller-owned device workspace of                              */
+/* hymd_thermostat_workspace_bytes() bytes, zero-filled before its first use */
+/* (calls that share one workspace must be ordered on one stream), and do    */
+/* not synchronise.  hymd_csvr_thermostat keeps each COM group's velocity in */
+/* it as the shift of the next call's one-pass sums: give every thermostat   */
+/* its own workspace so its results depend on its own history only.         */
+/* ------------------------------------------------------------------------ */
+int64_t hymd_thermostat_workspace_bytes(void);
+/* csvr_thermostat (thermostat.py:111-219).  Coupling group g (0 <= g <
+ * n_groups <= 16, in the reference's order) holds the particles whose type t
+ * has type_group[t] == g (host int32[n_types], -1: no group); group_n[g]
+ * (host) its particle count (all > 0), gaussians[g] and chi_squares[g]
+ * (host) the N(0,1) and chi^2(3 group_n[g] - 1) numbers drawn for it;
+ * c = exp(-time_step * respa_inner / tau).  With remove_com and
+ * group_n[g] > 1 the group's COM-free velocities are rescaled, otherwise
+ * every velocity is rescaled from the kinetic energy of all particles (the
+ * reference's branches).  work (device double) += the thermostat work
+ * (config.thermostat_work). */
+int hymd_csvr_thermostat(double *v, const int32_t *types, int64_t n, int32_t n_types, const int32_t *type_group,
+                         int32_t n_groups, const int64_t *group_n, const double *gaussians,
+                         const double *chi_squares, double mass, double gas_constant, double target_temperature,
+                         double c, int32_t remove_com, double *work, void *workspace, int64_t workspace_bytes,
+                         void *stream);
+/* cancel_com_momentum (thermostat.py:12-15): v -= sum(v) / n_particles */
+int hymd_cancel_com_momentum(double *v, int64_t n, int64_t n_particles, void *workspace, int64_t workspace_bytes,
+                             void *stream);
+
+/* ------------------------------------------------------------------------ */
 /* Whole rRESPA outer steps enqueued from C (the MD loop of main.py:764-1169 */
 /* restricted to the hot path: kicks, drift + wrap, bonds + angles every     */
 /* inner step, update_field + compute_field_force every outer step).  The    */
