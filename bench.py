@@ -261,6 +261,28 @@ def peptide_solvent_stages(reps, check):
     return out
 
 
+def nvt_overhead(md, s, steps, nve_ms_per_step, use_graph):
+    """The same outer steps coupled to the CSVR thermostat after every step
+    (n_b = 1, one coupling group with COM removal: the reference's defaults,
+    main.py:1314-1316) at 310 K, tau 0.1 ps."""
+    import torch
+    from hymd_amd.config import Config
+    cfg = Config(n_steps=steps, time_step=s.time_step, mesh_size=s.mesh, sigma=s.sigma, kappa=s.kappa,
+                 mass=s.mass, respa_inner=s.respa_inner, target_temperature=310.0, tau=0.1,
+                 unique_names=list(s.type_names), n_types=s.n_types, n_b=1, n_particles=s.n)
+    prng = np.random.default_rng(1234)
+    md.simulate(3, cfg, prng, use_graph=use_graph)    # warm-up (builds the group tables)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    md.simulate(steps, cfg, prng, use_graph=use_graph)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / steps
+    return {"ms_per_step": round(ms, 4), "steps": steps, "thermostat_us_per_step": round(1e3 * (ms - nve_ms_per_step), 2),
+            "kinetic_temperature_K": round(2.0 * md.kinetic_energy() / (3.0 * s.n * cfg.gas_constant), 2),
+            "thermostat_work": md.thermostat_work,
+            "coupling": "CSVR every outer step, one group, COM removal, T0 310 K, tau 0.1 ps"}
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -281,6 +303,8 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 10 device copies of MIB MiB (calibrates HBM byte counters) and exit")
     ap.add_argument("--no-graph", action="store_true", help="launch each step's kernels eagerly (no hipGraph)")
+    ap.add_argument("--nvt-steps", type=int, default=30,
+                    help="outer steps with the CSVR thermostat after every step (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
@@ -350,6 +374,11 @@ def main():
                 "eager_ms_per_step": round(eager_ms, 4)}
 
     log(f"roofline stage {args.roofline_kernel}: {dur_ms * 1e3:.2f} us/launch over {launches} launches")
+    nvt = None
+    if args.nvt_steps > 0:
+        nvt = nvt_overhead(md, s, args.nvt_steps, ms_per_step, not args.no_graph)
+        log(f"nvt: {nvt['ms_per_step']:.4f} ms/step with the thermostat every step "
+            f"(+{nvt['thermostat_us_per_step']:.1f} us)")
     cpu = accuracy = None
     if args.cpu_steps > 0:
         log(f"cpu baseline: {args.cpu_steps} oracle outer steps")
@@ -409,6 +438,7 @@ def main():
         "accuracy_vs_fp64_oracle": accuracy,
         "batched_chi_grads": batch,
         "peptide_solvent_forces": peptide,
+        "nvt_csvr": nvt,
     }
     print(json.dumps(out))
 

@@ -19,8 +19,11 @@ struct hymd_md {
     hymd_topology *topo = nullptr;
     hipStream_t work = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipGraph_t graph = nullptr;
+    // captured outer steps: [1] starts with the last step's outer half kick
+    // pending (fused path, replayed back to back), [0] without (per-kernel
+    // path, or the fused path after the velocities were completed)
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
+    hipGraph_t graph[2] = {nullptr, nullptr};
     int64_t n = 0;
     int T = 0;
     double *x = nullptr, *v = nullptr;
@@ -299,13 +302,21 @@ extern "C" int hymd_md_create(hymd_md **out, hymd_pm *pm, hymd_topology *topo, i
     return HYMD_OK;
 }
 
+static void drop_graphs(hymd_md *m) {
+    for (int i = 0; i < 2; ++i) {
+        if (m->exec[i]) (void)hipGraphExecDestroy(m->exec[i]);
+        if (m->graph[i]) (void)hipGraphDestroy(m->graph[i]);
+        m->exec[i] = nullptr;
+        m->graph[i] = nullptr;
+    }
+}
+
 extern "C" int hymd_md_destroy(hymd_md *m) {
     if (!m) return HYMD_OK;
     if (m->work) (void)hipStreamSynchronize(m->work);
     for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
     if (m->fused) (void)hymd_fused_destroy(m->fused);
-    if (m->exec) (void)hipGraphExecDestroy(m->exec);
-    if (m->graph) (void)hipGraphDestroy(m->graph);
+    drop_graphs(m);
     if (m->ev_in) (void)hipEventDestroy(m->ev_in);
     if (m->ev_out) (void)hipEventDestroy(m->ev_out);
     if (m->work) (void)hipStreamDestroy(m->work);
@@ -313,11 +324,13 @@ extern "C" int hymd_md_destroy(hymd_md *m) {
     return HYMD_OK;
 }
 
-static int capture(hymd_md *m) {
+// capture one outer step starting with (pending = 1) or without a pending
+// outer half kick into graph[pending]
+static int capture(hymd_md *m, int pending) {
     const int timed = m->timed;
     m->timed = 0;  // no event records inside the graph
     HYMD_HIP(hipStreamBeginCapture(m->work, hipStreamCaptureModeThreadLocal));
-    int st = enqueue_step(m, nullptr, 1);
+    int st = enqueue_step(m, nullptr, pending);
     m->timed = timed;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(m->work, &g);
@@ -329,8 +342,8 @@ static int capture(hymd_md *m) {
         set_error("hipStreamEndCapture failed: %s", hipGetErrorString(e));
         return HYMD_ERR_HIP;
     }
-    m->graph = g;
-    HYMD_HIP(hipGraphInstantiate(&m->exec, g, nullptr, nullptr, 0));
+    m->graph[pending] = g;
+    HYMD_HIP(hipGraphInstantiate(&m->exec[pending], g, nullptr, nullptr, 0));
     return HYMD_OK;
 }
 
@@ -358,12 +371,10 @@ extern "C" int hymd_md_step(hymd_md *m, int32_t n_steps, int32_t use_graph, doub
     HYMD_HIP(hipStreamWaitEvent(m->work, m->ev_in, 0));
     const int plain = energies ? n_steps - 1 : n_steps;
     for (int s = 0; s < plain; ++s) {
-        // the captured step assumes a pending outer kick (fused) and is the
-        // plain per-kernel step otherwise
-        const bool graphable = use_graph && (!m->fused || m->pending);
-        if (graphable) {
-            if (!m->exec) HYMD_TRY(capture(m));
-            HYMD_HIP(hipGraphLaunch(m->exec, m->work));
+        if (use_graph) {
+            const int gi = m->fused ? m->pending : 0;
+            if (!m->exec[gi]) HYMD_TRY(capture(m, gi));
+            HYMD_HIP(hipGraphLaunch(m->exec[gi], m->work));
         } else {
             HYMD_TRY(enqueue_step(m, nullptr, m->pending));
         }
@@ -393,14 +404,7 @@ extern "C" int hymd_md_set_fused(hymd_md *m, int32_t enable) {
         return HYMD_ERR_UNSUPPORTED;
     }
     HYMD_HIP(hipStreamSynchronize(m->work));
-    if (m->exec) {
-        (void)hipGraphExecDestroy(m->exec);
-        m->exec = nullptr;
-    }
-    if (m->graph) {
-        (void)hipGraphDestroy(m->graph);
-        m->graph = nullptr;
-    }
+    drop_graphs(m);
     if (!enable) {
         if (m->fused) (void)hymd_fused_destroy(m->fused);
         m->fused = nullptr;
@@ -483,14 +487,7 @@ extern "C" int hymd_md_set_peptide(hymd_md *m, const float *charges, double elec
     HYMD_REQUIRE(!dih || n_dihedrals >= 0, "bad dihedral count");
     HYMD_HIP(hipStreamSynchronize(m->work));
     if (charges || dih) HYMD_TRY(hymd_md_set_fused(m, 0));  // the fused inner loop has no dihedral terms
-    if (m->exec) {
-        (void)hipGraphExecDestroy(m->exec);
-        m->exec = nullptr;
-    }
-    if (m->graph) {
-        (void)hipGraphDestroy(m->graph);
-        m->graph = nullptr;
-    }
+    drop_graphs(m);
     m->charges = charges;
     m->elec = elec_factor;
     m->e_self = self_energy;

@@ -332,3 +332,24 @@ def test_solvent_reorder_keeps_the_trajectory(hip, use_graph):
     assert eb["kinetic"] == pytest.approx(ea["kinetic"], rel=1e-13)   # an FP64 sum in another order
     a.close()
     b.close()
+
+
+def test_single_step_graph_replays_equal_eager_steps(hip):
+    """run(1) completes the velocities, so the next call starts without a
+    pending outer kick: its captured step (the no-pending graph) must run the
+    same kernels as the eager step."""
+    from hymd_amd import synth
+    from hymd_amd.md import RespaMD
+    s = synth.lipid_water(n_chains=600, n_water=5000, box=8.0, mesh=32, seed=5)
+    a = RespaMD(s)
+    b = RespaMD(s)
+    assert a.fused and b.fused
+    for _ in range(4):
+        a.run(1, use_graph=True)
+        b.run(1, use_graph=False)
+    a.run(3, use_graph=True)     # then the pending-kick graph, back to back
+    b.run(3, use_graph=False)
+    torch.cuda.synchronize()
+    assert torch.equal(a.x, b.x) and torch.equal(a.v, b.v) and torch.equal(a.f_field, b.f_field)
+    a.close()
+    b.close()
