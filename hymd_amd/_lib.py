@@ -82,6 +82,10 @@ SIGNATURES = {
     "hymd_csvr_thermostat": [_p, _p, _i64, _i32, _pi32, _i32, _pi64, _pd, _pd, _d, _d, _d, _d, _i32, _p, _p, _i64,
                              _p],
     "hymd_cancel_com_momentum": [_p, _i64, _i64, _p, _i64, _p],
+    "hymd_bond_builder_create": [ctypes.POINTER(_p), _i64, _i64, _p, _p, _p, _p, _p, _i32, _pi32, _pd, _i32, _pi32,
+                                 _pd, _i32, _pi32, _pi32, _pd, _i32, _pi64, _p],
+    "hymd_bond_builder_fill": [_p] + [_p] * 19,
+    "hymd_bond_builder_destroy": [_p],
 }
 _RESTYPES = {"hymd_last_error": ctypes.c_char_p, "hymd_md_single_tail": ctypes.c_int64,
              "hymd_thermostat_workspace_bytes": ctypes.c_int64}

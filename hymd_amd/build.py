@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libhymd_amd.so")
 ARCH = os.environ.get("HYMD_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["common.hip", "scan.hip", "fft.hip", "field.hip", "batch.hip", "bonded.hip", "integrator.hip", "fused.hip",
-           "md.hip", "thermostat.hip"]
+           "md.hip", "thermostat.hip", "topology.hip"]
 HEADERS = ["common.h", "fft.h", "field_dev.h", "bonded.h"]
 
 

@@ -270,6 +270,47 @@ int hymd_cancel_com_momentum(double *v, int64_t n, int64_t n_particles, void *wo
                              void *stream);
 
 /* ------------------------------------------------------------------------ */
+/* Bonded term arrays from the connectivity table: prepare_bonds            */
+/* (force.py:573-728 over prepare_bonds_old :325-505), terms in the         */
+/* reference's order (networkx traversal orders of the molecule graphs).    */
+/* ------------------------------------------------------------------------ */
+typedef struct hymd_bond_builder hymd_bond_builder;
+/* Rows mol_start[m] .. mol_start[m+1]-1 (device int64, n_mol+1) of the
+ * device arrays below are molecule m's particles, molecules in np.unique
+ * order, rows in the reference's row order:
+ *   index  int64 (n)  global index (the graph's node keys)
+ *   row    int64 (n)  local index written into the terms, or NULL (= position)
+ *   name   int32 (n)  name id
+ *   bonds  int64 (n, max_bonds) global indices of bond partners, -1 = none
+ * Type tables (host): bond_names (n_bond_types,2) + bond_params (r0, k);
+ * angle_names (n_angle_types,3) + angle_params (theta0 in radians, k);
+ * dihedral_names (n_dihedral_types,4), dihedral_kind (dih_type),
+ * dihedral_coeffs (n_dihedral_types,6,5) as np.resize gives them; name ids
+ * of -1 never match.  At most 256 types of each kind.  Builds and walks the
+ * molecule graphs, writes the term counts (bonds, angles, dihedrals) into
+ * counts (host) and synchronises.  Fails with HYMD_ERR_INVALID and a message
+ * starting "KeyError" (a bond partner outside its molecule) or "ValueError"
+ * (a dihedral whose reversed copy was stored with another type first), the
+ * reference's exceptions. */
+int hymd_bond_builder_create(hymd_bond_builder **builder, int64_t n, int64_t n_mol, const int64_t *mol_start,
+                             const int64_t *index, const int64_t *row, const int32_t *name, const int64_t *bonds,
+                             int32_t max_bonds, const int32_t *bond_names, const double *bond_params,
+                             int32_t n_bond_types, const int32_t *angle_names, const double *angle_params,
+                             int32_t n_angle_types, const int32_t *dihedral_names, const int32_t *dihedral_kind,
+                             const double *dihedral_coeffs, int32_t n_dihedral_types, int64_t counts[3],
+                             void *stream);
+/* The 16 arrays of prepare_bonds (device, sized by counts; indices int32):
+ * bonds a2, b2, equilibrium, strength; angles a3, b3, c3, equilibrium,
+ * strength; dihedrals a4, b4, c4, d4, coeff (D,6,5) float64, dih_type, last.
+ * type2 / type3 / type4 (optional, NULL = not written): the config entry of
+ * each term.  Enqueued on the builder's stream. */
+int hymd_bond_builder_fill(hymd_bond_builder *builder, int32_t *a2, int32_t *b2, double *eq2, double *k2, int32_t *a3,
+                           int32_t *b3, int32_t *c3, double *eq3, double *k3, int32_t *a4, int32_t *b4, int32_t *c4,
+                           int32_t *d4, double *coeff4, int32_t *kind4, int32_t *last4, int32_t *type2,
+                           int32_t *type3, int32_t *type4);
+int hymd_bond_builder_destroy(hymd_bond_builder *builder);
+
+/* ------------------------------------------------------------------------ */
 /* Whole rRESPA outer steps enqueued from C (the MD loop of main.py:764-1169 */
 /* restricted to the hot path: kicks, drift + wrap, bonds + angles every     */
 /* inner step, update_field + compute_field_force every outer step).  The    */
