@@ -261,6 +261,66 @@ def peptide_solvent_stages(reps, check):
     return out
 
 
+def lipid_water_bond_table(s):
+    """The synthetic lipid/water system as an input file holds it: a
+    (N, 2) bond table (chain neighbours, -1 padded), names, molecules."""
+    n = s.n
+    a, b = s.bonds[0], s.bonds[1]
+    table = -np.ones((n, 2), dtype=np.int64)
+    table[a, 1] = b
+    table[b, 0] = a
+    names = np.array(s.type_names)[s.types]
+    n_chain = len(a) // 11
+    molecules = np.r_[np.repeat(np.arange(n_chain), 12), n_chain + np.arange(n - 12 * n_chain)]
+    bonds = [[p, q, 0.47, 1250.0] for p, q in (("H", "H"), ("H", "L"), ("L", "L"), ("L", "T"), ("T", "T"))]
+    angles = [["H", "H", "L", 180.0, 25.0], ["H", "L", "L", 120.0, 25.0], ["L", "L", "T", 120.0, 25.0],
+              ["L", "T", "T", 180.0, 25.0], ["T", "T", "T", 180.0, 25.0]]
+    return molecules, names, table, bonds, angles
+
+
+def topology_build(s, reps, cpu_chains):
+    """prepare_bonds (force.py:573-728) of the lipid/water system on the
+    device, checked against the system's own term lists; the oracle on the
+    first `cpu_chains` chains and as many waters per chain is its CPU
+    baseline."""
+    import torch
+    from hymd_amd.topology import prepare_bonds
+
+    class Cfg:
+        pass
+
+    molecules, names, table, bonds, angles = lipid_water_bond_table(s)
+    cfg = Cfg()
+    cfg.bonds, cfg.angle_bonds, cfg.dihedrals = bonds, angles, []
+    idx = np.arange(s.n)
+    prepare_bonds(molecules[:24], names[:24], np.where(table[:24] < 24, table[:24], -1), idx[:24], cfg)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = prepare_bonds(molecules, names, table, idx, cfg)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    ok = (np.array_equal(out[0].cpu().numpy(), s.bonds[0]) and np.array_equal(out[1].cpu().numpy(), s.bonds[1])
+          and np.array_equal(np.stack([t.cpu().numpy() for t in out[4:7]]), np.stack(s.angles[:3])))
+    res = {"beads": s.n, "molecules": int(molecules[-1]) + 1, "ms": round(1e3 * min(times), 2),
+           "beads_per_s": s.n / min(times), "bonds": len(out[0]), "angles": len(out[4]), "matches_system_terms": ok,
+           "includes": "host marshalling (name ids, molecule segments, uploads) + device build + copies"}
+    if cpu_chains > 0:
+        from oracle import topology as OT          # CPU baseline only
+        n_chain = len(s.bonds[0]) // 11
+        rows = np.r_[np.arange(12 * cpu_chains), 12 * n_chain + np.arange(8 * cpu_chains)]
+        remap = -np.ones(s.n, dtype=np.int64)
+        remap[rows] = np.arange(len(rows))
+        sub = np.where(table[rows] >= 0, remap[np.maximum(table[rows], 0)], -1)
+        t0 = time.perf_counter()
+        OT.prepare_bonds(molecules[rows], names[rows], sub, np.arange(len(rows)), bonds, angles, [])
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"kind": "port", "cores": 1, "beads_per_s": len(rows) / dt,
+                               "sample": f"{cpu_chains} chains + {8 * cpu_chains} waters ({len(rows)} beads)"}
+    return res
+
+
 def nvt_overhead(md, s, steps, nve_ms_per_step, use_graph):
     """The same outer steps coupled to the CSVR thermostat after every step
     (n_b = 1, one coupling group with COM removal: the reference's defaults,
@@ -305,6 +365,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch each step's kernels eagerly (no hipGraph)")
     ap.add_argument("--nvt-steps", type=int, default=30,
                     help="outer steps with the CSVR thermostat after every step (0 = skip)")
+    ap.add_argument("--topology-reps", type=int, default=3,
+                    help="device prepare_bonds builds of the bench system (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
@@ -397,6 +459,12 @@ def main():
             "bonded_force_rel_rms": float(np.sqrt(np.mean((f_fast0 - fo_fast) ** 2)) / np.sqrt(np.mean(fo_fast ** 2))),
         }
 
+    topo = None
+    if args.topology_reps > 0:
+        log("prepare_bonds of the bench system")
+        topo = topology_build(s, args.topology_reps, 10000 if args.cpu_steps > 0 else 0)
+        log(f"prepare_bonds: {topo['ms']:.1f} ms for {s.n} beads")
+
     batch = None
     if args.batch_replicas > 0:
         md.close()
@@ -439,6 +507,7 @@ def main():
         "batched_chi_grads": batch,
         "peptide_solvent_forces": peptide,
         "nvt_csvr": nvt,
+        "prepare_bonds": topo,
     }
     print(json.dumps(out))
 
