@@ -595,6 +595,84 @@ __global__ void __launch_bounds__(kBlock) field_moments_kernel(const float2 *__r
                         partials + (int64_t)blockIdx.x * (2 * (MAXT * (MAXT + 1) / 2) + MAXT));
 }
 
+// comp_pressure's field terms (pressure.py:9-179) from the filtered spectra
+// phi~_t (pmesh normalisation) by Parseval, summed over the full spectrum
+// through the half-spectrum weights, with V_t = (A phi~)_t:
+//   [0]      S   = sum_k Re(phi~^* . V)
+//   [1..3]   S_d = sum_k k_d^2 Re(phi~^* . V)
+//   [4]      E1  = sum_k Re(psi^* sum_t q_t phi~_t)            (psi != NULL)
+//   [5..7]   E1_d = sum_k k_d^2 Re(psi^* sum_t q_t phi~_t)
+//   [8]      Eq  = sum_k Re(phi_q^* psi)                       (phi_q != NULL)
+//   [9..9+T) phi~_t(k = 0)
+// k_d is -pi/h on the Nyquist plane of an even edge (its square is what the
+// Laplacian transfer uses).  psi and phi_q are raw r2c spectra, scaled by
+// inv_g here.
+constexpr int kPressNv = 9 + 8;
+struct PressK {
+    double A[8][8];
+    double q[8];
+    double inv_g;
+};
+
+template <int MAXT>
+__global__ void __launch_bounds__(kBlock) field_pressure_kernel(const float2 *__restrict__ phif,
+                                                                 const float2 *__restrict__ psif,
+                                                                 const float2 *__restrict__ qf, KGeom g, int T,
+                                                                 PressK p, double *__restrict__ partials) {
+    double acc[kPressNv];
+#pragma unroll
+    for (int j = 0; j < kPressNv; ++j) acc[j] = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; kk < g.gc; kk += stride) {
+        double k[3], kd[3], wgt;
+        kvec(g, kk, k, kd, wgt);
+        double re[MAXT], im[MAXT];
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t) {
+            const float2 v = (t < T) ? phif[(int64_t)t * g.gc + kk] : make_float2(0.f, 0.f);
+            re[t] = (double)v.x;
+            im[t] = (double)v.y;
+        }
+        double sw = 0.0;
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t) {
+            double vr = 0.0, vi = 0.0;
+#pragma unroll
+            for (int u = 0; u < MAXT; ++u) {
+                vr += p.A[t][u] * re[u];
+                vi += p.A[t][u] * im[u];
+            }
+            sw += re[t] * vr + im[t] * vi;
+        }
+        const double k2[3] = {k[0] * k[0], k[1] * k[1], k[2] * k[2]};
+        acc[0] += wgt * sw;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) acc[1 + d] += wgt * k2[d] * sw;
+        if (psif) {
+            const float2 ps = psif[kk];
+            const double pr = p.inv_g * (double)ps.x, pi = p.inv_g * (double)ps.y;
+            double cr = 0.0, ci = 0.0;
+#pragma unroll
+            for (int t = 0; t < MAXT; ++t) {
+                cr += p.q[t] * re[t];
+                ci += p.q[t] * im[t];
+            }
+            const double e1 = pr * cr + pi * ci;
+            acc[4] += wgt * e1;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) acc[5 + d] += wgt * k2[d] * e1;
+            if (qf) {
+                const float2 qv = qf[kk];
+                acc[8] += wgt * (p.inv_g * (double)qv.x * pr + p.inv_g * (double)qv.y * pi);
+            }
+        }
+        if (kk == 0)
+#pragma unroll
+            for (int t = 0; t < MAXT; ++t) acc[9 + t] += re[t];
+    }
+    block_reduce_store<kPressNv>(acc, partials + (int64_t)blockIdx.x * kPressNv);
+}
+
 struct PmeK {
     double sigma2;
     double inv_g;
@@ -1280,6 +1358,63 @@ extern "C" int hymd_field_moments(hymd_pm *pm, const float *x, const int32_t *ty
     }
     HYMD_HIP(hipStreamSynchronize(s));
     for (int j = 0; j < nv; ++j) moments[j] = ((double *)pm->host.ptr)[j];
+    return HYMD_OK;
+}
+
+extern "C" int hymd_field_pressure(hymd_pm *pm, const void *phi_fourier, int32_t n_types, const double *coupling,
+                                   const double *shift, double constant, double sigma, const float *psi,
+                                   const float *phi_q, const double *type_charges, double self_energy,
+                                   double *out) {
+    HYMD_REQUIRE(pm && phi_fourier && coupling && shift && out, "null argument");
+    HYMD_REQUIRE(n_types >= 1 && n_types <= 8, "the pressure supports n_types <= 8");
+    HYMD_REQUIRE(!psi || type_charges, "psi needs the per-type charges");
+    HYMD_REQUIRE(!phi_q || psi, "phi_q enters with psi only");
+    const Mesh &m = pm->mesh;
+    const int T = n_types;
+    hipStream_t s = pm->stream;
+    PressK p{};
+    for (int t = 0; t < T; ++t) {
+        for (int u = 0; u < T; ++u) p.A[t][u] = coupling[t * T + u];
+        p.q[t] = psi ? type_charges[t] : 0.0;
+    }
+    p.inv_g = 1.0 / (double)m.g;
+    float2 *psif = nullptr, *qf = nullptr;
+    if (psi) {  // raw spectra of psi and phi_q into the gradient scratch (>= 4 spectra)
+        psif = (float2 *)pm->grad.ptr;
+        HYMD_TRY(fft_r2c_real(pm->fft, psi, psif, 1, s));
+        if (phi_q) {
+            qf = psif + m.gc;
+            HYMD_TRY(fft_r2c_real(pm->fft, phi_q, qf, 1, s));
+        }
+    }
+    KGeom g = kgeom(pm);
+    const int nb = moments_blocks(m.gc);
+    DevBuf &part = pm->mpart;
+    HYMD_TRY(part.reserve(sizeof(double) * (size_t)kPressNv * nb));
+    HYMD_TRY(pm->result.reserve(sizeof(double) * kPressNv));
+    HYMD_TRY(pm->host.reserve(sizeof(double) * kPressNv));
+    const float2 *phif = (const float2 *)phi_fourier;
+    if (T <= 2)
+        field_pressure_kernel<2><<<nb, kBlock, 0, s>>>(phif, psif, qf, g, T, p, (double *)part.ptr);
+    else if (T <= 4)
+        field_pressure_kernel<4><<<nb, kBlock, 0, s>>>(phif, psif, qf, g, T, p, (double *)part.ptr);
+    else
+        field_pressure_kernel<8><<<nb, kBlock, 0, s>>>(phif, psif, qf, g, T, p, (double *)part.ptr);
+    HYMD_HIP(hipGetLastError());
+    HYMD_TRY(reduce_partials((const double *)part.ptr, nb, kPressNv, (double *)pm->result.ptr, s));
+    HYMD_HIP(hipMemcpyAsync(pm->host.ptr, pm->result.ptr, sizeof(double) * kPressNv, hipMemcpyDeviceToHost, s));
+    HYMD_HIP(hipStreamSynchronize(s));
+    const double *r = (const double *)pm->host.ptr;
+    double bphi = 0.0;
+    for (int t = 0; t < T; ++t) bphi += shift[t] * r[9 + t];
+    const double V = m.box[0] * m.box[1] * m.box[2];
+    // p0 = -(dV/V) sum_cells (w + w_elec), p1 = (dV/V) sum V_bar phi~, p2_d = sigma^2 (dV/V) sum V_bar lap_d phi~
+    double p0 = -(0.5 * r[0] + bphi + constant);
+    if (phi_q) p0 += -0.5 * r[8] + self_energy / V;
+    const double p1 = r[0] + bphi + r[4];
+    out[0] = p0;
+    out[1] = p1;
+    for (int d = 0; d < 3; ++d) out[2 + d] = -sigma * sigma * (r[1 + d] + r[5 + d]);
     return HYMD_OK;
 }
 

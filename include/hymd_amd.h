@@ -20,6 +20,9 @@
  *              compute_dihedral_forces*.f90), dipole_forces_redistribution :855-877
  *   integrator.py integrate_velocity :9-41, integrate_position :44-73 as
  *              composed by the rRESPA loop main.py:802-887,1143-1169
+ *   thermostat.py csvr_thermostat :111-219, cancel_com_momentum :12-15
+ *   force.py   prepare_bonds :573-728 (over prepare_bonds_old :325-505)
+ *   pressure.py comp_pressure :9-179 (field terms, with comp_laplacian :182-200)
  */
 #ifndef HYMD_AMD_H
 #define HYMD_AMD_H
@@ -111,6 +114,20 @@ int hymd_field_q_energy(hymd_pm *pm, const float *phi_q, const float *psi, doubl
 int hymd_field_moments(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
                        const double *mass, double sigma, double *moments, const double *coupling,
                        const double *shift, float *force_mesh);
+
+/* comp_pressure's field terms (pressure.py:9-179, comp_laplacian
+ * pressure.py:182-200) from the filtered spectra of update_field:
+ *   phi_fourier device complex64 (n_types,Mx,My,Mz/2+1), pmesh normalisation (n_types <= 8)
+ *   coupling, shift, constant: the quadratic form of w (A, b, c0), host
+ *   psi, phi_q device float32 (Mx,My,Mz) grids of the PME path, or NULL: with psi,
+ *     V_bar_t carries type_charges[t] * psi (host, n_types); with phi_q too,
+ *     w carries 1/2 phi_q psi - self_energy / V (coulombtype PIC_Spectral)
+ * out host double[5]: p0, p1, p2_x, p2_y, p2_z (the 18-vector's entries 1-5).
+ * Parseval sums over the spectra in FP64; no Laplacian grids are formed.
+ * Synchronises the handle's stream. */
+int hymd_field_pressure(hymd_pm *pm, const void *phi_fourier, int32_t n_types, const double *coupling,
+                        const double *shift, double constant, double sigma, const float *psi, const float *phi_q,
+                        const double *type_charges, double self_energy, double *out);
 
 /* A fixed dihedral term list (compute_dihedral_forces__double.f90 +
  * dipole_reconstruction__double.f90, force.py:855-877) held on the device with
