@@ -66,6 +66,7 @@ SIGNATURES = {
     "hymd_respa_kick_drift": [_p, _p, _p, _i64, _p, _d, _p, _d, _d, _pd, _p],
     "hymd_respa_kick": [_p, _i64, _p, _d, _p, _d, _p],
     "hymd_kinetic_energy": [_p, _i64, _i32, _d, _pd, _p],
+    "hymd_scale_positions": [_p, _i32, _p, _i64, _i64, _i64, _pd, _p],
     "hymd_md_create": [ctypes.POINTER(_p), _p, _p, _i64, _i32, _p, _p, _p, _p, _p, _p, _p, _pd, _pd, _d, _pd, _d,
                        _d, _d, _i32, _pd],
     "hymd_md_step": [_p, _i32, _i32, _pd, _p],

@@ -20,6 +20,14 @@ class Chi:
 
 
 @dataclass
+class Target_pressure:
+    """Target pressures in bar, lateral and normal (barostat.py:37-40)."""
+
+    P_L: object = None
+    P_N: object = None
+
+
+@dataclass
 class Config:
     gas_constant: ClassVar[float] = 0.0083144621  # kJ mol-1 K-1 (input_parser.py:159)
     coulomb_constant: ClassVar[float] = 138.935458  # kJ nm mol-1 e-2 (input_parser.py:160)
@@ -45,7 +53,7 @@ class Config:
     dielectric_const: Optional[float] = None
     self_energy: Optional[float] = None
     type_charges: Optional[list] = None
-    barostat: Optional[str] = None
+    barostat: Optional[str] = None           # "isotropic" | "semiisotropic"
     rho0: Optional[float] = None
     a: Optional[float] = None
     m: Optional[list] = None                 # per-type paint mass (input_parser.py:1138-1141)
@@ -55,7 +63,11 @@ class Config:
     thermostat_work: float = 0.0             # input_parser.py:189
     thermostat_coupling_groups: List[List[str]] = field(default_factory=list)
     cancel_com_momentum: object = False      # int period or False (input_parser.py:192)
-    n_b: Optional[int] = None                # thermostat period; check_n_b defaults it to 1
+    n_b: Optional[int] = None                # thermostat / barostat period; check_n_b defaults it to 1
+    pressure: bool = False                   # input_parser.py:204-211
+    barostat_type: Optional[str] = None      # "berendsen" | "scr"
+    tau_p: Optional[float] = None
+    target_pressure: Optional[Target_pressure] = None
 
     def __post_init__(self):
         if self.box_size is not None:

@@ -88,9 +88,45 @@ int convert_f64_f32(float *f32, double *f64, int64_t n, bool to_f32, hipStream_t
     return HYMD_OK;
 }
 
+// x[r, d] *= f[d] for rows [r0, r1) of an (n, 3) array; x32 (optional)
+// receives float(x) of those rows
+template <typename T>
+__global__ void __launch_bounds__(kBlock) scale_rows_kernel(T *__restrict__ x, float *__restrict__ x32, int64_t r0,
+                                                             int64_t r1, double f0, double f1, double f2) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t n3 = 3 * (r1 - r0);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += stride) {
+        const int64_t e = 3 * r0 + i;
+        const int d = (int)(e % 3);
+        const double f = d == 0 ? f0 : (d == 1 ? f1 : f2);
+        const T v = (T)((double)x[e] * f);
+        x[e] = v;
+        if (x32) x32[e] = (float)v;
+    }
+}
+
 }  // namespace hymd
 
 using namespace hymd;
+
+extern "C" int hymd_scale_positions(void *x, int32_t precision, float *x32, int64_t n, int64_t row_begin,
+                                    int64_t row_end, const double factors[3], void *stream) {
+    HYMD_REQUIRE(factors && n >= 0 && 0 <= row_begin && row_begin <= row_end && row_end <= n, "bad row range");
+    HYMD_REQUIRE(precision == HYMD_F32 || precision == HYMD_F64, "precision must be HYMD_F32 or HYMD_F64");
+    HYMD_REQUIRE(row_end == row_begin || x, "NULL positions");
+    const int64_t m = row_end - row_begin;
+    hipStream_t s = (hipStream_t)stream;
+    if (m > 0) {
+        if (precision == HYMD_F64)
+            scale_rows_kernel<double><<<grid_for(3 * m), kBlock, 0, s>>>((double *)x, x32, row_begin, row_end,
+                                                                          factors[0], factors[1], factors[2]);
+        else
+            scale_rows_kernel<float><<<grid_for(3 * m), kBlock, 0, s>>>((float *)x, x32, row_begin, row_end,
+                                                                         factors[0], factors[1], factors[2]);
+    }
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
+}
 
 extern "C" int hymd_respa_kick_drift(double *x, double *v, float *x32, int64_t n, const float *f_fast, double c_fast,
                                      const float *f_slow, double c_slow, double dt, const double box[3],

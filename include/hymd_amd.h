@@ -23,6 +23,7 @@
  *   thermostat.py csvr_thermostat :111-219, cancel_com_momentum :12-15
  *   force.py   prepare_bonds :573-728 (over prepare_bonds_old :325-505)
  *   pressure.py comp_pressure :9-179 (field terms, with comp_laplacian :182-200)
+ *   barostat.py / barostat_scr.py isotropic, semiisotropic (box and position rescaling)
  */
 #ifndef HYMD_AMD_H
 #define HYMD_AMD_H
@@ -254,6 +255,12 @@ int hymd_respa_kick_drift(double *x, double *v, float *x32, int64_t n, const flo
                           const float *f_slow, double c_slow, double dt, const double box[3], void *stream);
 /* v += c1*f1 + c2*f2 (f2 may be NULL) */
 int hymd_respa_kick(double *v, int64_t n, const float *f1, double c1, const float *f2, double c2, void *stream);
+/* Box rescaling of the barostats (barostat.py:145-151, 279-302; barostat_scr.py:125-129):
+ * x[r, d] *= factors[d] for the rows [row_begin, row_end) of x (n,3), float32 or float64 per
+ * precision (each product in FP64, rounded to the array's type); x32 (float32 (n,3)) receives
+ * float(x) of those rows, or NULL */
+int hymd_scale_positions(void *x, int32_t precision, float *x32, int64_t n, int64_t row_begin, int64_t row_end,
+                         const double factors[3], void *stream);
 /* 1/2 m sum v^2 (field.py:695) into host double; v (n,3) float32 or float64 per precision */
 int hymd_kinetic_energy(const void *v, int64_t n, int32_t precision, double mass, double *out, void *stream);
 
