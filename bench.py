@@ -321,6 +321,50 @@ def topology_build(s, reps, cpu_chains):
     return res
 
 
+def pressure_terms(md, s, reps, check):
+    """comp_pressure's field terms (pressure.py:9-179) at the current
+    configuration: one update_field writing the filtered spectra, then the
+    device Parseval pass (hymd_field_pressure, synchronising); against the
+    FP64 oracle's real-space sums over its Laplacian grids on the same
+    positions, whose time is the CPU baseline."""
+    import torch
+    from hymd_amd.pressure import field_pressure
+    T = md.T
+    pf = md.pm.create("complex", batch=T)
+    fm = md.pm.create("real", batch=(T, 3))
+    md.pm.field_update(md.x32, md.types, T, md.A, md.b, md.c0, md.type_mass, md.sigma, fm, phi_fourier=pf)
+    field_pressure(md.pm, pf, T, md.A, md.b, md.c0, md.sigma)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        got = field_pressure(md.pm, pf, T, md.A, md.b, md.c0, md.sigma)
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    res = {"p0_p1_p2xyz": [float(v) for v in got], "ms_per_call": round(ms, 4), "calls": reps,
+           "spectra_bytes": int(pf.numel() * 8),
+           "includes": "the Parseval kernel, its reduction, the host read-back (the call synchronises)"}
+    if check:
+        from oracle import field as OF          # CPU baseline and check only
+        from oracle import pressure as OP
+        cores = min(os.cpu_count() or 1, 16)
+        OF.set_workers(cores)
+        ham = OF.Quadratic(md.A, md.b, md.c0)
+        x = md.x32.cpu().numpy().astype(np.float64)
+        ty = md.types.cpu().numpy()
+        t0 = time.perf_counter()
+        ref = OF.update_field(x, ty, T, s.mesh, s.box, md.sigma, ham)
+        t1 = time.perf_counter()
+        want = OP.comp_pressure(ref["phi"], ref["phi_fourier"], ham, np.zeros((1, 3)), s.mass, s.mesh, s.box,
+                                md.sigma, np.zeros(3), np.zeros(3))[1:6]
+        t2 = time.perf_counter()
+        scale = max(abs(want[0]), abs(want[1]))
+        res["oracle_p0_p1_p2xyz"] = [float(v) for v in want]
+        res["max_err_rel_to_p1"] = float(np.max(np.abs(got - want)) / scale)
+        res["cpu_baseline"] = {"kind": "port", "cores": cores, "ms_pressure_terms": round(1e3 * (t2 - t1), 1),
+                               "ms_update_field": round(1e3 * (t1 - t0), 1),
+                               "sample": "the whole configuration (numpy + scipy.fft workers)"}
+    return res
+
+
 def nvt_overhead(md, s, steps, nve_ms_per_step, use_graph):
     """The same outer steps coupled to the CSVR thermostat after every step
     (n_b = 1, one coupling group with COM removal: the reference's defaults,
@@ -365,6 +409,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch each step's kernels eagerly (no hipGraph)")
     ap.add_argument("--nvt-steps", type=int, default=30,
                     help="outer steps with the CSVR thermostat after every step (0 = skip)")
+    ap.add_argument("--pressure-reps", type=int, default=20,
+                    help="device comp_pressure field passes at the bench configuration (0 = skip)")
     ap.add_argument("--topology-reps", type=int, default=3,
                     help="device prepare_bonds builds of the bench system (0 = skip)")
     args = ap.parse_args()
@@ -436,6 +482,10 @@ def main():
                 "eager_ms_per_step": round(eager_ms, 4)}
 
     log(f"roofline stage {args.roofline_kernel}: {dur_ms * 1e3:.2f} us/launch over {launches} launches")
+    press = None
+    if args.pressure_reps > 0:
+        press = pressure_terms(md, s, args.pressure_reps, args.cpu_steps > 0 and not args.small)
+        log(f"pressure field terms: {press['ms_per_call']:.3f} ms per call")
     nvt = None
     if args.nvt_steps > 0:
         nvt = nvt_overhead(md, s, args.nvt_steps, ms_per_step, not args.no_graph)
@@ -508,6 +558,7 @@ def main():
         "peptide_solvent_forces": peptide,
         "nvt_csvr": nvt,
         "prepare_bonds": topo,
+        "pressure_field_terms": press,
     }
     print(json.dumps(out))
 
