@@ -550,7 +550,8 @@ def main():
                                "respa_inner %d, bonds %d, angles %d" % (s.n, s.box[0], s.mesh[0], s.n_types,
                                                                       s.respa_inner, len(s.bonds[0]),
                                                                       len(s.angles[0])),
-                   "precision": "fp32 paint/FFT/forces, fp64 positions/velocities and reductions"},
+                   "precision": "fp32 paint/FFT/forces (bond and angle terms fp32 as the reference's default "
+                                "real(4) kernels), fp64 positions/velocities, kicks and reductions"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "accuracy_vs_fp64_oracle": accuracy,

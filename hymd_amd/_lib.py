@@ -73,6 +73,8 @@ SIGNATURES = {
     "hymd_md_destroy": [_p],
     "hymd_md_time_stage": [_p, _i32, _i32],
     "hymd_md_set_fused": [_p, _i32],
+    "hymd_md_set_term_precision": [_p, _i32],
+    "hymd_fused_set_precision": [_p, _i32],
     "hymd_md_is_fused": [_p],
     "hymd_md_set_reorder": [_p, _i32, _p, _p, _p],
     "hymd_md_single_tail": [_p],

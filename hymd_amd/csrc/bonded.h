@@ -87,6 +87,43 @@ __device__ __forceinline__ bool angle_term(V3 ra, V3 rc, double t0, double k, V3
     return true;
 }
 
+// --- the same two terms in FP32 (the reference's default real(4) kernels,
+// compute_bond_forces.f90 / compute_angle_forces.f90), on separations taken
+// in FP64 and rounded once; no energies (the energy pass stays FP64) ------
+struct F3 {
+    float x, y, z;
+};
+__device__ __forceinline__ F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+__device__ __forceinline__ F3 to_f3(V3 a) { return F3{(float)a.x, (float)a.y, (float)a.z}; }
+__device__ __forceinline__ float dotf(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+__device__ __forceinline__ F3 bond_term_f(F3 rab, float r0, float k) {
+    const float rn = sqrtf(dotf(rab, rab));
+    const float s = -(k * (rn - r0)) / rn;
+    return f3(s * rab.x, s * rab.y, s * rab.z);
+}
+
+// theta = atan2(|n|, ra.rc) with n = ra x rc formed in FP64 from the FP64
+// separations, and the force directions n x ra and rc x n (perpendicular to
+// ra and rc, so free of cancellation): the error of theta and of the forces
+// stays at FP32 rounding for nearly straight angles too, where acos(cos)
+// would amplify it by 1 / sin(theta).
+//   fa = -k (theta - t0) (n x ra) / (|n| |ra|^2),  fc = -k (theta - t0) (rc x n) / (|n| |rc|^2)
+__device__ __forceinline__ void angle_term_f(V3 ra64, V3 rc64, float t0, float k, F3 &fa, F3 &fc) {
+    const F3 n = to_f3(cross(ra64, rc64));
+    const F3 ra = to_f3(ra64), rc = to_f3(rc64);
+    const float nn = sqrtf(dotf(n, n));
+    if (!(nn > 0.0f)) {
+        fa = fc = f3(0.f, 0.f, 0.f);
+        return;
+    }
+    const float theta = atan2f(nn, dotf(ra, rc));
+    const float g = -k * (theta - t0) / nn;
+    const float ga = g / dotf(ra, ra), gc = g / dotf(rc, rc);
+    fa = f3(ga * (n.y * ra.z - n.z * ra.y), ga * (n.z * ra.x - n.x * ra.z), ga * (n.x * ra.y - n.y * ra.x));
+    fc = f3(gc * (rc.y * n.z - rc.z * n.y), gc * (rc.z * n.x - rc.x * n.z), gc * (rc.x * n.y - rc.y * n.x));
+}
+
 // --- cosine series  E += sum c_n (1 + cos(n phi - d_n)), dE/dphi ----------
 __device__ __forceinline__ void cosine_series(const float *c, const float *d, double phi, double &e, double &de) {
 #pragma unroll

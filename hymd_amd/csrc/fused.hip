@@ -17,6 +17,7 @@
 // FP64 additions of the same force.
 #include <algorithm>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "bonded.h"
@@ -58,8 +59,11 @@ inline size_t fused_lds_bytes(int particles, int bonds, int angles) {
            sizeof(int) * ((size_t)particles + 1) + sizeof(uint32_t) * (2 * (size_t)bonds + 3 * (size_t)angles);
 }
 
-template <int PPT>
+// F32: bonds and angles evaluated in FP32 (slots and the gather in FP32),
+// kicks, drifts and positions in FP64 as always
+template <int PPT, bool F32>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT == 1 ? 4 : 3))) fused_inner_kernel(FusedArgs A) {
+    using R = typename std::conditional<F32, float, double>::type;
     extern __shared__ double smem_d[];
     const int blk = blockIdx.x;
     const int p0 = A.blk_start[blk];
@@ -70,8 +74,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT
     // LDS, laid out for this block's own counts: positions, bond slots (fa),
     // angle slots (fa, fc), entry offsets, entries
     double *xs = smem_d;                    // 3 * np
-    double *sb = xs + 3 * np;               // 3 * nb
-    double *sa = sb + 3 * nb;               // 6 * na
+    R *sb = (R *)(xs + 3 * np);             // 3 * nb
+    R *sa = sb + 3 * nb;                    // 6 * na
     int *coff = (int *)(sa + 6 * na);       // np + 1
     uint32_t *ent = (uint32_t *)(coff + np + 1);  // ne = 2 nb + 3 na
     for (int j = threadIdx.x; j <= np; j += blockDim.x) coff[j] = A.csr_off[p0 + j] - e0;
@@ -146,11 +150,18 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT
                 const V3 rab = mic(v3(xs[3 * ib] - xs[3 * ia], xs[3 * ib + 1] - xs[3 * ia + 1],
                                       xs[3 * ib + 2] - xs[3 * ia + 2]),
                                    box);
-                double en;
-                const V3 fa = bond_term(rab, bpar[k].x, bpar[k].y, en);
-                sb[3 * tb] = fa.x;
-                sb[3 * tb + 1] = fa.y;
-                sb[3 * tb + 2] = fa.z;
+                if constexpr (F32) {
+                    const F3 fa = bond_term_f(to_f3(rab), (float)bpar[k].x, (float)bpar[k].y);
+                    sb[3 * tb] = fa.x;
+                    sb[3 * tb + 1] = fa.y;
+                    sb[3 * tb + 2] = fa.z;
+                } else {
+                    double en;
+                    const V3 fa = bond_term(rab, bpar[k].x, bpar[k].y, en);
+                    sb[3 * tb] = fa.x;
+                    sb[3 * tb + 1] = fa.y;
+                    sb[3 * tb + 2] = fa.z;
+                }
             }
             if (tb < na) {
                 const int ia = (int)(aabc[k] & 0xffff), ib = (int)((aabc[k] >> 16) & 0xffff),
@@ -158,15 +169,26 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT
                 const V3 rb = v3(xs[3 * ib], xs[3 * ib + 1], xs[3 * ib + 2]);
                 const V3 ra = mic(v3(xs[3 * ia], xs[3 * ia + 1], xs[3 * ia + 2]) - rb, box);
                 const V3 rc = mic(v3(xs[3 * ic], xs[3 * ic + 1], xs[3 * ic + 2]) - rb, box);
-                V3 fa, fc;
-                double en;
-                angle_term(ra, rc, apar[k].x, apar[k].y, fa, fc, en);
-                sa[6 * tb] = fa.x;
-                sa[6 * tb + 1] = fa.y;
-                sa[6 * tb + 2] = fa.z;
-                sa[6 * tb + 3] = fc.x;
-                sa[6 * tb + 4] = fc.y;
-                sa[6 * tb + 5] = fc.z;
+                if constexpr (F32) {
+                    F3 fa, fc;
+                    angle_term_f(ra, rc, (float)apar[k].x, (float)apar[k].y, fa, fc);
+                    sa[6 * tb] = fa.x;
+                    sa[6 * tb + 1] = fa.y;
+                    sa[6 * tb + 2] = fa.z;
+                    sa[6 * tb + 3] = fc.x;
+                    sa[6 * tb + 4] = fc.y;
+                    sa[6 * tb + 5] = fc.z;
+                } else {
+                    V3 fa, fc;
+                    double en;
+                    angle_term(ra, rc, apar[k].x, apar[k].y, fa, fc, en);
+                    sa[6 * tb] = fa.x;
+                    sa[6 * tb + 1] = fa.y;
+                    sa[6 * tb + 2] = fa.z;
+                    sa[6 * tb + 3] = fc.x;
+                    sa[6 * tb + 4] = fc.y;
+                    sa[6 * tb + 5] = fc.z;
+                }
             }
         }
         __syncthreads();
@@ -175,16 +197,16 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT
         for (int k = 0; k < PPT; ++k) {
             const int j = threadIdx.x + k * kBlock;
             if (j >= np) continue;
-            double acc[3] = {0.0, 0.0, 0.0};
+            R acc[3] = {R(0), R(0), R(0)};
             for (int q = coff[j]; q < coff[j + 1]; ++q) {
                 const uint32_t e = ent[q];
                 const int role = e & 3, kind = (e >> 2) & 1, t = (int)(e >> 3);
                 if (kind == 0) {
-                    const double s = role == 0 ? -1.0 : 1.0;
+                    const R s = role == 0 ? R(-1) : R(1);
                     for (int d = 0; d < 3; ++d) acc[d] += s * sb[3 * t + d];
                 } else {
                     for (int d = 0; d < 3; ++d) {
-                        const double fa = sa[6 * t + d], fc = sa[6 * t + 3 + d];
+                        const R fa = sa[6 * t + d], fc = sa[6 * t + 3 + d];
                         acc[d] += role == 0 ? -fa : (role == 2 ? -fc : fa + fc);
                     }
                 }
@@ -269,6 +291,7 @@ struct hymd_fused {
     int max_bonds = 0, max_angles = 0, max_entries = 0, max_particles = 0;
     int64_t single_tail = 0;  // rows [single_tail, n) are molecules of one particle (no terms)
     size_t lds = 0;
+    bool f32 = false;         // bond / angle terms in FP32
     DevBuf blk_start, blk_bond_off, bond_ab, bond_par, blk_ang_off, ang_abc, ang_par, csr_off, csr_ent;
     DevBuf partials, result;
     HostBuf host;
@@ -526,8 +549,13 @@ int fused_inner_impl(hymd_fused *f, double *x, double *v, float *x32, float *f_f
     if (key) A.key = *key;
     else A.key.keys = nullptr;
     if (f->nblocks > 0) {
-        if (f->ppt == 1) fused_inner_kernel<1><<<f->nblocks, kBlock, f->lds, s>>>(A);
-        else fused_inner_kernel<2><<<f->nblocks, kBlock, f->lds, s>>>(A);
+        if (f->f32) {
+            if (f->ppt == 1) fused_inner_kernel<1, true><<<f->nblocks, kBlock, f->lds, s>>>(A);
+            else fused_inner_kernel<2, true><<<f->nblocks, kBlock, f->lds, s>>>(A);
+        } else {
+            if (f->ppt == 1) fused_inner_kernel<1, false><<<f->nblocks, kBlock, f->lds, s>>>(A);
+            else fused_inner_kernel<2, false><<<f->nblocks, kBlock, f->lds, s>>>(A);
+        }
     }
     HYMD_HIP(hipGetLastError());
     if (red) {
@@ -548,6 +576,13 @@ int fused_inner_impl(hymd_fused *f, double *x, double *v, float *x32, float *f_f
     return HYMD_OK;
 }
 }  // namespace hymd
+
+extern "C" int hymd_fused_set_precision(hymd_fused *f, int32_t precision) {
+    HYMD_REQUIRE(f, "null handle");
+    HYMD_REQUIRE(precision == HYMD_F32 || precision == HYMD_F64, "precision must be HYMD_F32 or HYMD_F64");
+    f->f32 = precision == HYMD_F32;
+    return HYMD_OK;
+}
 
 extern "C" int hymd_fused_inner(hymd_fused *f, double *x, double *v, float *x32, float *f_fast, const float *f_slow,
                                 double h_in, double h_out, double dt, int32_t n_inner, int32_t pending,

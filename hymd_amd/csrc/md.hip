@@ -34,6 +34,7 @@ struct hymd_md {
     int respa_inner = 1;
     hymd_fused *fused = nullptr;  // fused inner loop (null: per-kernel path)
     int pending = 0;              // fused: outer half kick of the last step not yet applied
+    int term_precision = HYMD_F64; // fused inner loop: bond / angle terms in FP64 or FP32
     // peptide terms (optional): PME of the charges, dihedrals with dipoles
     // whose charges go through PME and back onto the backbone
     const float *charges = nullptr;
@@ -413,10 +414,21 @@ extern "C" int hymd_md_set_fused(hymd_md *m, int32_t enable) {
     if (m->fused) return HYMD_OK;
     const TopoTerms t = topology_terms(m->topo);
     HYMD_REQUIRE(t.n == m->n, "topology and system sizes differ");
-    return hymd_fused_create(&m->fused, t.n, t.m2, t.a2, t.b2, t.r0, t.k2, t.m3, t.a3, t.b3, t.c3, t.t0, t.k3);
+    HYMD_TRY(hymd_fused_create(&m->fused, t.n, t.m2, t.a2, t.b2, t.r0, t.k2, t.m3, t.a3, t.b3, t.c3, t.t0, t.k3));
+    return hymd_fused_set_precision(m->fused, m->term_precision);
 }
 
 extern "C" int hymd_md_is_fused(hymd_md *m) { return (m && m->fused) ? 1 : 0; }
+
+extern "C" int hymd_md_set_term_precision(hymd_md *m, int32_t precision) {
+    HYMD_REQUIRE(m, "null handle");
+    HYMD_REQUIRE(precision == HYMD_F32 || precision == HYMD_F64, "precision must be HYMD_F32 or HYMD_F64");
+    HYMD_HIP(hipStreamSynchronize(m->work));
+    m->term_precision = precision;
+    drop_graphs(m);  // the captured steps hold the other kernel
+    if (m->fused) HYMD_TRY(hymd_fused_set_precision(m->fused, precision));
+    return HYMD_OK;
+}
 
 // Every `every` outer steps (0: never) the trailing one-particle molecules
 // are put in the mesh-row order of their current positions.  types (the

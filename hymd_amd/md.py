@@ -49,9 +49,14 @@ class RespaMD:
     many outer steps, as they diffuse; ``x``, ``v``, ``types``, the forces and
     the order maps are permuted in place, the physics is unchanged (the field
     path's sums are order-independent and those rows carry no bonded terms).
+    ``term_precision`` sets the arithmetic of the bond and angle terms in the
+    library's fused inner loop: "fp32" (default) as the reference's default
+    real(4) kernels, "fp64" as its --double-precision kernels; kicks, drifts,
+    positions, velocities and energies are FP64 either way, and the Python
+    ``step()`` path evaluates the terms in FP64.
     """
 
-    def __init__(self, system, stream=None, spatial_order=True, reorder_every=25):
+    def __init__(self, system, stream=None, spatial_order=True, reorder_every=25, term_precision="fp32"):
         _lib.require_gpu()
         self.system = system
         if spatial_order:
@@ -75,6 +80,9 @@ class RespaMD:
         self._order = torch.as_tensor(np.asarray(order, dtype=np.int64), device=dev)
         self._inverse = torch.as_tensor(np.asarray(inverse, dtype=np.int64), device=dev)
         self.reorder_every = int(reorder_every) if spatial_order else 0
+        if term_precision not in ("fp32", "fp64"):
+            raise ValueError("term_precision must be 'fp32' or 'fp64'")
+        self.term_precision = term_precision
         self.x = torch.as_tensor(np.ascontiguousarray(s.positions, dtype=np.float64), device=dev)
         self.x = torch.remainder(self.x, torch.as_tensor(self.box, device=dev)).contiguous()
         self.v = torch.as_tensor(np.ascontiguousarray(s.velocities, dtype=np.float64), device=dev)
@@ -151,6 +159,7 @@ class RespaMD:
                  dptr(self.force_mesh), darr(np.ravel(self.A)), darr(self.b), float(self.c0),
                  darr(self.type_mass), self.sigma, self.mass, self.dt, self.n_inner, self._box_c)
             self._native = h
+            call("hymd_md_set_term_precision", h, _lib.HYMD_F32 if self.term_precision == "fp32" else _lib.HYMD_F64)
             if self.reorder_every > 0 and _lib.lib().hymd_md_is_fused(h):
                 call("hymd_md_set_reorder", h, self.reorder_every, dptr(self.types, torch.int32, "types"),
                      dptr(self._order, torch.int64, "order"), dptr(self._inverse, torch.int64, "where"))

@@ -242,6 +242,11 @@ int hymd_fused_destroy(hymd_fused *fused);
  * (the second half kick of one sub-step and the first of the next are separate
  * additions); x32 = float(x).  energies host double[2], virials host double[6]
  * of the last sub-step, or NULL. */
+/* bond / angle terms of the fused inner loop in FP32 (HYMD_F32, the
+ * reference's default real(4) kernels compute_bond_forces.f90 /
+ * compute_angle_forces.f90) or FP64 (HYMD_F64, the __double kernels; the
+ * default); kicks, drifts, positions and the energy pass stay FP64 */
+int hymd_fused_set_precision(hymd_fused *fused, int32_t precision);
 int hymd_fused_inner(hymd_fused *fused, double *x, double *v, float *x32, float *f_fast, const float *f_slow,
                      double h_in, double h_out, double dt, int32_t n_inner, int32_t pending, const double box[3],
                      double *energies, double *virials, void *stream);
@@ -367,6 +372,8 @@ int hymd_md_destroy(hymd_md *md);
 /* use the fused inner loop (1) or the per-kernel inner loop (0); returns
  * HYMD_ERR_UNSUPPORTED if the topology does not allow fusing */
 int hymd_md_set_fused(hymd_md *md, int32_t enable);
+/* term precision of the fused inner loop (hymd_fused_set_precision); synchronises */
+int hymd_md_set_term_precision(hymd_md *md, int32_t precision);
 /* 1 if the fused inner loop is active */
 int hymd_md_is_fused(hymd_md *md);
 /* Every `every` outer steps (0: never), put the trailing one-particle

@@ -242,8 +242,8 @@ def test_fused_inner_loop_matches_per_kernel_path(hip, make):
         s = synth.lipid_water(n_chains=3000, n_water=20000, box=12.0, mesh=32, seed=7)
     else:
         s = synth.diblock_melt(n_chains=500, box=7.5, mesh=32, seed=1000)
-    a = RespaMD(s)
-    b = RespaMD(s)
+    a = RespaMD(s, term_precision="fp64")
+    b = RespaMD(s, term_precision="fp64")
     a.set_fused(False)
     assert b.fused and not a.fused
     # one outer step: the same additions on the same per-term FP64 values up
@@ -284,8 +284,9 @@ def test_fused_rejects_non_contiguous_molecules(hip):
         md.set_fused(True)
     md.run(2)  # the per-kernel path still runs
     # the spatial layout keeps every molecule whole and contiguous, so the
-    # same input runs fused, with the same trajectory in input order
-    sp = RespaMD(s)
+    # same input runs fused, with the same trajectory in input order (FP64
+    # terms, as the per-kernel path evaluates them)
+    sp = RespaMD(s, term_precision="fp64")
     assert sp.fused
     sp.run(2)
     torch.cuda.synchronize()
@@ -351,5 +352,39 @@ def test_single_step_graph_replays_equal_eager_steps(hip):
     b.run(3, use_graph=False)
     torch.cuda.synchronize()
     assert torch.equal(a.x, b.x) and torch.equal(a.v, b.v) and torch.equal(a.f_field, b.f_field)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("make", ["lipid", "diblock"])
+def test_fp32_terms_follow_the_fp64_loop(hip, make):
+    """FP32 bond / angle terms (the reference's default real(4) kernels) in the
+    fused loop: fast forces within FP32 rounding of the FP64 loop's after one
+    outer step; positions below the float32 resolution the field path sees
+    (about 1e-6 nm at these box sizes) after one outer step and within 1e-5
+    nm after eight."""
+    from hymd_amd import synth
+    from hymd_amd.md import RespaMD
+    if make == "lipid":
+        s = synth.lipid_water(n_chains=3000, n_water=20000, box=12.0, mesh=32, seed=7)
+    else:
+        s = synth.diblock_melt(n_chains=500, box=7.5, mesh=32, seed=1000)
+    a = RespaMD(s, term_precision="fp64")
+    b = RespaMD(s, term_precision="fp32")
+    assert a.fused and b.fused
+    a.run(1, use_graph=False)
+    b.run(1, use_graph=False)
+    torch.cuda.synchronize()
+    assert rms_rel(b.f_fast.cpu().numpy(), a.f_fast.cpu().numpy()) < 1e-6
+    assert (a.x - b.x).abs().max().item() < 1e-7
+    a.run(7, use_graph=True)
+    b.run(7, use_graph=True)
+    ea = a.run(1, energies=True)
+    eb = b.run(1, energies=True)
+    torch.cuda.synchronize()
+    dx = (a.x - b.x).abs().max().item()
+    assert dx < 1e-5, dx
+    for k in ("bond", "angle", "kinetic"):
+        assert eb[k] == pytest.approx(ea[k], rel=1e-5)
     a.close()
     b.close()
