@@ -74,6 +74,10 @@ SIGNATURES = {
     "hymd_md_time_stage": [_p, _i32, _i32],
     "hymd_md_set_fused": [_p, _i32],
     "hymd_md_set_term_precision": [_p, _i32],
+    "hymd_md_set_thermostat": [_p, _i32, _pi32, _i32, _pi64, _d, _d, _d, _i32, _i32, _p, _p, _i64],
+    "hymd_md_push_draws": [_p, _pd, _pd, _i32],
+    "hymd_md_set_com_removal": [_p, _i32, _i64, _p, _i64],
+    "hymd_md_set_step": [_p, _i64],
     "hymd_fused_set_precision": [_p, _i32],
     "hymd_md_is_fused": [_p],
     "hymd_md_set_reorder": [_p, _i32, _p, _p, _p],

@@ -282,6 +282,35 @@ TopoTerms topology_terms(const hymd_topology *t);
 // this handle (same positions and types)
 int field_readout_last(hymd_pm *pm, int n_types, const float *force_mesh, float *force);
 
+// CSVR thermostat (thermostat.hip): the groups of one call, in the
+// reference's order, and the tables fixed for a particle set.
+struct CsvrGroups {
+    int n_groups;
+    int n_com;                            // COM classes; class n_com = "none"
+    int g2;                               // COM classes padded to the moments kernel's G
+    int nb;                               // workgroups of the moments kernel
+    int cls[HYMD_MAX_TYPES];              // COM class of group g, -1: all-particle update
+    double n[HYMD_MAX_TYPES];             // particles in group g
+    double R[HYMD_MAX_TYPES];             // N(0,1) draw of group g
+    double S[HYMD_MAX_TYPES];             // chi^2(3 n_g - 1) draw of group g
+    double half_mass, kt_per_particle, c; // m/2, 3/2 R_gas T_0, exp(-dt n_inner / tau)
+};
+struct CsvrSetup {
+    CsvrGroups gp{};
+    uint64_t tab = 0;  // class of each type, 4 bits per type
+};
+int csvr_setup(CsvrSetup *cs, int32_t n_types, const int32_t *type_group, int32_t n_groups, const int64_t *group_n,
+               double mass, double gas_constant, double target_temperature, double c, int32_t remove_com);
+// Thermostat v (FP64 (n,3)) in place on stream s with the draws of one call;
+// with fkick, the velocities are first completed by the pending outer half
+// kick v + ckick * fkick (float32 (n,3)), which the scale pass writes back.
+int csvr_enqueue(const CsvrSetup &cs, double *v, const int32_t *types, int64_t n, const double *gaussians,
+                 const double *chi_squares, const float *fkick, double ckick, double *work, double *ws,
+                 hipStream_t s);
+// v -= sum(v) / n_particles
+int com_enqueue(double *v, int64_t n, int64_t n_particles, double *ws, hipStream_t s);
+int64_t thermostat_workspace_doubles();
+
 // Row-binning keys written by a producer kernel instead of the field path's
 // own key pass: keys[i] = (mesh row of particle i's CIC base cell, its rank
 // among that row's particles), counts[row] incremented once per particle.

@@ -35,6 +35,18 @@ struct hymd_md {
     hymd_fused *fused = nullptr;  // fused inner loop (null: per-kernel path)
     int pending = 0;              // fused: outer half kick of the last step not yet applied
     int term_precision = HYMD_F64; // fused inner loop: bond / angle terms in FP64 or FP32
+    // coupling after outer steps (main.py:1314-1321): CSVR thermostat every
+    // n_b steps, COM momentum removal every com_every steps; `step` counts the
+    // outer steps done, draws holds 2 n_groups numbers (R, then chi^2) per call
+    int64_t step = 0;
+    int n_b = 0;
+    CsvrSetup thermo;
+    double *thermo_work = nullptr, *thermo_ws = nullptr;
+    std::vector<double> draws;
+    size_t draw_next = 0;
+    int com_every = 0;
+    int64_t com_n = 0;
+    double *com_ws = nullptr;
     // peptide terms (optional): PME of the charges, dihedrals with dipoles
     // whose charges go through PME and back onto the backbone
     const float *charges = nullptr;
@@ -365,8 +377,41 @@ static int finish_pending(hymd_md *m) {
 // and makes `stream` wait for them.  use_graph != 0 replays a captured graph of
 // one step.  energies (host double[5]) or NULL: energies after the last step
 // (that step runs eagerly).  Velocities are complete when the call returns.
+// after every outer step: thermostat (the pending outer half kick folded into
+// its passes), then COM momentum removal, each on its period
+static int couple_step(hymd_md *m) {
+    ++m->step;
+    const bool th = m->n_b > 0 && m->step % m->n_b == 0;
+    const bool com = m->com_every > 0 && m->step % m->com_every == 0;
+    if (th) {
+        const int G = m->thermo.gp.n_groups;
+        const double *R = m->draws.data() + m->draw_next * 2 * G;
+        const bool pend = m->fused && m->pending;
+        const double half_out = 0.5 * m->respa_inner * m->dt / m->mass;
+        HYMD_TRY(csvr_enqueue(m->thermo, m->v, m->types, m->n, R, R + G, pend ? m->f_field : nullptr,
+                              pend ? half_out : 0.0, m->thermo_work, m->thermo_ws, m->work));
+        m->pending = 0;
+        if (++m->draw_next * 2 * G >= m->draws.size()) {
+            m->draws.clear();
+            m->draw_next = 0;
+        }
+    }
+    if (com) {
+        HYMD_TRY(finish_pending(m));
+        HYMD_TRY(com_enqueue(m->v, m->n, m->com_n, m->com_ws, m->work));
+    }
+    return HYMD_OK;
+}
+
 extern "C" int hymd_md_step(hymd_md *m, int32_t n_steps, int32_t use_graph, double *energies, void *stream) {
     HYMD_REQUIRE(m && n_steps >= 0, "bad argument");
+    if (m->n_b > 0) {
+        const int64_t need = (m->step + n_steps) / m->n_b - m->step / m->n_b;
+        const int G = m->thermo.gp.n_groups;
+        const int64_t have = (int64_t)(m->draws.size() / (2 * G)) - (int64_t)m->draw_next;
+        HYMD_REQUIRE(need <= have, "the thermostat needs %lld draw sets for these steps, %lld were pushed",
+                     (long long)need, (long long)have);
+    }
     hipStream_t caller = (hipStream_t)stream;
     HYMD_HIP(hipEventRecord(m->ev_in, caller));
     HYMD_HIP(hipStreamWaitEvent(m->work, m->ev_in, 0));
@@ -384,12 +429,14 @@ extern "C" int hymd_md_step(hymd_md *m, int32_t n_steps, int32_t use_graph, doub
             HYMD_TRY(reorder_singles(m));
             m->since_reorder = 0;
         }
+        HYMD_TRY(couple_step(m));
     }
     if (energies && n_steps > 0) {
         std::memset(energies, 0, 5 * sizeof(double));
         HYMD_TRY(enqueue_step(m, energies, m->pending));
         if (m->fused) m->pending = 1;
         HYMD_TRY(finish_pending(m));
+        HYMD_TRY(couple_step(m));
         HYMD_TRY(hymd_kinetic_energy(m->v, m->n, HYMD_F64, m->mass, &energies[3], m->work));
     }
     HYMD_TRY(finish_pending(m));
@@ -419,6 +466,58 @@ extern "C" int hymd_md_set_fused(hymd_md *m, int32_t enable) {
 }
 
 extern "C" int hymd_md_is_fused(hymd_md *m) { return (m && m->fused) ? 1 : 0; }
+
+extern "C" int hymd_md_set_thermostat(hymd_md *m, int32_t n_types, const int32_t *type_group, int32_t n_groups,
+                                      const int64_t *group_n, double gas_constant, double target_temperature,
+                                      double c, int32_t remove_com, int32_t n_b, double *work, void *workspace,
+                                      int64_t workspace_bytes) {
+    HYMD_REQUIRE(m && n_b >= 0, "bad argument");
+    m->draws.clear();
+    m->draw_next = 0;
+    if (n_b == 0) {
+        m->n_b = 0;
+        return HYMD_OK;
+    }
+    HYMD_REQUIRE(n_types == m->T, "n_types differs from the handle's");
+    HYMD_REQUIRE(work && workspace && workspace_bytes >= thermostat_workspace_doubles() * (int64_t)sizeof(double),
+                 "the thermostat needs its work accumulator and a workspace of hymd_thermostat_workspace_bytes()");
+    HYMD_TRY(csvr_setup(&m->thermo, n_types, type_group, n_groups, group_n, m->mass, gas_constant,
+                        target_temperature, c, remove_com));
+    m->thermo_work = work;
+    m->thermo_ws = (double *)workspace;
+    m->n_b = n_b;
+    return HYMD_OK;
+}
+
+extern "C" int hymd_md_push_draws(hymd_md *m, const double *gaussians, const double *chi_squares, int32_t count) {
+    HYMD_REQUIRE(m && m->n_b > 0, "no thermostat is set");
+    HYMD_REQUIRE(count >= 0 && (count == 0 || (gaussians && chi_squares)), "bad draws");
+    const int G = m->thermo.gp.n_groups;
+    for (int c = 0; c < count; ++c) {
+        m->draws.insert(m->draws.end(), gaussians + (size_t)c * G, gaussians + (size_t)(c + 1) * G);
+        m->draws.insert(m->draws.end(), chi_squares + (size_t)c * G, chi_squares + (size_t)(c + 1) * G);
+    }
+    return HYMD_OK;
+}
+
+extern "C" int hymd_md_set_com_removal(hymd_md *m, int32_t every, int64_t n_particles, void *workspace,
+                                       int64_t workspace_bytes) {
+    HYMD_REQUIRE(m && every >= 0, "bad argument");
+    if (every > 0)
+        HYMD_REQUIRE(n_particles > 0 && workspace &&
+                         workspace_bytes >= thermostat_workspace_doubles() * (int64_t)sizeof(double),
+                     "COM removal needs n_particles and a workspace of hymd_thermostat_workspace_bytes()");
+    m->com_every = every;
+    m->com_n = n_particles;
+    m->com_ws = (double *)workspace;
+    return HYMD_OK;
+}
+
+extern "C" int hymd_md_set_step(hymd_md *m, int64_t step) {
+    HYMD_REQUIRE(m && step >= 0, "bad argument");
+    m->step = step;
+    return HYMD_OK;
+}
 
 extern "C" int hymd_md_set_term_precision(hymd_md *m, int32_t precision) {
     HYMD_REQUIRE(m, "null handle");

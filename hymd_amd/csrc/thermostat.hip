@@ -47,6 +47,7 @@ template <int G>
 __global__ void __launch_bounds__(kBlock) csvr_moments_kernel(const double *__restrict__ v,
                                                                const int32_t *__restrict__ types, int64_t n,
                                                                uint64_t cls_tab, const double *__restrict__ shift,
+                                                               const float *__restrict__ fk, double ck,
                                                                double *__restrict__ partials) {
     constexpr int NV = 4 * G + 1;
     double acc[NV], sh[3 * G > 0 ? 3 * G : 1];
@@ -67,6 +68,11 @@ __global__ void __launch_bounds__(kBlock) csvr_moments_kernel(const double *__re
             px[u] = ok ? v[3 * i] : 0.0;
             py[u] = ok ? v[3 * i + 1] : 0.0;
             pz[u] = ok ? v[3 * i + 2] : 0.0;
+            if (fk && ok) {  // the pending outer half kick, as the kick kernel adds it
+                px[u] = px[u] + ck * (double)fk[3 * i];
+                py[u] = py[u] + ck * (double)fk[3 * i + 1];
+                pz[u] = pz[u] + ck * (double)fk[3 * i + 2];
+            }
             pc[u] = !ok ? -1 : (G > 0 && types ? type_class(cls_tab, types[i]) : 0);
         }
 #pragma unroll
@@ -110,18 +116,6 @@ __global__ void __launch_bounds__(kBlock) csvr_moments_kernel(const double *__re
     }
 }
 
-// The groups of one call, in the reference's order.
-struct CsvrGroups {
-    int n_groups;
-    int n_com;                            // COM classes; class n_com = "none"
-    int g2;                               // COM classes padded to the moments kernel's G
-    int nb;                               // workgroups of the moments kernel
-    int cls[kThermoMaxGroups];            // COM class of group g, -1: all-particle update
-    double n[kThermoMaxGroups];           // particles in group g
-    double R[kThermoMaxGroups];           // N(0,1) draw of group g
-    double S[kThermoMaxGroups];           // chi^2(3 n_g - 1) draw of group g
-    double half_mass, kt_per_particle, c; // m/2, 3/2 R_gas T_0, exp(-dt n_inner / tau)
-};
 
 // Fixed-order column sums of the partials (one wavefront per column, lanes
 // over rows, then a shuffle tree) into sums[] (LDS) and ws[kWsSums..].
@@ -211,6 +205,7 @@ __global__ void __launch_bounds__(1024) csvr_solve_kernel(CsvrGroups gp, double 
 __global__ void __launch_bounds__(kBlock) csvr_scale_kernel(double *__restrict__ v,
                                                              const int32_t *__restrict__ types, int64_t n,
                                                              uint64_t cls_tab, int n_com,
+                                                             const float *__restrict__ fk, double ck,
                                                              const double *__restrict__ ws) {
     __shared__ double coef[4 * kThermoClasses];
     for (int q = threadIdx.x; q < 4 * (n_com + 1); q += blockDim.x) coef[q] = ws[kWsCoef + q];
@@ -219,9 +214,15 @@ __global__ void __launch_bounds__(kBlock) csvr_scale_kernel(double *__restrict__
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const int c = n_com > 0 ? type_class(cls_tab, types[i]) : 0;
         const double a = coef[4 * c];
-        v[3 * i] = a * v[3 * i] + coef[4 * c + 1];
-        v[3 * i + 1] = a * v[3 * i + 1] + coef[4 * c + 2];
-        v[3 * i + 2] = a * v[3 * i + 2] + coef[4 * c + 3];
+        double vx = v[3 * i], vy = v[3 * i + 1], vz = v[3 * i + 2];
+        if (fk) {
+            vx = vx + ck * (double)fk[3 * i];
+            vy = vy + ck * (double)fk[3 * i + 1];
+            vz = vz + ck * (double)fk[3 * i + 2];
+        }
+        v[3 * i] = a * vx + coef[4 * c + 1];
+        v[3 * i + 1] = a * vy + coef[4 * c + 2];
+        v[3 * i + 2] = a * vz + coef[4 * c + 3];
     }
 }
 
@@ -245,14 +246,14 @@ __global__ void __launch_bounds__(kBlock) com_subtract_kernel(double *__restrict
 }
 
 static int launch_moments(int g, int nb, const double *v, const int32_t *types, int64_t n, uint64_t tab,
-                          const double *shift, double *part, hipStream_t s) {
+                          const double *shift, const float *fk, double ck, double *part, hipStream_t s) {
     switch (g) {
-        case 0: csvr_moments_kernel<0><<<nb, kBlock, 0, s>>>(v, types, n, tab, shift, part); break;
-        case 1: csvr_moments_kernel<1><<<nb, kBlock, 0, s>>>(v, types, n, tab, shift, part); break;
-        case 2: csvr_moments_kernel<2><<<nb, kBlock, 0, s>>>(v, types, n, tab, shift, part); break;
-        case 4: csvr_moments_kernel<4><<<nb, kBlock, 0, s>>>(v, types, n, tab, shift, part); break;
-        case 8: csvr_moments_kernel<8><<<nb, kBlock, 0, s>>>(v, types, n, tab, shift, part); break;
-        case 16: csvr_moments_kernel<16><<<nb, kBlock, 0, s>>>(v, types, n, tab, shift, part); break;
+        case 0: csvr_moments_kernel<0><<<nb, kBlock, 0, s>>>(v, types, n, tab, shift, fk, ck, part); break;
+        case 1: csvr_moments_kernel<1><<<nb, kBlock, 0, s>>>(v, types, n, tab, shift, fk, ck, part); break;
+        case 2: csvr_moments_kernel<2><<<nb, kBlock, 0, s>>>(v, types, n, tab, shift, fk, ck, part); break;
+        case 4: csvr_moments_kernel<4><<<nb, kBlock, 0, s>>>(v, types, n, tab, shift, fk, ck, part); break;
+        case 8: csvr_moments_kernel<8><<<nb, kBlock, 0, s>>>(v, types, n, tab, shift, fk, ck, part); break;
+        case 16: csvr_moments_kernel<16><<<nb, kBlock, 0, s>>>(v, types, n, tab, shift, fk, ck, part); break;
         default: set_error("internal: bad class count %d", g); return HYMD_ERR_INVALID;
     }
     HYMD_HIP(hipGetLastError());
@@ -271,23 +272,20 @@ using namespace hymd;
 
 extern "C" int64_t hymd_thermostat_workspace_bytes(void) { return kWsDoubles * (int64_t)sizeof(double); }
 
-extern "C" int hymd_csvr_thermostat(double *v, const int32_t *types, int64_t n, int32_t n_types,
-                                    const int32_t *type_group, int32_t n_groups, const int64_t *group_n,
-                                    const double *gaussians, const double *chi_squares, double mass,
-                                    double gas_constant, double target_temperature, double c, int32_t remove_com,
-                                    double *work, void *workspace, int64_t workspace_bytes, void *stream) {
-    HYMD_REQUIRE(n >= 0 && (n == 0 || (v && types)), "bad velocity/type arrays");
+namespace hymd {
+
+int csvr_setup(CsvrSetup *cs, int32_t n_types, const int32_t *type_group, int32_t n_groups, const int64_t *group_n,
+               double mass, double gas_constant, double target_temperature, double c, int32_t remove_com) {
+    HYMD_REQUIRE(cs, "NULL setup");
     HYMD_REQUIRE(n_types >= 1 && n_types <= HYMD_MAX_TYPES, "n_types must be in [1, %d]", HYMD_MAX_TYPES);
     HYMD_REQUIRE(n_groups >= 1 && n_groups <= kThermoMaxGroups, "n_groups must be in [1, %d]", kThermoMaxGroups);
-    HYMD_REQUIRE(type_group && group_n && gaussians && chi_squares && work, "NULL argument");
-    HYMD_REQUIRE(workspace && workspace_bytes >= kWsDoubles * (int64_t)sizeof(double),
-                 "workspace must hold hymd_thermostat_workspace_bytes() bytes");
+    HYMD_REQUIRE(type_group && group_n, "NULL argument");
     HYMD_REQUIRE(mass > 0.0 && c >= 0.0 && c <= 1.0, "need mass > 0 and c in [0, 1]");
     for (int g = 0; g < n_groups; ++g)
         HYMD_REQUIRE(group_n[g] > 0, "thermostat coupling group %d has no particles", g);
-
     // COM classes: groups with COM removal and more than one particle
-    CsvrGroups gp{};
+    CsvrGroups &gp = cs->gp;
+    gp = CsvrGroups{};
     gp.n_groups = n_groups;
     gp.half_mass = 0.5 * mass;
     gp.kt_per_particle = 1.5 * gas_constant * target_temperature;
@@ -296,8 +294,6 @@ extern "C" int hymd_csvr_thermostat(double *v, const int32_t *types, int64_t n, 
     for (int g = 0; g < n_groups; ++g) {
         gp.cls[g] = (remove_com && group_n[g] > 1) ? n_com++ : -1;
         gp.n[g] = (double)group_n[g];
-        gp.R[g] = gaussians[g];
-        gp.S[g] = chi_squares[g];
     }
     gp.n_com = n_com;
     uint64_t tab = 0;
@@ -311,35 +307,65 @@ extern "C" int hymd_csvr_thermostat(double *v, const int32_t *types, int64_t n, 
     }
     // a class index must fit 4 bits: with 16 COM classes every type is in one of them
     HYMD_REQUIRE(n_com < 16 || typed_in_com == n_types, "internal: class table overflow");
+    cs->tab = tab;
+    gp.g2 = pow2_classes(n_com);
+    return HYMD_OK;
+}
 
-    hipStream_t s = (hipStream_t)stream;
-    double *ws = (double *)workspace;
+int csvr_enqueue(const CsvrSetup &cs, double *v, const int32_t *types, int64_t n, const double *gaussians,
+                 const double *chi_squares, const float *fkick, double ckick, double *work, double *ws,
+                 hipStream_t s) {
+    CsvrGroups gp = cs.gp;
+    for (int g = 0; g < gp.n_groups; ++g) {
+        gp.R[g] = gaussians[g];
+        gp.S[g] = chi_squares[g];
+    }
     const int nb = grid_for(n > 0 ? n : 1, kBlock, kThermoMaxBlocks);
-    const int g2 = pow2_classes(n_com);
-    gp.g2 = g2;
     gp.nb = nb;
-    HYMD_TRY(launch_moments(g2, nb, v, types, n, tab, ws + kWsShift, ws + kWsPart, s));
+    HYMD_TRY(launch_moments(gp.g2, nb, v, types, n, cs.tab, ws + kWsShift, fkick, ckick, ws + kWsPart, s));
     csvr_solve_kernel<<<1, 1024, 0, s>>>(gp, ws, work);
     HYMD_HIP(hipGetLastError());
-    if (n > 0) csvr_scale_kernel<<<nb, kBlock, 0, s>>>(v, types, n, tab, n_com, ws);
+    if (n > 0) csvr_scale_kernel<<<nb, kBlock, 0, s>>>(v, types, n, cs.tab, gp.n_com, fkick, ckick, ws);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }
 
-extern "C" int hymd_cancel_com_momentum(double *v, int64_t n, int64_t n_particles, void *workspace,
-                                        int64_t workspace_bytes, void *stream) {
+int com_enqueue(double *v, int64_t n, int64_t n_particles, double *ws, hipStream_t s) {
     HYMD_REQUIRE(n >= 0 && (n == 0 || v) && n_particles > 0, "bad argument");
-    HYMD_REQUIRE(workspace && workspace_bytes >= kWsDoubles * (int64_t)sizeof(double),
-                 "workspace must hold hymd_thermostat_workspace_bytes() bytes");
     if (n == 0) return HYMD_OK;
-    hipStream_t s = (hipStream_t)stream;
-    double *ws = (double *)workspace;
     const int nb = grid_for(n, kBlock, kThermoMaxBlocks);
     // one class holding every particle, no shift: columns 0-2 = sum(v)
-    HYMD_TRY(launch_moments(1, nb, v, nullptr, n, 0, nullptr, ws + kWsPart, s));
+    HYMD_TRY(launch_moments(1, nb, v, nullptr, n, 0, nullptr, nullptr, 0.0, ws + kWsPart, s));
     com_sum_kernel<<<1, 1024, 0, s>>>(ws, nb);
     HYMD_HIP(hipGetLastError());
     com_subtract_kernel<<<nb, kBlock, 0, s>>>(v, n, ws + kWsSums, (double)n_particles);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
+}
+
+int64_t thermostat_workspace_doubles() { return kWsDoubles; }
+
+}  // namespace hymd
+
+extern "C" int hymd_csvr_thermostat(double *v, const int32_t *types, int64_t n, int32_t n_types,
+                                    const int32_t *type_group, int32_t n_groups, const int64_t *group_n,
+                                    const double *gaussians, const double *chi_squares, double mass,
+                                    double gas_constant, double target_temperature, double c, int32_t remove_com,
+                                    double *work, void *workspace, int64_t workspace_bytes, void *stream) {
+    HYMD_REQUIRE(n >= 0 && (n == 0 || (v && types)), "bad velocity/type arrays");
+    HYMD_REQUIRE(gaussians && chi_squares && work, "NULL argument");
+    HYMD_REQUIRE(workspace && workspace_bytes >= kWsDoubles * (int64_t)sizeof(double),
+                 "workspace must hold hymd_thermostat_workspace_bytes() bytes");
+    CsvrSetup cs;
+    HYMD_TRY(csvr_setup(&cs, n_types, type_group, n_groups, group_n, mass, gas_constant, target_temperature, c,
+                        remove_com));
+    return csvr_enqueue(cs, v, types, n, gaussians, chi_squares, nullptr, 0.0, work, (double *)workspace,
+                        (hipStream_t)stream);
+}
+
+extern "C" int hymd_cancel_com_momentum(double *v, int64_t n, int64_t n_particles, void *workspace,
+                                        int64_t workspace_bytes, void *stream) {
+    HYMD_REQUIRE(workspace && workspace_bytes >= kWsDoubles * (int64_t)sizeof(double),
+                 "workspace must hold hymd_thermostat_workspace_bytes() bytes");
+    return com_enqueue(v, n, n_particles, (double *)workspace, (hipStream_t)stream);
 }

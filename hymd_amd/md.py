@@ -102,6 +102,7 @@ class RespaMD:
         self.slow_forces()
         self._native = None
         self._thermostat = None
+        self._com_ws = None
 
     @property
     def order(self):
@@ -205,26 +206,47 @@ class RespaMD:
         outer step ``step`` (counted from ``step0`` + 1), the CSVR thermostat
         when ``config.target_temperature`` is set and step % config.n_b == 0,
         then COM momentum removal when step % config.cancel_com_momentum == 0.
-        The thermostat work accumulates on the device in ``thermostat_work``."""
-        from .thermostat import CsvrThermostat, cancel_com_momentum
+        The thermostat's draws for all those calls are taken from ``prng`` in
+        the reference's order first, then the library runs every step and
+        coupling in one call (the thermostat folds the pending outer half kick
+        into its passes).  The work accumulates on the device in
+        ``thermostat_work``."""
+        from .thermostat import CsvrThermostat, _new_workspace
+        h = self._native_handle()
+        n_steps, step0 = int(n_steps), int(step0)
         n_b = int(config.n_b or 1)
         n_com = int(config.cancel_com_momentum or 0)
-        if config.target_temperature and self._thermostat is None:
-            self._thermostat = CsvrThermostat(self.types, config, n_types=self.T)
-        step, end = int(step0), int(step0) + int(n_steps)
-        while step < end:
-            nxt = end
-            if config.target_temperature:
-                nxt = min(nxt, (step // n_b + 1) * n_b)
-            if n_com:
-                nxt = min(nxt, (step // n_com + 1) * n_com)
-            self.run(nxt - step, use_graph=use_graph)
-            step = nxt
-            if config.target_temperature and step % n_b == 0:
-                self._thermostat(self.v, prng)
-            if n_com and step % n_com == 0:
-                cancel_com_momentum(self.v, config)
-        return step
+        dev = self.v.device
+        if config.target_temperature:
+            if self._thermostat is None:
+                self._thermostat = CsvrThermostat(self.types, config, n_types=self.T)
+            th = self._thermostat
+            c = float(np.exp(-(config.time_step * config.respa_inner) / config.tau))
+            call("hymd_md_set_thermostat", h, self.T, th._type_group, th.n_groups, th._group_n,
+                 float(config.gas_constant), float(config.target_temperature), c, 1 if th.remove_com else 0, n_b,
+                 dptr(th.work), dptr(th._ws), th._ws.numel() * 8)
+            calls = (step0 + n_steps) // n_b - step0 // n_b
+            if calls:
+                draws = [th.draw(prng) for _ in range(calls)]
+                R = np.ascontiguousarray([d[0] for d in draws], dtype=np.float64)
+                S = np.ascontiguousarray([d[1] for d in draws], dtype=np.float64)
+                call("hymd_md_push_draws", h, R.ctypes.data_as(_lib._pd), S.ctypes.data_as(_lib._pd), calls)
+        else:
+            call("hymd_md_set_thermostat", h, self.T, None, 0, None, 0.0, 0.0, 0.0, 0, 0, None, None, 0)
+        if n_com:
+            if self._com_ws is None:
+                self._com_ws = _new_workspace(dev)
+            n_particles = int(config.n_particles or self.n)
+            call("hymd_md_set_com_removal", h, n_com, n_particles, dptr(self._com_ws), self._com_ws.numel() * 8)
+        else:
+            call("hymd_md_set_com_removal", h, 0, 0, None, 0)
+        call("hymd_md_set_step", h, step0)
+        try:
+            self.run(n_steps, use_graph=use_graph)
+        finally:   # plain run() calls stay uncoupled
+            call("hymd_md_set_thermostat", h, self.T, None, 0, None, 0.0, 0.0, 0.0, 0, 0, None, None, 0)
+            call("hymd_md_set_com_removal", h, 0, 0, None, 0)
+        return step0 + n_steps
 
     @property
     def thermostat_work(self):

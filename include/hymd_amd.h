@@ -372,6 +372,24 @@ int hymd_md_destroy(hymd_md *md);
 /* use the fused inner loop (1) or the per-kernel inner loop (0); returns
  * HYMD_ERR_UNSUPPORTED if the topology does not allow fusing */
 int hymd_md_set_fused(hymd_md *md, int32_t enable);
+/* Coupling after outer steps, as main.py:1314-1321: after outer step s
+ * (counted by the handle; hymd_md_set_step sets the count of steps done) the
+ * CSVR thermostat when s % n_b == 0 (hymd_csvr_thermostat's arguments; mass
+ * is the handle's; work and workspace as there, kept by the handle; n_b = 0
+ * turns it off), then COM momentum removal when s % every == 0
+ * (hymd_cancel_com_momentum; every = 0 turns it off).  The thermostat folds
+ * the step's pending outer half kick into its passes (same FP64 additions).
+ * push_draws queues `count` calls' numbers (gaussians, chi_squares: count x
+ * n_groups each, in call order); hymd_md_step fails before enqueuing anything
+ * when its steps need more thermostat calls than are queued.  Energies of an
+ * energies step are taken after the coupling (the kinetic energy). */
+int hymd_md_set_thermostat(hymd_md *md, int32_t n_types, const int32_t *type_group, int32_t n_groups,
+                           const int64_t *group_n, double gas_constant, double target_temperature, double c,
+                           int32_t remove_com, int32_t n_b, double *work, void *workspace, int64_t workspace_bytes);
+int hymd_md_push_draws(hymd_md *md, const double *gaussians, const double *chi_squares, int32_t count);
+int hymd_md_set_com_removal(hymd_md *md, int32_t every, int64_t n_particles, void *workspace,
+                            int64_t workspace_bytes);
+int hymd_md_set_step(hymd_md *md, int64_t step);
 /* term precision of the fused inner loop (hymd_fused_set_precision); synchronises */
 int hymd_md_set_term_precision(hymd_md *md, int32_t precision);
 /* 1 if the fused inner loop is active */
