@@ -94,6 +94,10 @@ SIGNATURES = {
                                  _pd, _i32, _pi32, _pi32, _pd, _i32, _pi64, _p],
     "hymd_bond_builder_fill": [_p] + [_p] * 19,
     "hymd_bond_builder_destroy": [_p],
+    "hymd_index_bonds_create": [ctypes.POINTER(_p), _i64, _p, _p, _i32, _pi32, _pi32, _pd, _pi32, _pd, _pi32, _pd,
+                                _pi32, _pi64, _p],
+    "hymd_index_bonds_fill": [_p] + [_p] * 16,
+    "hymd_index_bonds_destroy": [_p],
 }
 _RESTYPES = {"hymd_last_error": ctypes.c_char_p, "hymd_md_single_tail": ctypes.c_int64,
              "hymd_thermostat_workspace_bytes": ctypes.c_int64}

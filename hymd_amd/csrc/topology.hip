@@ -447,3 +447,188 @@ extern "C" int hymd_bond_builder_fill(hymd_bond_builder *b, int32_t *a2, int32_t
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }
+
+// ---------------------------------------------------------------------------
+// prepare_index_based_bonds (force.py:508-570): every molecule copies the
+// term list of its residue template with the indices offset by its first
+// row.  One thread per molecule: counts, exclusive scans, fill.
+// ---------------------------------------------------------------------------
+namespace hymd {
+
+struct IndexTemplates {
+    int n_tmpl;
+    const int *off2, *off3, *off4;  // [n_tmpl + 1] term ranges per template
+    const int *i2, *i3, *i4;        // 0-based indices within the molecule, 2 / 3 / 4 per term
+    const double *p2, *p3;          // (r0, k), (theta0 rad, k)
+    const double *c4;               // [30] per dihedral
+    const int *k4;                  // dih_type
+};
+
+__global__ void __launch_bounds__(kBlock) index_count_kernel(int64_t n_mol, const int *__restrict__ tmpl_of,
+                                                             IndexTemplates t, int *__restrict__ cnt2,
+                                                             int *__restrict__ cnt3, int *__restrict__ cnt4) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < n_mol; m += stride) {
+        const int r = tmpl_of[m];
+        const bool ok = r >= 0 && r < t.n_tmpl;
+        cnt2[m] = ok ? t.off2[r + 1] - t.off2[r] : 0;
+        cnt3[m] = ok ? t.off3[r + 1] - t.off3[r] : 0;
+        cnt4[m] = ok ? t.off4[r + 1] - t.off4[r] : 0;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) index_fill_kernel(int64_t n_mol, const int64_t *__restrict__ first_row,
+                                                            const int *__restrict__ tmpl_of, IndexTemplates t,
+                                                            const int *__restrict__ o2, const int *__restrict__ o3,
+                                                            const int *__restrict__ o4, TopoOut o) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; m < n_mol; m += stride) {
+        const int r = tmpl_of[m];
+        if (r < 0 || r >= t.n_tmpl) continue;
+        const int f = (int)first_row[m];
+        for (int q = t.off2[r], w = o2[m]; q < t.off2[r + 1]; ++q, ++w) {
+            o.a2[w] = f + t.i2[2 * q];
+            o.b2[w] = f + t.i2[2 * q + 1];
+            o.eq2[w] = t.p2[2 * q];
+            o.k2[w] = t.p2[2 * q + 1];
+        }
+        for (int q = t.off3[r], w = o3[m]; q < t.off3[r + 1]; ++q, ++w) {
+            o.a3[w] = f + t.i3[3 * q];
+            o.b3[w] = f + t.i3[3 * q + 1];
+            o.c3[w] = f + t.i3[3 * q + 2];
+            o.eq3[w] = t.p3[2 * q];
+            o.k3[w] = t.p3[2 * q + 1];
+        }
+        for (int q = t.off4[r], w = o4[m]; q < t.off4[r + 1]; ++q, ++w) {
+            o.a4[w] = f + t.i4[4 * q];
+            o.b4[w] = f + t.i4[4 * q + 1];
+            o.c4[w] = f + t.i4[4 * q + 2];
+            o.d4[w] = f + t.i4[4 * q + 3];
+            o.kind4[w] = t.k4[q];
+            o.last4[w] = 0;
+            for (int c = 0; c < 30; ++c) o.coef4[(int64_t)w * 30 + c] = t.c4[30 * q + c];
+        }
+    }
+}
+
+}  // namespace hymd
+
+struct hymd_index_bonds {
+    hipStream_t stream = nullptr;
+    int64_t n_mol = 0;
+    const int64_t *first_row = nullptr;
+    const int32_t *tmpl_of = nullptr;
+    IndexTemplates t{};
+    DevBuf tables, counts, offsets, tiles;
+    int64_t total[3] = {0, 0, 0};
+};
+
+extern "C" int hymd_index_bonds_destroy(hymd_index_bonds *b) {
+    if (!b) return HYMD_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    delete b;
+    return HYMD_OK;
+}
+
+extern "C" int hymd_index_bonds_create(hymd_index_bonds **out, int64_t n_mol, const int64_t *first_row,
+                                       const int32_t *template_of, int32_t n_templates, const int32_t *offsets,
+                                       const int32_t *idx2, const double *par2, const int32_t *idx3,
+                                       const double *par3, const int32_t *idx4, const double *coeff4,
+                                       const int32_t *kind4, int64_t counts[3], void *stream) {
+    HYMD_REQUIRE(out && counts, "NULL output");
+    *out = nullptr;
+    HYMD_REQUIRE(n_mol >= 0 && n_templates >= 0 && offsets, "bad sizes");
+    HYMD_REQUIRE(n_mol == 0 || (first_row && template_of), "NULL molecule arrays");
+    const int nt = n_templates;
+    const int n2 = offsets[nt], n3 = offsets[2 * (nt + 1) - 1], n4 = offsets[3 * (nt + 1) - 1];
+    for (int k = 0; k < 3; ++k)
+        for (int r = 0; r < nt; ++r)
+            HYMD_REQUIRE(offsets[k * (nt + 1) + r] <= offsets[k * (nt + 1) + r + 1] && offsets[k * (nt + 1)] == 0,
+                         "template offsets must start at 0 and not decrease");
+    HYMD_REQUIRE((n2 == 0 || (idx2 && par2)) && (n3 == 0 || (idx3 && par3)) && (n4 == 0 || (idx4 && coeff4 && kind4)),
+                 "NULL template table");
+    auto *b = new hymd_index_bonds();
+    b->stream = (hipStream_t)stream;
+    b->n_mol = n_mol;
+    b->first_row = first_row;
+    b->tmpl_of = template_of;
+    // one staging copy of the small template tables
+    const size_t nd = (size_t)2 * n2 + 2 * n3 + 30 * n4 + 1;
+    const size_t ni = (size_t)3 * (nt + 1) + 2 * n2 + 3 * n3 + 4 * n4 + n4 + 1;
+    int st = b->tables.reserve(sizeof(double) * nd + sizeof(int) * ni);
+    if (st == HYMD_OK) st = b->counts.reserve(sizeof(int) * 3 * (size_t)(n_mol + 1));
+    if (st == HYMD_OK) st = b->offsets.reserve(sizeof(int) * 3 * (size_t)(n_mol + 2));
+    if (st == HYMD_OK) st = b->tiles.reserve(sizeof(int) * (size_t)(scan_tiles(n_mol) + 1));
+    if (st != HYMD_OK) {
+        delete b;
+        return st;
+    }
+    std::vector<double> hd;
+    std::vector<int> hi;
+    hd.insert(hd.end(), par2, par2 + 2 * n2);
+    hd.insert(hd.end(), par3, par3 + 2 * n3);
+    hd.insert(hd.end(), coeff4, coeff4 + 30 * n4);
+    hd.push_back(0.0);
+    hi.insert(hi.end(), offsets, offsets + 3 * (nt + 1));
+    hi.insert(hi.end(), idx2, idx2 + 2 * n2);
+    hi.insert(hi.end(), idx3, idx3 + 3 * n3);
+    hi.insert(hi.end(), idx4, idx4 + 4 * n4);
+    hi.insert(hi.end(), kind4, kind4 + n4);
+    hi.push_back(0);
+    double *td = (double *)b->tables.ptr;
+    int *ti = (int *)(td + hd.size());
+    hipStream_t s = b->stream;
+    HYMD_HIP(hipMemcpyAsync(td, hd.data(), sizeof(double) * hd.size(), hipMemcpyHostToDevice, s));
+    HYMD_HIP(hipMemcpyAsync(ti, hi.data(), sizeof(int) * hi.size(), hipMemcpyHostToDevice, s));
+    IndexTemplates &t = b->t;
+    t.n_tmpl = nt;
+    t.off2 = ti;
+    t.off3 = ti + (nt + 1);
+    t.off4 = ti + 2 * (nt + 1);
+    t.i2 = ti + 3 * (nt + 1);
+    t.i3 = t.i2 + 2 * n2;
+    t.i4 = t.i3 + 3 * n3;
+    t.k4 = t.i4 + 4 * n4;
+    t.p2 = td;
+    t.p3 = td + 2 * n2;
+    t.c4 = td + 2 * n2 + 2 * n3;
+    int tot[3] = {0, 0, 0};
+    if (n_mol > 0) {
+        int *cnt = (int *)b->counts.ptr;
+        int *off = (int *)b->offsets.ptr;
+        index_count_kernel<<<grid_for(n_mol), kBlock, 0, s>>>(n_mol, template_of, t, cnt, cnt + (n_mol + 1),
+                                                              cnt + 2 * (n_mol + 1));
+        HYMD_HIP(hipGetLastError());
+        for (int k = 0; k < 3; ++k)
+            HYMD_TRY(exclusive_scan(cnt + k * (n_mol + 1), off + k * (n_mol + 2), n_mol, (int *)b->tiles.ptr,
+                                    nullptr, false, s));
+        for (int k = 0; k < 3; ++k)
+            HYMD_HIP(hipMemcpyAsync(&tot[k], off + k * (n_mol + 2) + n_mol, sizeof(int), hipMemcpyDeviceToHost, s));
+    }
+    HYMD_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < 3; ++k) b->total[k] = counts[k] = tot[k];
+    *out = b;
+    return HYMD_OK;
+}
+
+extern "C" int hymd_index_bonds_fill(hymd_index_bonds *b, int32_t *a2, int32_t *b2, double *eq2, double *k2,
+                                     int32_t *a3, int32_t *b3, int32_t *c3, double *eq3, double *k3, int32_t *a4,
+                                     int32_t *b4, int32_t *c4, int32_t *d4, double *coeff4, int32_t *kind4,
+                                     int32_t *last4) {
+    HYMD_REQUIRE(b, "NULL handle");
+    HYMD_REQUIRE(b->total[0] == 0 || (a2 && b2 && eq2 && k2), "bond outputs are required");
+    HYMD_REQUIRE(b->total[1] == 0 || (a3 && b3 && c3 && eq3 && k3), "angle outputs are required");
+    HYMD_REQUIRE(b->total[2] == 0 || (a4 && b4 && c4 && d4 && coeff4 && kind4 && last4),
+                 "dihedral outputs are required");
+    if (b->n_mol == 0) return HYMD_OK;
+    const int64_t nm = b->n_mol;
+    const int *off = (const int *)b->offsets.ptr;
+    TopoOut o{};
+    o.a2 = a2, o.b2 = b2, o.eq2 = eq2, o.k2 = k2;
+    o.a3 = a3, o.b3 = b3, o.c3 = c3, o.eq3 = eq3, o.k3 = k3;
+    o.a4 = a4, o.b4 = b4, o.c4 = c4, o.d4 = d4, o.coef4 = coeff4, o.kind4 = kind4, o.last4 = last4;
+    index_fill_kernel<<<grid_for(nm), kBlock, 0, b->stream>>>(nm, b->first_row, b->tmpl_of, b->t, off,
+                                                              off + (nm + 2), off + 2 * (nm + 2), o);
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
+}

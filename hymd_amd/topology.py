@@ -18,7 +18,8 @@ the molecule segments (np.unique order, rows in order) and the type tables
 coefficients to (6, 5), force.py:711, computed as the reference computes
 them); the graph traversal runs on the device, one thread per molecule.
 A bond partner outside its molecule raises KeyError and an ambiguous
-reversed dihedral ValueError, as the reference does.
+reversed dihedral ValueError, as the reference does.  With ``topol`` the
+index-based input of force.py:508-570 is expanded on the device instead.
 """
 import ctypes
 
@@ -70,14 +71,116 @@ def _c(a, ctype):
     return a.ctypes.data_as(ctypes.POINTER(ctype)) if a.size else None, a
 
 
+# Cosine-series coefficient tables of the helix, sheet and coil propensity
+# dihedrals (force.py:280-302): the same numbers as the reference's table.
+_ALPHA = [[7.406, -5.298, -2.570, 1.336, 0.739], [-0.28632126, 1.2099146, 1.18122138, 0.49075168, 0.98495911]]
+_BETA = [[3.770, 5.929, -4.151, -0.846, 0.190], [-0.2300693, -0.0583289, 0.99342396, 1.03237971, 2.90160988]]
+_COIL = [[1.416, -0.739, 0.990, -0.397, 0.136], [1.3495933, 0.45649087, 2.30441057, -0.12274901, -0.26179939]]
+
+
+def propensity_potential_coeffs(x):
+    """force.py:280-322: (4, 5) coefficients for a propensity lambda in
+    [-1, 1]; outside it the reference logs an error and returns None."""
+    alpha, beta, coil = np.array(_ALPHA), np.array(_BETA), np.array(_COIL)
+    zero = np.zeros((2, 5))
+    if np.isclose(x, -1):
+        return np.concatenate((alpha, zero))
+    if np.isclose(x, 0):
+        return np.concatenate((coil, zero))
+    if np.isclose(x, 1):
+        return np.concatenate((beta, zero))
+    if abs(x) > 1:
+        return None
+    coil[0] *= 1 - abs(x)
+    if x < 0:
+        alpha[0] *= 0.5 * (abs(x) - x)
+        return np.concatenate((alpha, coil))
+    beta[0] *= 0.5 * (abs(x) + x)
+    return np.concatenate((beta, coil))
+
+
+def _topol_coeffs(dih_type, coeff):
+    """Coefficients of one topology dihedral entry (force.py:555-567)."""
+    if dih_type == 0 and isinstance(coeff[0], (float, int)):
+        return propensity_potential_coeffs(coeff[0])
+    if dih_type == 1 and len(coeff) == 3:
+        return np.array(propensity_potential_coeffs(coeff[0][0]).tolist() + list(coeff[1:]))
+    if dih_type == 2:
+        return np.array(coeff)
+    return np.insert(np.array(coeff), 2, np.zeros((2, 5)), axis=0)
+
+
+def _prepare_index_based(molecules, topol, dev, stream):
+    """prepare_index_based_bonds (force.py:508-570) + prepare_bonds' arrays on
+    the device: residue templates marshalled on the host, one thread per
+    molecule expands them."""
+    molecules = _host(molecules).astype(np.int64, copy=False).reshape(-1)
+    uniq, first = np.unique(molecules, return_index=True)
+    entries = topol["system"]["molecules"]
+    names = [str(e[0]) for e in entries]
+    cum = np.cumsum([int(e[1]) for e in entries]) if entries else np.zeros(0, np.int64)
+    k = np.searchsorted(cum, uniq + 1, side="left")       # first entry with resid <= cumulative count
+    bad = np.flatnonzero(k >= len(names))
+    stop = int(bad[0]) if len(bad) else len(uniq)        # the reference stops at the first miss
+    used = sorted({names[int(i)] for i in k[:stop]}, key=names.index)
+    tmpl_id = {nm: t for t, nm in enumerate(used)}
+    template_of = np.full(len(uniq), -1, dtype=np.int32)
+    template_of[:stop] = [tmpl_id[names[int(i)]] for i in k[:stop]]
+    off = np.zeros((3, len(used) + 1), dtype=np.int32)
+    i2, p2, i3, p3, i4, c4, k4 = [], [], [], [], [], [], []
+    for t, nm in enumerate(used):
+        res = topol[nm]
+        for b in res.get("bonds", []):
+            i2.append([b[0] - 1, b[1] - 1])
+            p2.append([b[3], b[4]])
+        for a in res.get("angles", []):
+            i3.append([a[0] - 1, a[1] - 1, a[2] - 1])
+            p3.append([np.radians(a[4]), a[5]])
+        for d in res.get("dihedrals", []):
+            i4.append([d[0] - 1, d[1] - 1, d[2] - 1, d[3] - 1])
+            c4.append(np.resize(np.asarray(_topol_coeffs(d[4], d[5]), dtype=np.float64), (6, 5)).ravel())
+            k4.append(int(d[4]))
+        off[:, t + 1] = [len(i2), len(i3), len(i4)]
+    arr = [_c(np.array(i2, np.int32).reshape(-1, 2), ctypes.c_int32), _c(np.array(p2, np.float64).reshape(-1, 2),
+           ctypes.c_double), _c(np.array(i3, np.int32).reshape(-1, 3), ctypes.c_int32),
+           _c(np.array(p3, np.float64).reshape(-1, 2), ctypes.c_double), _c(np.array(i4, np.int32).reshape(-1, 4),
+           ctypes.c_int32), _c(np.array(c4, np.float64).reshape(-1, 30), ctypes.c_double),
+           _c(np.array(k4, np.int32), ctypes.c_int32)]
+    offp, offa = _c(off, ctypes.c_int32)
+    d_first = torch.as_tensor(first.astype(np.int64), device=dev)
+    d_tmpl = torch.as_tensor(template_of, device=dev)
+    counts = (ctypes.c_int64 * 3)()
+    h = ctypes.c_void_p()
+    _lib.call("hymd_index_bonds_create", ctypes.byref(h), len(uniq), dptr(d_first), dptr(d_tmpl), len(used), offp,
+              *[a[0] for a in arr], counts, _lib.stream_ptr(stream))
+    try:
+        nb, na, nd = (int(c) for c in counts)
+        out = [torch.empty(nb, dtype=torch.int32, device=dev), torch.empty(nb, dtype=torch.int32, device=dev),
+               torch.empty(nb, dtype=torch.float64, device=dev), torch.empty(nb, dtype=torch.float64, device=dev),
+               *[torch.empty(na, dtype=torch.int32, device=dev) for _ in range(3)],
+               torch.empty(na, dtype=torch.float64, device=dev), torch.empty(na, dtype=torch.float64, device=dev),
+               *[torch.empty(nd, dtype=torch.int32, device=dev) for _ in range(4)],
+               torch.empty((nd, 6, 5), dtype=torch.float64, device=dev),
+               torch.empty(nd, dtype=torch.int32, device=dev), torch.empty(nd, dtype=torch.int32, device=dev)]
+        _lib.call("hymd_index_bonds_fill", h, *[dptr(t) for t in out])
+        torch.cuda.current_stream().synchronize() if stream is None else stream.synchronize()
+    finally:
+        _lib.lib().hymd_index_bonds_destroy(h)
+    return tuple(out)
+
+
 def prepare_bonds(molecules, names, bonds, indices, config, topol=None, device=None, stream=None,
                   return_types=False):
     """force.py:573-728 on the device (see the module docstring).  With
-    ``return_types`` a 17th item holds the config entry index of every term,
-    (type2, type3, type4)."""
+    ``topol`` (a parsed TOML topology) the terms come from its residue
+    templates (prepare_index_based_bonds, force.py:508-570) and ``names``,
+    ``bonds``, ``indices`` are not used.  With ``return_types`` a 17th item
+    holds the config entry index of every term, (type2, type3, type4)
+    (connectivity path only)."""
     if topol is not None:
-        raise NotImplementedError("the index-based topology input (prepare_index_based_bonds, force.py:508-570) "
-                                  "is not built here; pass the connectivity table")
+        _lib.require_gpu()
+        dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
+        return _prepare_index_based(molecules, topol, dev, stream)
     _lib.require_gpu()
     dev = device if device is not None else torch.device("cuda", torch.cuda.current_device())
     molecules = _host(molecules).astype(np.int64, copy=False)

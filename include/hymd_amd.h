@@ -21,7 +21,8 @@
  *   integrator.py integrate_velocity :9-41, integrate_position :44-73 as
  *              composed by the rRESPA loop main.py:802-887,1143-1169
  *   thermostat.py csvr_thermostat :111-219, cancel_com_momentum :12-15
- *   force.py   prepare_bonds :573-728 (over prepare_bonds_old :325-505)
+ *   force.py   prepare_bonds :573-728 (over prepare_bonds_old :325-505 and
+ *              prepare_index_based_bonds :508-570)
  *   pressure.py comp_pressure :9-179 (field terms, with comp_laplacian :182-200)
  *   barostat.py / barostat_scr.py isotropic, semiisotropic (box and position rescaling)
  */
@@ -338,6 +339,26 @@ int hymd_bond_builder_fill(hymd_bond_builder *builder, int32_t *a2, int32_t *b2,
                            int32_t *d4, double *coeff4, int32_t *kind4, int32_t *last4, int32_t *type2,
                            int32_t *type3, int32_t *type4);
 int hymd_bond_builder_destroy(hymd_bond_builder *builder);
+
+/* prepare_bonds with a topology (prepare_index_based_bonds, force.py:508-570):
+ * molecule m (n_mol, np.unique order) copies the term list of residue
+ * template template_of[m] (device int32; -1 or out of range: no terms) with
+ * its indices offset by first_row[m] (device int64).  Templates (host):
+ * offsets int32 [3][n_templates + 1] (term ranges of bonds, angles,
+ * dihedrals, each from 0), idx2 [B][2] / par2 (r0, k), idx3 [A][3] / par3
+ * (theta0 in radians, k), idx4 [D][4] / coeff4 [D][6][5] / kind4 (dih_type),
+ * 0-based within the molecule.  counts (host) receives the term totals;
+ * synchronises.  fill writes prepare_bonds' 16 arrays (last = 0). */
+typedef struct hymd_index_bonds hymd_index_bonds;
+int hymd_index_bonds_create(hymd_index_bonds **builder, int64_t n_mol, const int64_t *first_row,
+                            const int32_t *template_of, int32_t n_templates, const int32_t *offsets,
+                            const int32_t *idx2, const double *par2, const int32_t *idx3, const double *par3,
+                            const int32_t *idx4, const double *coeff4, const int32_t *kind4, int64_t counts[3],
+                            void *stream);
+int hymd_index_bonds_fill(hymd_index_bonds *builder, int32_t *a2, int32_t *b2, double *eq2, double *k2, int32_t *a3,
+                          int32_t *b3, int32_t *c3, double *eq3, double *k3, int32_t *a4, int32_t *b4, int32_t *c4,
+                          int32_t *d4, double *coeff4, int32_t *kind4, int32_t *last4);
+int hymd_index_bonds_destroy(hymd_index_bonds *builder);
 
 /* ------------------------------------------------------------------------ */
 /* Whole rRESPA outer steps enqueued from C (the MD loop of main.py:764-1169 */

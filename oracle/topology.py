@@ -27,11 +27,23 @@ documented traversal orders, which the reference's results inherit:
 A bond partner outside the molecule has no node attributes in the
 reference (KeyError); here too.
 
+prepare_index_based_bonds (force.py:508-570) with propensity_potential_coeffs
+(force.py:280-322): per molecule (np.unique order; resid = mol + 1 looked
+up in the cumulative counts of topol["system"]["molecules"], stopping at the
+first molecule without a residue) the residue's 1-based term lists offset by
+the molecule's first row; angles in degrees -> np.radians; dihedral
+coefficients from a propensity, inserted zero rows or as given; last = 0.
+
 Pinning: the term lists of test/test_force.py:18-45, 103-127, 189-212 on the
 dppc_single and alanine_octapeptide fixtures (tests/golden/
 topology_goldens.json), and networkx itself on random molecular graphs
 (adjacency order, BFS discovery order and parents, the set of simple 4-node
-paths; tests/test_oracle_topology.py).
+paths; tests/test_oracle_topology.py); the index-based path, which no
+reference test covers, by hand-written expected arrays of a two-residue
+topology, the stop at the first molecule without a residue, and the
+propensity coefficients' closed forms (lambda = -1, 0, 1 give the alpha, coil
+and beta tables; in between the first row of the helix or sheet table is
+scaled by |lambda| and the coil's by 1 - |lambda|).
 """
 import numpy as np
 
@@ -181,6 +193,91 @@ def prepare_bonds(molecules, names, bonds, indices, bond_types, angle_types, dih
     dtype = np.array([t[5] for t in b4], dtype=np.int64)
     last = np.zeros(len(b4), dtype=np.int64)
     last[bb_index] = 1
+    return (a2[:, 0], a2[:, 1], p2[:, 0], p2[:, 1],
+            a3[:, 0], a3[:, 1], a3[:, 2], p3[:, 0], p3[:, 1],
+            a4[:, 0], a4[:, 1], a4[:, 2], a4[:, 3], coeff, dtype, last)
+
+
+# ---------------------------------------------------------------------------
+# index-based topology input (force.py:280-322, 508-570)
+# ---------------------------------------------------------------------------
+_ALPHA = [[7.406, -5.298, -2.570, 1.336, 0.739], [-0.28632126, 1.2099146, 1.18122138, 0.49075168, 0.98495911]]
+_BETA = [[3.770, 5.929, -4.151, -0.846, 0.190], [-0.2300693, -0.0583289, 0.99342396, 1.03237971, 2.90160988]]
+_COIL = [[1.416, -0.739, 0.990, -0.397, 0.136], [1.3495933, 0.45649087, 2.30441057, -0.12274901, -0.26179939]]
+
+
+def propensity_potential_coeffs(x):
+    """force.py:280-322: helical / coil / sheet cosine-series coefficients
+    (4, 5) for a propensity lambda in [-1, 1] (None outside, as the reference
+    logs an error and returns nothing)."""
+    alpha, beta, coil = np.array(_ALPHA), np.array(_BETA), np.array(_COIL)
+    zero = np.zeros((2, 5))
+    if np.isclose(x, -1):
+        return np.concatenate((alpha, zero))
+    if np.isclose(x, 0):
+        return np.concatenate((coil, zero))
+    if np.isclose(x, 1):
+        return np.concatenate((beta, zero))
+    ax = abs(x)
+    if ax > 1:
+        return None
+    coil[0] *= 1 - ax
+    if x < 0:
+        alpha[0] *= 0.5 * (ax - x)
+        return np.concatenate((alpha, coil))
+    beta[0] *= 0.5 * (ax + x)
+    return np.concatenate((beta, coil))
+
+
+def dihedral_coeffs_of_entry(dih_type, coeff):
+    """The coefficient array of one topology dihedral entry (force.py:555-567)."""
+    if dih_type == 0 and isinstance(coeff[0], (float, int)):
+        return propensity_potential_coeffs(coeff[0])
+    if dih_type == 1 and len(coeff) == 3:
+        return np.array(propensity_potential_coeffs(coeff[0][0]).tolist() + list(coeff[1:]))
+    if dih_type == 2:
+        return np.array(coeff)
+    return np.insert(np.array(coeff), 2, np.zeros((2, 5)), axis=0)
+
+
+def prepare_index_based_bonds(molecules, topol):
+    """force.py:508-570: bonds_2, bonds_3, bonds_4 from per-residue term
+    lists with 1-based indices, offset by each molecule's first row."""
+    molecules = np.asarray(molecules)
+    b2, b3, b4 = [], [], []
+    for mol in np.unique(molecules):
+        resid = mol + 1
+        resname, count = None, 0
+        for name, k in topol["system"]["molecules"]:
+            count += k
+            if resid <= count:
+                resname = name
+                break
+        if resname is None:
+            break
+        first = int(np.nonzero(molecules == mol)[0][0])
+        res = topol[resname]
+        for t in res.get("bonds", []):
+            b2.append([t[0] - 1 + first, t[1] - 1 + first, t[3], t[4]])
+        for t in res.get("angles", []):
+            b3.append([t[0] - 1 + first, t[1] - 1 + first, t[2] - 1 + first, np.radians(t[4]), t[5]])
+        for t in res.get("dihedrals", []):
+            b4.append([t[0] - 1 + first, t[1] - 1 + first, t[2] - 1 + first, t[3] - 1 + first,
+                       dihedral_coeffs_of_entry(t[4], t[5]), t[4], 0])
+    return b2, b3, b4
+
+
+def prepare_bonds_topol(molecules, topol):
+    """force.py:573-728 with a topology: the 16 arrays."""
+    b2, b3, b4 = prepare_index_based_bonds(molecules, topol)
+    a2 = np.array([t[:2] for t in b2], dtype=np.int64).reshape(-1, 2)
+    p2 = np.array([t[2:] for t in b2], dtype=np.float64).reshape(-1, 2)
+    a3 = np.array([t[:3] for t in b3], dtype=np.int64).reshape(-1, 3)
+    p3 = np.array([t[3:] for t in b3], dtype=np.float64).reshape(-1, 2)
+    a4 = np.array([t[:4] for t in b4], dtype=np.int64).reshape(-1, 4)
+    coeff = np.array([resize_coeffs(t[4]) for t in b4], dtype=np.float64).reshape(-1, 6, 5)
+    dtype = np.array([t[5] for t in b4], dtype=np.int64)
+    last = np.array([t[6] for t in b4], dtype=np.int64)
     return (a2[:, 0], a2[:, 1], p2[:, 0], p2[:, 1],
             a3[:, 0], a3[:, 1], a3[:, 2], p3[:, 0], p3[:, 1],
             a4[:, 0], a4[:, 1], a4[:, 2], a4[:, 3], coeff, dtype, last)

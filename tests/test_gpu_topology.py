@@ -179,3 +179,53 @@ def test_lipid_water_terms_at_full_size(hip):
     np.testing.assert_array_equal(np.stack([a3, b3, c3]), np.stack(s.angles[:3]))
     np.testing.assert_array_equal(t0_, s.angles[3])
     assert len(out[9]) == 0
+
+
+def _compare_topol(molecules, topol):
+    from hymd_amd.topology import prepare_bonds
+    want = OT.prepare_bonds_topol(molecules, topol)
+    got = prepare_bonds(molecules, None, None, None, None, topol=topol)
+    for k, (g, w) in enumerate(zip(got, want)):
+        g = g.cpu().numpy()
+        assert g.shape == w.shape, (k, g.shape, w.shape)
+        np.testing.assert_array_equal(g, w, err_msg=f"array {k}")
+    return want
+
+
+def test_index_based_topology_matches_oracle(hip):
+    from tests.test_oracle_topology import TOPOL
+    w = _compare_topol(np.array([0, 0, 0, 0, 2, 1, 1, 1, 1, 3]), TOPOL)
+    assert len(w[0]) == 6 and len(w[9]) == 8
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_index_based_topologies(hip, seed):
+    """Residue templates of 1-12 beads with random terms, molecule ids
+    shuffled over the rows, a tail of molecules beyond the topology's count
+    (where the reference stops)."""
+    rng = np.random.default_rng(seed)
+    topol = {"system": {"molecules": []}}
+    for r in range(4):
+        nb = int(rng.integers(1, 12))
+        res = {}
+        if nb > 1:
+            res["bonds"] = [[int(i), int(i) + 1, 1, float(rng.uniform(0.3, 0.5)), 1000.0] for i in range(1, nb)]
+        if nb > 2:
+            res["angles"] = [[int(i), int(i) + 1, int(i) + 2, 1, float(rng.uniform(90, 180)), 25.0]
+                             for i in range(1, nb - 1)]
+        if nb > 3:
+            res["dihedrals"] = [[1, 2, 3, 4, 0, [float(rng.choice([-1.0, -0.4, 0.0, 0.7, 1.0]))]],
+                                [2, 3, 4, 1, 2, rng.normal(size=(2, 5)).tolist()],
+                                [4, 3, 2, 1, 1, [[0.3], [1, 2, 3, 4, 5], [5, 4, 3, 2, 1]]]]
+        topol[f"R{r}"] = res
+        topol["system"]["molecules"].append([f"R{r}", int(rng.integers(1, 40))])
+    sizes = {f"R{r}": 1 + max([t[1] for t in topol[f"R{r}"].get("bonds", [[0, 0]])]) for r in range(4)}
+    mols, rows = [], []
+    mid = 0
+    for name, count in topol["system"]["molecules"]:
+        for _ in range(count):
+            mols += [mid] * sizes[name]
+            mid += 1
+    mols += [mid] * 3 + [mid + 1] * 2                     # beyond the topology
+    molecules = np.array(mols)[rng.permutation(len(mols))] if seed % 2 else np.array(mols)
+    _compare_topol(molecules, topol)

@@ -121,3 +121,55 @@ def test_traversal_orders_match_networkx(seed):
         assert G.bfs2(s) == want
         simple = {tuple(p) for t in H.nodes for p in nx.all_simple_paths(H, s, t, cutoff=3) if len(p) == 4}
         assert set(G.paths4(s)) == simple and len(G.paths4(s)) == len(simple)
+
+
+TOPOL = {
+    "system": {"molecules": [["PEP", 2], ["WAT", 1]]},
+    "PEP": {"bonds": [[1, 2, 1, 0.35, 4000.0], [2, 3, 1, 0.36, 4100.0], [3, 4, 1, 0.37, 4200.0]],
+            "angles": [[1, 2, 3, 1, 127.0, 20.0], [2, 3, 4, 1, 100.0, 25.0]],
+            "dihedrals": [[1, 2, 3, 4, 0, [-1.0]],
+                          [4, 3, 2, 1, 1, [[0.5], [1, 2, 3, 4, 5], [6, 7, 8, 9, 10]]],
+                          [1, 2, 3, 4, 2, [[1, 1, 1, 1, 1], [2, 2, 2, 2, 2]]],
+                          [2, 3, 4, 1, 3, [[1, 0, 0, 0, 0], [0, 1, 0, 0, 0]]]]},
+    "WAT": {},
+}
+
+
+def test_index_based_topology_by_hand():
+    """Two 4-bead peptides (rows 0-3 and 5-8, molecule ids 0 and 1), a water
+    (row 4, id 2) and a particle of molecule 3, which has no residue: the
+    expansion stops there."""
+    molecules = np.array([0, 0, 0, 0, 2, 1, 1, 1, 1, 3])
+    b2, b3, b4 = OT.prepare_index_based_bonds(molecules, TOPOL)
+    assert [t[:2] for t in b2] == [[0, 1], [1, 2], [2, 3], [5, 6], [6, 7], [7, 8]]
+    assert [t[2:] for t in b2] == [[0.35, 4000.0], [0.36, 4100.0], [0.37, 4200.0]] * 2
+    assert [t[:3] for t in b3] == [[0, 1, 2], [1, 2, 3], [5, 6, 7], [6, 7, 8]]
+    assert b3[0][3] == np.radians(127.0) and b3[1][4] == 25.0
+    assert [t[:4] for t in b4] == [[0, 1, 2, 3], [3, 2, 1, 0], [0, 1, 2, 3], [1, 2, 3, 0],
+                                   [5, 6, 7, 8], [8, 7, 6, 5], [5, 6, 7, 8], [6, 7, 8, 5]]
+    assert [t[5] for t in b4] == [0, 1, 2, 3] * 2 and all(t[6] == 0 for t in b4)
+    np.testing.assert_array_equal(b4[0][4][:2], [[7.406, -5.298, -2.570, 1.336, 0.739],
+                                                 [-0.28632126, 1.2099146, 1.18122138, 0.49075168, 0.98495911]])
+    np.testing.assert_array_equal(b4[0][4][2:], 0.0)
+    assert b4[1][4].shape == (6, 5)
+    np.testing.assert_array_equal(b4[1][4][4:], [[1, 2, 3, 4, 5], [6, 7, 8, 9, 10]])
+    np.testing.assert_array_equal(b4[2][4], [[1] * 5, [2] * 5])
+    np.testing.assert_array_equal(b4[3][4], [[1, 0, 0, 0, 0], [0, 1, 0, 0, 0], [0] * 5, [0] * 5])
+    arrays = OT.prepare_bonds_topol(molecules, TOPOL)
+    np.testing.assert_array_equal(arrays[13][2], np.resize(np.array([[1] * 5, [2] * 5], float), (6, 5)))
+    assert list(arrays[15]) == [0] * 8
+
+
+def test_propensity_closed_forms():
+    a, b, c = np.array(OT._ALPHA), np.array(OT._BETA), np.array(OT._COIL)
+    np.testing.assert_array_equal(OT.propensity_potential_coeffs(-1.0)[:2], a)
+    np.testing.assert_array_equal(OT.propensity_potential_coeffs(0.0)[:2], c)
+    np.testing.assert_array_equal(OT.propensity_potential_coeffs(1.0)[:2], b)
+    for x in (-0.3, 0.6):
+        p = OT.propensity_potential_coeffs(x)
+        main = a if x < 0 else b
+        np.testing.assert_allclose(p[0], abs(x) * main[0], rtol=1e-15)
+        np.testing.assert_array_equal(p[1], main[1])
+        np.testing.assert_allclose(p[2], (1 - abs(x)) * c[0], rtol=1e-15)
+        np.testing.assert_array_equal(p[3], c[1])
+    assert OT.propensity_potential_coeffs(1.5) is None
