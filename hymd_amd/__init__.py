@@ -9,7 +9,11 @@ behind the C-ABI in include/hymd_amd.h):
 * ``hymd_amd.hamiltonian``    hamiltonian.py: DefaultNoChi, DefaultWithChi, SquaredPhi
 * ``hymd_amd.force``          force.py compiled kernels: compute_*_forces__fortran(+__double),
                                dipole_forces_redistribution; ``force_kernels`` = cbf/caf/cdf(+_d)
-* ``hymd_amd.md``             the rRESPA loop of main.py on the device
+* ``hymd_amd.md``             the rRESPA loop of main.py on the device (+ thermostat / COM coupling)
+* ``hymd_amd.topology``       force.py: prepare_bonds (connectivity table or TOML topology)
+* ``hymd_amd.thermostat``     thermostat.py: csvr_thermostat, cancel_com_momentum
+* ``hymd_amd.pressure``       pressure.py: comp_pressure
+* ``hymd_amd.barostat``       barostat.py / barostat_scr.py: isotropic, semiisotropic
 * ``hymd_amd.synth``          seeded synthetic systems
 """
 __version__ = "0.1.0"
