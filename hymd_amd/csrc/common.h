@@ -342,6 +342,11 @@ int fused_inner_impl(hymd_fused *f, double *x, double *v, float *x32, float *f_f
                      double *virials, void *stream, const RowKeyTarget *key);
 
 // Pinned host staging for small results.
+// Device scratch of at least n doubles for reductions that synchronise
+// before returning: one grow-only buffer per device and host thread, never
+// freed (no HIP call at process exit).  NULL on failure (message set).
+double *sync_scratch(size_t n);
+
 struct HostBuf {
     void *ptr = nullptr;
     size_t bytes = 0;

@@ -35,6 +35,31 @@ void DevBuf::release() {
     bytes = 0;
 }
 
+double *sync_scratch(size_t n) {
+    constexpr int kMaxDevices = 64;
+    static thread_local void *buf[kMaxDevices] = {};
+    static thread_local size_t cap[kMaxDevices] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error("sync_scratch: no current device");
+        return nullptr;
+    }
+    const size_t need = sizeof(double) * (n > 0 ? n : 1);
+    if (cap[dev] < need) {
+        if (buf[dev]) (void)hipFree(buf[dev]);
+        buf[dev] = nullptr;
+        cap[dev] = 0;
+        hipError_t e = hipMalloc(&buf[dev], need);
+        if (e != hipSuccess) {
+            buf[dev] = nullptr;
+            set_error("hipMalloc(%zu) failed: %s", need, hipGetErrorString(e));
+            return nullptr;
+        }
+        cap[dev] = need;
+    }
+    return (double *)buf[dev];
+}
+
 int HostBuf::reserve(size_t need) {
     if (need <= bytes) return HYMD_OK;
     release();

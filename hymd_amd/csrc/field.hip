@@ -1511,14 +1511,13 @@ extern "C" int hymd_self_energy_q(const float *charges, int64_t n, double sigma,
     HYMD_REQUIRE(out && (charges || n == 0) && n >= 0 && sigma > 0.0, "bad argument");
     hipStream_t s = (hipStream_t)stream;
     const int nb = grid_for(n, kBlock, 1024);
-    DevBuf part, res;
-    HYMD_TRY(part.reserve(sizeof(double) * nb));
-    HYMD_TRY(res.reserve(sizeof(double)));
-    self_energy_kernel<<<nb, kBlock, 0, s>>>(charges, n, (double *)part.ptr);
+    double *part = sync_scratch(nb + 1);  // nb partials, then the sum
+    if (!part) return HYMD_ERR_NOMEM;
+    self_energy_kernel<<<nb, kBlock, 0, s>>>(charges, n, part);
     HYMD_HIP(hipGetLastError());
-    HYMD_TRY(reduce_partials((const double *)part.ptr, nb, 1, (double *)res.ptr, s));
+    HYMD_TRY(reduce_partials(part, nb, 1, part + nb, s));
     double q2 = 0.0;
-    HYMD_HIP(hipMemcpyAsync(&q2, res.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
+    HYMD_HIP(hipMemcpyAsync(&q2, part + nb, sizeof(double), hipMemcpyDeviceToHost, s));
     HYMD_HIP(hipStreamSynchronize(s));
     *out = elec_factor * std::sqrt(1.0 / (2.0 * M_PI * sigma * sigma)) * q2;
     return HYMD_OK;

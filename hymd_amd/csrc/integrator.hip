@@ -154,17 +154,16 @@ extern "C" int hymd_kinetic_energy(const void *v, int64_t n, int32_t precision, 
     HYMD_REQUIRE(precision == HYMD_F32 || precision == HYMD_F64, "precision must be HYMD_F32 or HYMD_F64");
     hipStream_t s = (hipStream_t)stream;
     const int nb = grid_for(3 * n, kBlock, 1024);
-    DevBuf part, res;
-    HYMD_TRY(part.reserve(sizeof(double) * nb));
-    HYMD_TRY(res.reserve(sizeof(double)));
+    double *part = sync_scratch(nb + 1);  // nb partials, then the sum
+    if (!part) return HYMD_ERR_NOMEM;
     if (precision == HYMD_F32)
-        ke_kernel<float><<<nb, kBlock, 0, s>>>((const float *)v, 3 * n, (double *)part.ptr);
+        ke_kernel<float><<<nb, kBlock, 0, s>>>((const float *)v, 3 * n, part);
     else
-        ke_kernel<double><<<nb, kBlock, 0, s>>>((const double *)v, 3 * n, (double *)part.ptr);
+        ke_kernel<double><<<nb, kBlock, 0, s>>>((const double *)v, 3 * n, part);
     HYMD_HIP(hipGetLastError());
-    HYMD_TRY(reduce_partials((const double *)part.ptr, nb, 1, (double *)res.ptr, s));
+    HYMD_TRY(reduce_partials(part, nb, 1, part + nb, s));
     double sum = 0.0;
-    HYMD_HIP(hipMemcpyAsync(&sum, res.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
+    HYMD_HIP(hipMemcpyAsync(&sum, part + nb, sizeof(double), hipMemcpyDeviceToHost, s));
     HYMD_HIP(hipStreamSynchronize(s));
     *out = 0.5 * mass * sum;
     return HYMD_OK;
