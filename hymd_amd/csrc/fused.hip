@@ -60,9 +60,14 @@ inline size_t fused_lds_bytes(int particles, int bonds, int angles) {
 }
 
 // F32: bonds and angles evaluated in FP32 (slots and the gather in FP32),
-// kicks, drifts and positions in FP64 as always
+// kicks, drifts and positions in FP64 as always.
+// Occupancy hints: a 26 KB block fits six times in a CU's LDS, i.e. six waves
+// per SIMD; the FP32 one-particle-per-thread form is held to 80 VGPRs so that
+// registers allow the same six (116 us against 127 us at 96 VGPRs on the
+// 1,048,576-bead lipid/water outer step).
 template <int PPT, bool F32>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PPT == 1 ? 4 : 3))) fused_inner_kernel(FusedArgs A) {
+__global__ void __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(PPT == 1 ? (F32 ? 6 : 4) : 3))) fused_inner_kernel(FusedArgs A) {
     using R = typename std::conditional<F32, float, double>::type;
     extern __shared__ double smem_d[];
     const int blk = blockIdx.x;
