@@ -415,7 +415,8 @@ def main():
                     help="device prepare_bonds builds of the bench system (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the particle-field path has no multi-GPU implementation; "
+              "the value is one GPU's throughput", file=sys.stderr)
         sys.exit(2)
 
     import torch
