@@ -83,3 +83,37 @@ def test_barostat_matches_oracle(hip, geom, kind, rows_quirk):
     assert np.array_equal(out[0].BoxSize, cfg.box_size)
     assert not np.array_equal(box_ref, np.array([4.0, 4.0, 5.0]))     # the box did move
     assert change is (not (geom == "isotropic" and kind == "scr"))  # barostat_scr.py:132 returns False
+
+
+@pytest.mark.parametrize("geom", ["isotropic", "semiisotropic"])
+def test_field_after_box_move_matches_oracle(hip, geom):
+    """After a barostat step the moved mesh (same grid, new box) gives the
+    oracle's filtered densities and force grids of the rescaled positions in
+    the new box: cell size, wave numbers, filter and cell volume all follow
+    the box."""
+    from hymd_amd import barostat
+    from hymd_amd.field import update_field
+    from tests.conftest import rms_rel
+    from oracle import field as OF
+    print(f"field after box move {geom}", flush=True)
+    cfg, H, pm_stuff, fl, xd, v, bp, ap = _setup(seed=11)
+    cfg.barostat_type = "berendsen"
+    box0 = cfg.box_size.copy()
+    kw = {} if geom == "isotropic" else {"rows_quirk": False}
+    getattr(barostat, geom)(None, pm_stuff, fl[0], None, None, H, xd, v, cfg, fl[1], fl[6], fl[5], bp, ap, 2,
+                            np.random.default_rng(1), **kw)
+    assert not np.array_equal(cfg.box_size, box0)
+    T = cfg.n_types
+    rng = np.random.default_rng(11)      # the types of _setup(seed=11): drawn after the positions
+    rng.uniform(0.0, 1.0, (xd.shape[0], 3))
+    types = rng.integers(0, T, xd.shape[0])
+    ty = torch.as_tensor(types, dtype=torch.int32).cuda()
+    pm = pm_stuff[0]
+    update_field(fl[0], fl[6], fl[5], None, fl[2], H, pm, xd, ty, cfg, fl[4], fl[1], fl[3], None)
+    x_new = xd.to(torch.float32).cpu().numpy().astype(np.float64)
+    A, b, c0 = H.quadratic()
+    ref = OF.update_field(x_new, types, T, cfg.mesh, cfg.box_size, cfg.sigma, OF.Quadratic(A, b, c0))
+    for t in range(T):
+        assert rms_rel(fl[0][t].cpu().numpy(), ref["phi"][t]) < 1e-6
+        for d in range(3):
+            assert rms_rel(fl[2][t][d].cpu().numpy(), ref["force_mesh"][t][d]) < 1e-5
