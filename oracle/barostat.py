@@ -23,8 +23,12 @@ SCR draws prng.normal() once (isotropic) or twice, Rxy then Rz
 
 Pinning (tests/test_oracle_barostat.py): numpy's own evaluation of the
 reference's indexing expressions on arrays, P = P_t gives alpha = 1 (SCR:
-with R = 0), the Berendsen factor's first-order expansion, and the SCR noise
-variance 2 n_b^2 R T beta dt / (V tau_p) of log alpha.
+with R = 0), the unit factor against the physical constants, and dynamics on
+an ideal gas P = N R T / V: Berendsen coupling relaxes ln V to N R T / P_t at
+the rate n_b dt beta P_t / tau_p per call, from either side (isotropic and
+semi-isotropic), and SCR's stationary variance of ln V is
+n_b R T / (V P_t[bar]) (the reference's noise term carries no unit factor and
+n_b^2, so this is 1/(16.61 N) at n_b = 1 instead of the NPT ensemble's 1/N).
 """
 import numpy as np
 

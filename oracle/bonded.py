@@ -4,16 +4,19 @@
 compute_bond_forces*.f90, compute_angle_forces*.f90,
 compute_dihedral_forces*.f90 and dipole_reconstruction*.f90); ``single=True``
 gives the real(4) variants, ``single=False`` the __double ones.
-``ref_cbf``/``ref_caf``/``ref_cdf`` call the reference's own Fortran compiled
-into ``_ref/libhymd_ref.so`` (see Makefile) -- available only where that was
-built.  All return values follow the f2py call surface of force.py:8-23:
+All return values follow the f2py call surface of force.py:8-23:
 ``(f, energy, pr)`` for bonds/angles, ``(f, dipoles, tmat, energy)`` for
 dihedrals, with C-order float64 arrays.
 
-Pinning: bonds/angles/dihedral energies and forces against the reference's
-goldens (test/test_force.py:47-273; dihedral forces there carry the opposite
-sign, see tests), central finite differences of the energies, and
-element-wise agreement with the compiled reference (``_ref``).
+Pinning (tests/test_oracle_bonded.py): bonds/angles/dihedral energies and
+forces against the reference's goldens (test/test_force.py:47-273; dihedral
+forces there carry the opposite sign), central finite differences of the
+energies, the cosine-series and CBT energies on geometries of known gamma and
+phi, the dipole pair's geometry (delta, psi, theta(gamma) of the fit), the
+transfer matrices against finite differences of the dipole displacement and
+the redistribution by virtual work.  The reference's Fortran is reached only
+through f2py's generated wrapper (f2py needs meson, absent here), so it is not
+built.
 """
 import ctypes
 import os
@@ -23,19 +26,16 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "_build", "liboracle_bonded.so")
-_REF = os.path.join(_HERE, "_ref", "libhymd_ref.so")
 _lib = None
-_ref = None
 
 _d = ctypes.c_double
 _i64 = ctypes.c_int64
 _p = ctypes.c_void_p
 
 
-def build(ref=False):
-    """Compile the C restatement (and the reference Fortran if ``ref``)."""
-    targets = ["oracle"] + (["ref"] if ref else [])
-    subprocess.run(["make", "-s", "-C", _HERE] + targets, check=True)
+def build():
+    """Compile the C restatement."""
+    subprocess.run(["make", "-s", "-C", _HERE, "oracle"], check=True)
 
 
 def _load():
@@ -56,20 +56,6 @@ def _load():
         lib.oracle_clamp_count.restype = ctypes.c_long
         _lib = lib
     return _lib
-
-
-def ref_available():
-    return os.path.exists(_REF)
-
-
-def _load_ref():
-    global _ref
-    if _ref is None:
-        lib = ctypes.CDLL(_REF)
-        for name in ("ref_cbf", "ref_cbf_d", "ref_caf", "ref_caf_d", "ref_cdf", "ref_cdf_d"):
-            getattr(lib, name).restype = None
-        _ref = lib
-    return _ref
 
 
 def _ptr(a):
@@ -146,59 +132,3 @@ def dipole_forces_redistribution(n, f_dipoles, tmat, a, b, c, d, dtype, last):
     lib.oracle_dipole_redistribution(n, _ptr(out), _ptr(fd), _ptr(tm), m, _ptr(a), _ptr(b),
                                      _ptr(c), _ptr(d), _ptr(dtype), _ptr(last))
     return out
-
-
-# ---------------------------------------------------------------------------
-# The compiled reference (Fortran order, as f2py passes np.asfortranarray)
-# ---------------------------------------------------------------------------
-def ref_cbf(r, box, a, b, r0, k, single=False):
-    lib = _load_ref()
-    ft = np.float32 if single else np.float64
-    r = np.asfortranarray(r, dtype=ft)
-    n, m = len(r), len(a)
-    f = np.asfortranarray(np.zeros((n, 3), dtype=ft))
-    pr = np.zeros(3, dtype=ft)
-    e = _d(0.0)
-    a, b = _i32(a), _i32(b)
-    r0, k, box = _f64(r0), _f64(k), _f64(box)
-    fn = lib.ref_cbf if single else lib.ref_cbf_d
-    fn(_i64(n), _i64(m), _ptr(f), _ptr(r), _ptr(box), _ptr(a), _ptr(b), _ptr(r0), _ptr(k),
-       ctypes.byref(e), _ptr(pr))
-    return np.ascontiguousarray(f, dtype=np.float64), e.value, pr.astype(np.float64)
-
-
-def ref_caf(r, box, a, b, c, t0, k, single=False):
-    lib = _load_ref()
-    ft = np.float32 if single else np.float64
-    r = np.asfortranarray(r, dtype=ft)
-    n, m = len(r), len(a)
-    f = np.asfortranarray(np.zeros((n, 3), dtype=ft))
-    pr = np.zeros(3, dtype=ft)
-    e = _d(0.0)
-    a, b, c = _i32(a), _i32(b), _i32(c)
-    t0, k, box = _f64(t0), _f64(k), _f64(box)
-    fn = lib.ref_caf if single else lib.ref_caf_d
-    fn(_i64(n), _i64(m), _ptr(f), _ptr(r), _ptr(box), _ptr(a), _ptr(b), _ptr(c), _ptr(t0),
-       _ptr(k), ctypes.byref(e), _ptr(pr))
-    return np.ascontiguousarray(f, dtype=np.float64), e.value, pr.astype(np.float64)
-
-
-def ref_cdf(r, box, a, b, c, d, coeff, dtype, bb_index, dipole_flag, single=False):
-    lib = _load_ref()
-    ft = np.float32 if single else np.float64
-    r = np.asfortranarray(r, dtype=ft)
-    n, m = len(r), len(a)
-    f = np.asfortranarray(np.zeros((n, 3), dtype=ft))
-    dip = np.asfortranarray(np.zeros((m, 4, 3), dtype=ft))
-    tm = np.asfortranarray(np.zeros((m, 6, 3, 3), dtype=ft))
-    coeff = np.asfortranarray(np.asarray(coeff, dtype=np.float32).reshape(m, 6, 5))
-    e = _d(0.0)
-    a, b, c, d = _i32(a), _i32(b), _i32(c), _i32(d)
-    dtype, bb_index = _i32(dtype), _i32(bb_index)
-    box = _f64(box)
-    fn = lib.ref_cdf if single else lib.ref_cdf_d
-    fn(_i64(n), _i64(m), _ptr(f), _ptr(r), _ptr(dip), _ptr(tm), _ptr(box), _ptr(a), _ptr(b),
-       _ptr(c), _ptr(d), _ptr(coeff), _ptr(dtype), _ptr(bb_index), ctypes.c_int32(int(dipole_flag)),
-       ctypes.byref(e))
-    return (np.ascontiguousarray(f, dtype=np.float64), np.ascontiguousarray(dip, dtype=np.float64),
-            np.ascontiguousarray(tm, dtype=np.float64), e.value)

@@ -38,12 +38,8 @@ def goldens():
 
 @pytest.fixture(scope="session")
 def oracle_built():
-    """Compile oracle/bonded.c (and the reference Fortran when its sources exist)."""
+    """Compile oracle/bonded.c."""
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "oracle"], check=True)
-    if os.path.isdir("/root/reference/hymd") and not os.path.exists(
-            os.path.join(ROOT, "oracle", "_ref", "libhymd_ref.so")):
-        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=False,
-                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     from oracle import bonded
     return bonded
 
@@ -58,13 +54,21 @@ def hip():
     return _lib
 
 
+def _pair(a, b):
+    """Promote to float64 or complex128 (complex data keeps its imaginary part)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    kind = np.complex128 if (np.iscomplexobj(a) or np.iscomplexobj(b)) else np.float64
+    return a.astype(kind), b.astype(kind)
+
+
 def rel_err(a, b):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
+    """max |a - b| / max |b| (moduli for complex data)."""
+    a, b = _pair(a, b)
     return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
 
 
 def rms_rel(a, b):
-    a = np.asarray(a, dtype=np.float64)
-    b = np.asarray(b, dtype=np.float64)
-    return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.sqrt(np.mean(b ** 2)), 1e-300))
+    """sqrt(mean |a - b|^2) / sqrt(mean |b|^2) (moduli for complex data)."""
+    a, b = _pair(a, b)
+    return float(np.sqrt(np.mean(np.abs(a - b) ** 2)) / max(np.sqrt(np.mean(np.abs(b) ** 2)), 1e-300))

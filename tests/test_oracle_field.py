@@ -7,7 +7,10 @@
 * field forces on grid nodes = continuum Gaussian-core pair forces
   (compute_gaussian_core.f90:60-89 with a coupling matrix);
 * sum of forces = 0, 90-degree rotation permutes forces;
-* PME: two opposite charges = Gaussian-smeared Coulomb; neutral sum F = 0.
+* PME: two opposite charges = Gaussian-smeared Coulomb; neutral sum F = 0;
+* PME energy (field_q_energy with the self energy removed): the Gaussian-smeared
+  Coulomb pair energies of a neutral set of charges on nodes, with the
+  periodic (tinfoil) dipole term of a cubic lattice.
 """
 import numpy as np
 import pytest
@@ -201,3 +204,50 @@ def test_self_energy():
     q = np.array([1.0, -1.0, 0.5])
     e = F.self_energy_q(q, 0.5, 138.935458, 45.0)
     assert e == pytest.approx(138.935458 / 45.0 / np.sqrt(2 * np.pi * 0.25) * 2.25, rel=1e-14)
+
+
+@pytest.mark.parametrize("sep, tol", [(8, 1e-5), (16, 2e-4)])
+def test_pme_energy_two_charges(sep, tol):
+    """E_q = dV sum(1/2 phi_q psi) - E_self (field.py:697-699) for charges on
+    nodes.  With the filter applied once the pair potential is
+    k_e/eps erf(r/(sqrt2 sigma))/r and its r -> 0 limit is exactly E_self's
+    per-charge term (field.py:231-238), so the self terms cancel and
+    E_q = sum_{i<j} k_e/eps q_i q_j erf(r_ij/(sqrt2 sigma))/r_ij
+          - 2 pi/(3V) k_e/eps |sum_i q_i r_i|^2
+    for a neutral set in a cubic periodic box without the k = 0 term (the
+    r^2 term of the periodic Coulomb potential; the next term is O(r^4/L^5)).
+    A dropped 1/2, a missing self energy or a wrong dV fail by O(1)."""
+    M, L, sigma, ke, eps = 128, 24.0, 0.5, 138.935458, 45.0
+    h = L / M
+    box = np.full(3, L)
+    x = np.array([[60, 64, 64], [60 + sep, 64, 64]], dtype=float) * h
+    q = np.array([1.0, -1.0])
+    out = F.update_field_force_q(x, q, M, box, sigma, ke, eps)
+    e_self = F.self_energy_q(q, sigma, ke, eps)
+    e = F.field_q_energy(out["phi_q"], out["psi"], e_self, M, box)
+    r = sep * h
+    dipole = sep * h  # |sum q_i r_i|
+    expect = ke / eps * (-1.0) * erf(r / (np.sqrt(2.0) * sigma)) / r - 2 * np.pi / (3 * L ** 3) * ke / eps * dipole ** 2
+    assert e == pytest.approx(expect, rel=tol)
+    # the self energy is not small against the pair energy: leaving it out fails
+    assert abs(e + e_self - expect) > 100 * tol * abs(expect)
+
+
+def test_pme_energy_neutral_cluster():
+    """Four charges (+1, -0.5, -1.5, +1) on nodes: the pair-sum closed form of
+    test_pme_energy_two_charges within the O(r^4/L^5) lattice term."""
+    M, L, sigma, ke, eps = 128, 24.0, 0.5, 138.935458, 45.0
+    h = L / M
+    box = np.full(3, L)
+    x = np.array([[40, 64, 64], [70, 60, 64], [50, 80, 70], [44, 71, 59]], dtype=float) * h
+    q = np.array([1.0, -0.5, -1.5, 1.0])
+    out = F.update_field_force_q(x, q, M, box, sigma, ke, eps)
+    e = F.field_q_energy(out["phi_q"], out["psi"], F.self_energy_q(q, sigma, ke, eps), M, box)
+    pair = 0.0
+    for i in range(4):
+        for j in range(i + 1, 4):
+            r = np.linalg.norm(x[i] - x[j])
+            pair += ke / eps * q[i] * q[j] * erf(r / (np.sqrt(2.0) * sigma)) / r
+    p = (q[:, None] * x).sum(axis=0)
+    expect = pair - 2 * np.pi / (3 * L ** 3) * ke / eps * float(p @ p)
+    assert e == pytest.approx(expect, rel=2e-3)
