@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import call, darr, dptr
+from ._lib import HymdError, call, darr, dptr
 
 
 def _unpack_sym(vals, T):
@@ -51,7 +51,10 @@ def moments(pm, x32, types, n_types, sigma, mass=None, coupling=None, shift=None
 
 
 def energy_and_gradients(P, Q, phi0, chi, kappa, rho0, a, sigma, volume):
-    """Energy and exact parameter derivatives from the moments (host, FP64)."""
+    """Energy and exact parameter derivatives from the moments (host, FP64).
+    Raises HymdError for rho0 <= 0 or kappa <= 0 (W divides by kappa rho0)."""
+    if not (rho0 > 0.0 and kappa > 0.0):
+        raise HymdError(f"the field energy needs rho0 > 0 and kappa > 0 (got rho0={rho0}, kappa={kappa})")
     T = P.shape[0]
     chi = np.array(chi, dtype=np.float64)
     np.fill_diagonal(chi, 0.0)
@@ -70,11 +73,17 @@ def energy_and_gradients(P, Q, phi0, chi, kappa, rho0, a, sigma, volume):
 
 def energy_and_gradients_batched(P, Q, phi0, chi, kappa, rho0, a, sigma, volume):
     """energy_and_gradients over R replicas at once: P, Q, chi (R,T,T), phi0
-    (R,T), the rest (R,).  Returns arrays E, dchi (R,T,T), dkappa, dsigma."""
+    (R,T), the rest (R,).  Returns arrays E, dchi (R,T,T), dkappa, dsigma.
+    An empty replica (rho0 = 0: no particles) has no field and gets zero
+    energy and gradients; kappa <= 0 raises HymdError."""
     T = P.shape[1]
     chi = np.array(chi, dtype=np.float64)
     chi[:, np.arange(T), np.arange(T)] = 0.0
     kappa, rho0, a, sigma, V = (np.asarray(z, dtype=np.float64) for z in (kappa, rho0, a, sigma, volume))
+    if np.any(~(kappa > 0.0)) or np.any(~(rho0 >= 0.0)):
+        raise HymdError("the field energy needs kappa > 0 and rho0 >= 0 for every replica")
+    empty = rho0 == 0.0
+    rho0 = np.where(empty, 1.0, rho0)
     inv = 1.0 / (kappa * rho0)
     A = inv[:, None, None] + chi / rho0[:, None, None]
     s0 = phi0.sum(axis=1)
@@ -83,6 +92,8 @@ def energy_and_gradients_batched(P, Q, phi0, chi, kappa, rho0, a, sigma, volume)
     dchi[:, np.arange(T), np.arange(T)] = 0.0
     dkappa = V * (-0.5 * P.sum(axis=(1, 2)) + a * s0 - 0.5 * a * a) / (kappa * kappa * rho0)
     dsigma = -sigma * V * np.einsum("rts,rts->r", A, Q)
+    for arr in (E, dchi, dkappa, dsigma):
+        arr[empty] = 0.0
     return E, dchi, dkappa, dsigma
 
 

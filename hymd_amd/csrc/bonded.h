@@ -87,9 +87,11 @@ __device__ __forceinline__ bool angle_term(V3 ra, V3 rc, double t0, double k, V3
     return true;
 }
 
-// --- the same two terms in FP32 (the reference's default real(4) kernels,
-// compute_bond_forces.f90 / compute_angle_forces.f90), on separations taken
-// in FP64 and rounded once; no energies (the energy pass stays FP64) ------
+// --- the same two terms in FP32 arithmetic (this path's FP32 forces; the
+// reference's real(4) kernels compute_bond_forces.f90 / compute_angle_forces.f90
+// difference positions in FP32 and evaluate the terms in FP64), on
+// separations taken in FP64 and rounded once; no energies (the energy pass
+// stays FP64) ------
 struct F3 {
     float x, y, z;
 };

@@ -265,6 +265,7 @@ int exclusive_scan(int *in, int *out, int64_t n, int *tiles, int *cursor, bool z
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    uint64_t allocs = 0;  // successful allocations so far: a change means `ptr` moved
     int reserve(size_t need);
     void release();
     ~DevBuf() { release(); }
@@ -281,6 +282,13 @@ TopoTerms topology_terms(const hymd_topology *t);
 // compute_field_force at the particle set sorted by the last update_field on
 // this handle (same positions and types)
 int field_readout_last(hymd_pm *pm, int n_types, const float *force_mesh, float *force);
+
+// Changes whenever a buffer of the handle is reallocated or its box is set:
+// work captured against the handle (graphs hold its pointers and its mesh by
+// value) is stale once this differs from its value at capture.
+uint64_t pm_generation(const hymd_pm *pm);
+// the handle's current box edges
+void pm_box(const hymd_pm *pm, double box[3]);
 
 // CSVR thermostat (thermostat.hip): the groups of one call, in the
 // reference's order, and the tables fixed for a particle set.

@@ -26,6 +26,7 @@ int DevBuf::reserve(size_t need) {
         return HYMD_ERR_NOMEM;
     }
     bytes = need;
+    ++allocs;
     return HYMD_OK;
 }
 

@@ -783,7 +783,21 @@ struct hymd_pm {
     DevBuf counts, start, tiles;  // per mesh row (counts kept zero between sorts)
     int64_t sorted_n = 0;
     bool sorted_weights = false;
+    uint64_t box_version = 0;  // hymd_pm_set_box calls
 };
+
+uint64_t hymd::pm_generation(const hymd_pm *pm) {
+    uint64_t g = pm->box_version;
+    for (const DevBuf *b : {&pm->paint, &pm->spec, &pm->grad, &pm->cscratch, &pm->rscratch, &pm->partials,
+                            &pm->result, &pm->mpart, &pm->keys, &pm->perm, &pm->xs, &pm->ws, &pm->counts,
+                            &pm->start, &pm->tiles})
+        g += b->allocs;
+    return g;
+}
+
+void hymd::pm_box(const hymd_pm *pm, double box[3]) {
+    for (int d = 0; d < 3; ++d) box[d] = pm->mesh.box[d];
+}
 
 // LDS budget of one x-line workgroup (two per CU)
 constexpr size_t kXlineLdsBytes = 80 * 1024;
@@ -993,7 +1007,9 @@ extern "C" int hymd_pm_set_stream(hymd_pm *pm, void *stream) {
 extern "C" int hymd_pm_set_box(hymd_pm *pm, const double box[3]) {
     HYMD_REQUIRE(pm && box, "null argument");
     int32_t mesh[3] = {pm->mesh.mx, pm->mesh.my, pm->mesh.mz};
-    return set_mesh(pm, mesh, box);
+    HYMD_TRY(set_mesh(pm, mesh, box));
+    ++pm->box_version;
+    return HYMD_OK;
 }
 
 static FieldK field_params(int T, const double *coupling, const double *shift, double constant, double sigma,

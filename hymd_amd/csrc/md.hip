@@ -24,6 +24,9 @@ struct hymd_md {
     // path, or the fused path after the velocities were completed)
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     hipGraph_t graph[2] = {nullptr, nullptr};
+    // pm_generation(pm) when the graphs were captured: they hold the pm's
+    // buffer pointers and its mesh by value
+    uint64_t pm_gen = 0;
     int64_t n = 0;
     int T = 0;
     double *x = nullptr, *v = nullptr;
@@ -295,6 +298,7 @@ extern "C" int hymd_md_create(hymd_md **out, hymd_pm *pm, hymd_topology *topo, i
     m->dt = dt;
     m->respa_inner = respa_inner;
     for (int d = 0; d < 3; ++d) m->box[d] = box[d];
+    m->pm_gen = pm_generation(pm);
     hipError_t e = hipStreamCreateWithFlags(&m->work, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_in, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_out, hipEventDisableTiming);
@@ -357,7 +361,19 @@ static int capture(hymd_md *m, int pending) {
     }
     m->graph[pending] = g;
     HYMD_HIP(hipGraphInstantiate(&m->exec[pending], g, nullptr, nullptr, 0));
+    m->pm_gen = pm_generation(m->pm);
     return HYMD_OK;
+}
+
+// The pm handle moved a buffer or took a new box since the graphs were
+// captured (a call on the exposed pm with more particles, a barostat's
+// hymd_pm_set_box): follow its box and capture again.
+static void check_pm(hymd_md *m) {
+    const uint64_t g = pm_generation(m->pm);
+    if (g == m->pm_gen) return;
+    drop_graphs(m);
+    pm_box(m->pm, m->box);
+    m->pm_gen = g;
 }
 
 // Runs n_steps outer steps ordered after the work already queued on `stream`,
@@ -417,6 +433,7 @@ extern "C" int hymd_md_step(hymd_md *m, int32_t n_steps, int32_t use_graph, doub
     HYMD_HIP(hipStreamWaitEvent(m->work, m->ev_in, 0));
     const int plain = energies ? n_steps - 1 : n_steps;
     for (int s = 0; s < plain; ++s) {
+        check_pm(m);  // an eager step may have grown a pm buffer too
         if (use_graph) {
             const int gi = m->fused ? m->pending : 0;
             if (!m->exec[gi]) HYMD_TRY(capture(m, gi));
@@ -432,6 +449,7 @@ extern "C" int hymd_md_step(hymd_md *m, int32_t n_steps, int32_t use_graph, doub
         HYMD_TRY(couple_step(m));
     }
     if (energies && n_steps > 0) {
+        check_pm(m);
         std::memset(energies, 0, 5 * sizeof(double));
         HYMD_TRY(enqueue_step(m, energies, m->pending));
         if (m->fused) m->pending = 1;

@@ -50,8 +50,10 @@ class RespaMD:
     the order maps are permuted in place, the physics is unchanged (the field
     path's sums are order-independent and those rows carry no bonded terms).
     ``term_precision`` sets the arithmetic of the bond and angle terms in the
-    library's fused inner loop: "fp32" (default) as the reference's default
-    real(4) kernels, "fp64" as its --double-precision kernels; kicks, drifts,
+    library's fused inner loop: "fp32" (default: FP32 term arithmetic, the
+    FP32 forces this path allows; the reference's real(4) kernels difference
+    positions in FP32 but evaluate the terms in FP64) or "fp64" (as its
+    --double-precision kernels); kicks, drifts,
     positions, velocities and energies are FP64 either way, and the Python
     ``step()`` path evaluates the terms in FP64.
     """
@@ -302,6 +304,30 @@ class RespaMD:
             torch.add(self.f_ba, self.f_dih.to(torch.float32), out=self.f_fast)
             if energy:
                 self.energies["dihedral"] = ed
+
+    def fused_fast_forces(self, term_precision=None):
+        """Bond + angle forces (float32, state order) at the current positions
+        as the fused inner loop's kernel evaluates them: one sub-step with
+        dt = 0 and zero kicks, on copies of x and v (the state is untouched).
+        ``term_precision`` "fp32"/"fp64" (default: the handle's).  Raises
+        HymdError when the topology cannot be fused."""
+        prec = term_precision or self.term_precision
+        h = ctypes.c_void_p()
+        b, a = self.topo._b, self.topo._a
+        call("hymd_fused_create", ctypes.byref(h), self.n, b[0].numel(), *[dptr(t) for t in b], a[0].numel(),
+             *[dptr(t) for t in a])
+        try:
+            call("hymd_fused_set_precision", h, _lib.HYMD_F32 if prec == "fp32" else _lib.HYMD_F64)
+            x = self.x.clone()
+            v = torch.zeros_like(self.v)
+            x32 = torch.empty_like(self.x32)
+            f = torch.zeros_like(self.f_fast)
+            call("hymd_fused_inner", h, dptr(x), dptr(v), dptr(x32), dptr(f), dptr(self.f_fast), 0.0, 0.0, 0.0,
+                 1, 0, self._box_c, None, None, _lib.stream_ptr())
+            torch.cuda.synchronize()
+        finally:
+            _lib.lib().hymd_fused_destroy(h)
+        return f
 
     def kick_drift(self, f_fast, c_fast, f_slow=None, c_slow=0.0):
         call("hymd_respa_kick_drift", dptr(self.x), dptr(self.v), dptr(self.x32), self.n, dptr(f_fast),

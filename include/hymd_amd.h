@@ -243,10 +243,11 @@ int hymd_fused_destroy(hymd_fused *fused);
  * (the second half kick of one sub-step and the first of the next are separate
  * additions); x32 = float(x).  energies host double[2], virials host double[6]
  * of the last sub-step, or NULL. */
-/* bond / angle terms of the fused inner loop in FP32 (HYMD_F32, the
- * reference's default real(4) kernels compute_bond_forces.f90 /
- * compute_angle_forces.f90) or FP64 (HYMD_F64, the __double kernels; the
- * default); kicks, drifts, positions and the energy pass stay FP64 */
+/* bond / angle terms of the fused inner loop in FP32 arithmetic (HYMD_F32;
+ * the reference's real(4) kernels compute_bond_forces.f90 /
+ * compute_angle_forces.f90 difference positions in FP32 and evaluate the
+ * terms in FP64) or FP64 (HYMD_F64, as the __double kernels; the default);
+ * kicks, drifts, positions and the energy pass stay FP64 */
 int hymd_fused_set_precision(hymd_fused *fused, int32_t precision);
 int hymd_fused_inner(hymd_fused *fused, double *x, double *v, float *x32, float *f_fast, const float *f_slow,
                      double h_in, double h_out, double dt, int32_t n_inner, int32_t pending, const double box[3],

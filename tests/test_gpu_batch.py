@@ -77,6 +77,10 @@ def test_batch_gradients_vs_fd(hip):
     x32 = rs.positions.astype(np.float32)
     E, dchi, dk, ds = b.energy_and_gradients(torch.as_tensor(x32).cuda(), torch.as_tensor(rs.types).cuda(),
                                              rs.chi, rs.kappa, rs.sigma)
+    # every output finite; the empty replica has no field: zero energy and gradients
+    for arr in (E, dchi, dk, ds):
+        assert np.all(np.isfinite(arr))
+    assert E[4] == 0.0 and dk[4] == 0.0 and ds[4] == 0.0 and np.all(dchi[4] == 0.0)
     h = 1e-3
     for r in (0, 2, 5):
         x, t = rs.replica(r)

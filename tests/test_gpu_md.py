@@ -358,7 +358,8 @@ def test_single_step_graph_replays_equal_eager_steps(hip):
 
 @pytest.mark.parametrize("make", ["lipid", "diblock"])
 def test_fp32_terms_follow_the_fp64_loop(hip, make):
-    """FP32 bond / angle terms (the reference's default real(4) kernels) in the
+    """FP32 bond / angle term arithmetic (the reference's real(4) kernels
+    evaluate the terms in FP64 after an FP32 position difference) in the
     fused loop: fast forces within FP32 rounding of the FP64 loop's after one
     outer step; positions below the float32 resolution the field path sees
     (about 1e-6 nm at these box sizes) after one outer step and within 1e-5
@@ -386,5 +387,75 @@ def test_fp32_terms_follow_the_fp64_loop(hip, make):
     assert dx < 1e-5, dx
     for k in ("bond", "angle", "kinetic"):
         assert eb[k] == pytest.approx(ea[k], rel=1e-5)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("precision, tol", [("fp32", 2e-6), ("fp64", 1e-7)])
+def test_fused_kernel_forces_against_the_oracle(hip, oracle_built, precision, tol):
+    """The fused inner loop's own bond + angle forces (one sub-step with dt = 0
+    and zero kicks) on the lipid/water workload, whose head-linker angles have
+    theta0 = 120 degrees and whose tail angles are straight (theta0 = 180),
+    against the FP64 oracle cbf_d + caf_d (compute_bond_forces__double.f90,
+    compute_angle_forces__double.f90): relative RMS 2e-6 for FP32 term
+    arithmetic, 1e-7 (the float32 rounding of the force output) for FP64
+    terms."""
+    from hymd_amd import synth
+    from hymd_amd.md import RespaMD
+    s = synth.lipid_water(n_chains=4000, n_water=30000, box=14.0, mesh=32, seed=17)
+    assert np.any(np.isclose(s.angles[3], np.radians(120.0))) and np.any(np.isclose(s.angles[3], np.pi))
+    md = RespaMD(s, term_precision=precision)
+    assert md.fused
+    f = md.in_input_order(md.fused_fast_forces()).cpu().numpy().astype(np.float64)
+    x = md.in_input_order(md.x).cpu().numpy()
+    ob, _, _ = oracle_built.cbf(x, s.box, *s.bonds)
+    oa, _, _ = oracle_built.caf(x, s.box, *s.angles)
+    ref = ob + oa
+    assert rms_rel(f, ref) < tol
+    assert rel_err(f, ref) < 20 * tol
+    # the state did not move
+    assert torch.equal(md.in_input_order(md.x).cpu(), torch.as_tensor(x))
+    md.close()
+
+
+def test_captured_step_follows_pm_buffer_growth_and_box(hip):
+    """The captured outer step holds the pm handle's buffer pointers and mesh:
+    after a call on the exposed ``md.pm`` with more particles (its sort
+    buffers are reallocated) and after a new box (``pm.set_box``, as the
+    barostats re-create the mesh, barostat.py:163-164), the next graph step
+    equals the eager step bit for bit."""
+    from hymd_amd import synth
+    from hymd_amd.md import RespaMD
+    s = synth.lipid_water(n_chains=600, n_water=5000, box=8.0, mesh=32, seed=23)
+    a = RespaMD(s)
+    b = RespaMD(s)
+    for _ in range(2):
+        a.run(1, use_graph=True)
+        b.run(1, use_graph=False)
+    # grow a's pm sort buffers: a field update of twice as many particles
+    big = torch.rand((4 * s.n, 3), dtype=torch.float32, device="cuda") * float(s.box[0])
+    tb = torch.randint(0, a.T, (4 * s.n,), dtype=torch.int32, device="cuda")
+    fm = a.pm.create("real", batch=(a.T, 3))
+    a.pm.field_update(big, tb, a.T, a.A, a.b, a.c0, a.type_mass, a.sigma, fm)
+    torch.cuda.synchronize()
+    del big, tb, fm
+    torch.cuda.empty_cache()
+    a.run(2, use_graph=True)
+    b.run(2, use_graph=False)
+    torch.cuda.synchronize()
+    assert torch.equal(a.x, b.x) and torch.equal(a.v, b.v) and torch.equal(a.f_field, b.f_field)
+    # a 3% larger box on both: positions scaled, mesh moved
+    f = 1.03
+    new_box = np.asarray(s.box, dtype=np.float64) * f
+    for md in (a, b):
+        md.x.mul_(f)
+        md.x32.copy_(md.x.to(torch.float32))
+        md.pm.set_box(new_box)
+    a.run(2, use_graph=True)
+    b.run(2, use_graph=False)
+    torch.cuda.synchronize()
+    assert torch.equal(a.x, b.x) and torch.equal(a.v, b.v) and torch.equal(a.f_field, b.f_field)
+    # the wrap follows the new box
+    assert float(a.x.max()) < float(new_box[0]) and float(a.x.max()) > float(s.box[0]) * 0.99
     a.close()
     b.close()
