@@ -261,6 +261,72 @@ def peptide_solvent_stages(reps, check):
     return out
 
 
+def markov_melt_leg(steps, sweep_points):
+    """The large single system: 16,777,216 beads, 100 nm box, 256^3 mesh, 6
+    Markov-typed types in 16-bead chains, 5 % charges with PME on the same
+    mesh, respa_inner 1.  `steps` outer steps of the library's captured step
+    (fused inner loop, field + PME slow forces), then a chi sweep of
+    `sweep_points` perturbations on the frozen final positions: one moments
+    pass, then per point the energy derivatives, the forces from the held
+    spectra and the gradient of the force loss through the adjoint chain."""
+    import torch
+
+    from hymd_amd import synth
+    from hymd_amd.md import RespaMD
+    from hymd_amd.sweep import ChiSweep
+    t0 = time.perf_counter()
+    s = synth.markov_melt()
+    gen_s = time.perf_counter() - t0
+    md = RespaMD(s)
+    md.run(3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    md.run(steps)
+    torch.cuda.synchronize()
+    step_ms = 1e3 * (time.perf_counter() - t0) / steps
+    e = md.run(1, energies=True)
+    stages = {}
+    for name in ("bonded", "field", "readout", "pme"):
+        md.time_stage(name)
+        md.run(3, use_graph=False)
+        torch.cuda.synchronize()
+        stages[name] = round(md.stage_time()[0], 4)
+        md.time_stage(None)
+    out = {"workload": "markov_melt: %d beads (%d chains x 16), %.0f nm box, %d^3 mesh for density and PME, %d "
+                       "types, %d charges, %d bonds, %d angles, respa_inner %d"
+                       % (s.n, s.n // 16, s.box[0], s.mesh[0], s.n_types, int(np.count_nonzero(s.charges)),
+                          len(s.bonds[0]), len(s.angles[0]), s.respa_inner),
+           "generate_s": round(gen_s, 1), "fused_inner_loop": md.fused, "steps": steps,
+           "ms_per_outer_step": round(step_ms, 4), "particle_steps_per_s": s.n / (step_ms * 1e-3),
+           "stage_ms": stages, "energies_last_step": {k: e[k] for k in ("field", "field_q", "bond", "angle",
+                                                                          "kinetic")}}
+    if sweep_points > 0:
+        rng = np.random.default_rng(2004)
+        x32 = md.x32.clone()
+        types = md.types.clone()
+        base = s.chi.copy()
+        sw = ChiSweep(md.pm, x32, types, s.n_types, s.kappa, s.sigma)
+        target = sw.forces(base).clone()
+        chis = []
+        for _ in range(sweep_points):
+            d = np.triu(rng.uniform(-3.0, 3.0, base.shape), 1)
+            chis.append(np.clip(base + d + d.T, 0.0, None))
+        sw.run(chis[:2], target)   # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = sw.run(chis, target)
+        total = time.perf_counter() - t0
+        out["chi_sweep"] = {"points": sweep_points, "moments_pass_ms": round(1e3 * sw.moments_s, 3),
+                            "ms_total": round(1e3 * total, 2), "ms_per_point": round(1e3 * total / sweep_points, 3),
+                            "per_point": "energy, dE/dchi, dE/dkappa, dE/dsigma from the moments; forces from "
+                                         "the held spectra; L = 1/(2N) sum |F - F_base|^2 and dL/dchi, dL/dkappa, "
+                                         "dL/dsigma through the adjoint chain",
+                            "L_range": [min(r["L"] for r in res), max(r["L"] for r in res)]}
+        del sw
+    md.close()
+    return out
+
+
 def lipid_water_bond_table(s):
     """The synthetic lipid/water system as an input file holds it: a
     (N, 2) bond table (chain neighbours, -1 padded), names, molecules."""
@@ -413,6 +479,10 @@ def main():
                     help="device comp_pressure field passes at the bench configuration (0 = skip)")
     ap.add_argument("--topology-reps", type=int, default=3,
                     help="device prepare_bonds builds of the bench system (0 = skip)")
+    ap.add_argument("--large-steps", type=int, default=200,
+                    help="outer steps of the 16,777,216-bead / 256^3 Markov melt with PME (0 = skip)")
+    ap.add_argument("--sweep-points", type=int, default=32,
+                    help="chi-sweep points on the large system's frozen final positions")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the particle-field path has no multi-GPU implementation; "
@@ -545,6 +615,14 @@ def main():
         log(f"peptide/solvent: {peptide['device_ms_outer_step_forces']:.3f} ms of device force stages, "
             f"{peptide['md_ms_per_outer_step']:.3f} ms per outer step")
 
+    large = None
+    if args.large_steps > 0 and not args.small:
+        torch.cuda.empty_cache()
+        log("markov melt: 16,777,216 beads, 256^3 mesh, PME")
+        large = markov_melt_leg(args.large_steps, args.sweep_points)
+        log(f"markov melt: {large['ms_per_outer_step']:.3f} ms per outer step"
+            + (f", chi sweep {large['chi_sweep']['ms_per_point']:.2f} ms per point" if "chi_sweep" in large else ""))
+
     out = {
         "metric": METRIC,
         "value": value,
@@ -574,6 +652,7 @@ def main():
         "nvt_csvr": nvt,
         "prepare_bonds": topo,
         "pressure_field_terms": press,
+        "markov_melt_256": large,
     }
     print(json.dumps(out))
 

@@ -42,6 +42,8 @@ SIGNATURES = {
     "hymd_field_q_energy": [_p, _p, _p, _d, _pd],
     "hymd_self_energy_q": [_p, _i64, _d, _d, _pd, _p],
     "hymd_field_moments": [_p, _p, _p, _i64, _i32, _pd, _d, _pd, _pd, _pd, _p],
+    "hymd_field_forces_held": [_p, _i32, _pd, _pd, _d, _p, _p],
+    "hymd_field_adjoint": [_p, _p, _p, _i64, _i32, _pd, _d, _pd, _p, _i32, _pd, _pd],
     "hymd_field_pressure": [_p, _p, _i32, _pd, _pd, _d, _d, _p, _p, _pd, _d, _pd],
     "hymd_md_set_peptide": [_p, _p, _d, _d, _p, _i64, _p, _p],
     "hymd_md_peptide_state": [_p, _p, _p, _pd],

@@ -17,6 +17,15 @@ These are exact derivatives of the grid energy.  The position gradient
 returned by backward is minus the reference's forces (the CIC readout of the
 spectral gradient, field.py:607-613 + 152-200), which equals the exact
 position gradient of the grid energy only up to O(h^2).
+
+The forces F_i = readout(c2r(-i k H A H rho^))(x_i) are linear in A, so for
+a loss L(E, F) the force part of dL/dA is one adjoint pass
+(``hymd_field_adjoint``): the upstream gradient dL/dF is CIC-painted per type
+and component (paint is the readout's adjoint), transformed, and reduced
+against the density spectra.  With A = 1/(kappa rho0) + chi/rho0:
+dL/dchi_ts = (dL/dA_ts + dL/dA_st)/rho0 per unordered pair, dL/dkappa =
+-sum_ts dL/dA_ts / (kappa^2 rho0), and dL/dsigma adds the adjoint's sigma
+term (both H factors of the force spectrum depend on sigma).
 """
 import ctypes
 
@@ -144,3 +153,85 @@ class FieldEnergy(torch.autograd.Function):
 def field_energy(x, chi, kappa, sigma, pm, types, rho0, a):
     """Differentiable field energy (see FieldEnergy)."""
     return FieldEnergy.apply(x, chi, kappa, sigma, pm, types, rho0, a)
+
+
+def field_adjoint(pm, x32, types, n_types, sigma, coupling, grad_force, mass=None, reuse=False):
+    """dL/dA (T, T) and dL/dsigma for the upstream gradient ``grad_force`` =
+    dL/dF (device float32 (N, 3)) of the field forces of (coupling, sigma).
+    ``reuse``: the binning and density spectra of the last ``moments`` call
+    on ``pm`` (same positions and types) are used."""
+    T = int(n_types)
+    dA = (ctypes.c_double * (T * T))()
+    ds = ctypes.c_double(0.0)
+    n = int(grad_force.shape[0])
+    call("hymd_field_adjoint", pm.handle, None if reuse else dptr(x32, torch.float32, "positions"),
+         None if reuse else dptr(types, torch.int32, "types"), n, T,
+         darr(mass if mass is not None else [1.0] * T), float(sigma), darr(np.ravel(coupling)),
+         dptr(grad_force.contiguous(), torch.float32, "grad_force"), 1 if reuse else 0, dA, ctypes.byref(ds))
+    return np.array(dA[:]).reshape(T, T), ds.value
+
+
+class FieldEnergyForces(torch.autograd.Function):
+    """(E, F) = FieldEnergyForces.apply(x, chi, kappa, sigma, pm, types, rho0, a).
+
+    E as FieldEnergy; F (N, 3) float32 the field forces (compute_field_force of
+    update_field).  Backward for a loss L(E, F): dL/dchi (per entry, as
+    FieldEnergy), dL/dkappa and dL/dsigma combine the energy's moments with
+    the adjoint pass over dL/dF; dL/dx is -F dL/dE.  The force output's
+    contribution to dL/dx (a Hessian product) is not provided: backward raises
+    HymdError when x needs a gradient and dL/dF is non-zero."""
+
+    @staticmethod
+    def forward(ctx, x, chi, kappa, sigma, pm, types, rho0, a):
+        T = chi.shape[0]
+        x32 = x.detach().to(torch.float32).contiguous()
+        ty = types.to(torch.int32).contiguous()
+        k = float(kappa)
+        sg = float(sigma)
+        chi_np = chi.detach().cpu().numpy().astype(np.float64)
+        chi_np = 0.5 * (chi_np + chi_np.T)
+        np.fill_diagonal(chi_np, 0.0)
+        inv = 1.0 / (k * rho0)
+        A = np.full((T, T), inv) + chi_np / rho0
+        b = np.full(T, -a * inv)
+        fm = pm.create("real", batch=(T, 3))
+        P, Q, phi0 = moments(pm, x32, ty, T, sg, coupling=A, shift=b, force_mesh=fm)
+        E, dchi, dk, ds, _ = energy_and_gradients(P, Q, phi0, chi_np, k, rho0, a, sg, np.prod(pm.BoxSize))
+        F = torch.empty((x32.shape[0], 3), dtype=torch.float32, device=x32.device)
+        pm.readout_forces(x32, ty, T, fm, F)
+        ctx.save_for_backward(F, x32, ty)
+        ctx.pm, ctx.T, ctx.A, ctx.sigma, ctx.kappa, ctx.rho0 = pm, T, A, sg, k, rho0
+        ctx.grads = (0.5 * dchi, dk, ds)
+        ctx.x_dtype = x.dtype
+        ctx.chi_meta = (chi.dtype, chi.device)
+        return torch.as_tensor(E, dtype=torch.float64, device=x.device), F
+
+    @staticmethod
+    def backward(ctx, grad_e, grad_f):
+        F, x32, ty = ctx.saved_tensors
+        ge = 0.0 if grad_e is None else float(grad_e)
+        dchi_e, dk_e, ds_e = ctx.grads
+        dchi = ge * dchi_e
+        dk = ge * dk_e
+        ds = ge * ds_e
+        has_f = grad_f is not None and bool(torch.any(grad_f != 0))
+        if has_f:
+            if ctx.needs_input_grad[0]:
+                raise HymdError("dL/dx through the force output (a Hessian product) is not provided")
+            dA, dsig = field_adjoint(ctx.pm, x32, ty, ctx.T, ctx.sigma, ctx.A, grad_f.to(torch.float32))
+            sym = (dA + dA.T) / ctx.rho0
+            np.fill_diagonal(sym, 0.0)
+            dchi = dchi + 0.5 * sym
+            dk = dk - float(np.sum(dA)) / (ctx.kappa * ctx.kappa * ctx.rho0)
+            ds = ds + dsig
+        dx = None
+        if ctx.needs_input_grad[0]:
+            dx = (-F.to(torch.float64) * ge).to(ctx.x_dtype)
+        cdt, cdev = ctx.chi_meta
+        return (dx, torch.as_tensor(dchi, dtype=cdt, device=cdev), torch.as_tensor(dk, dtype=torch.float64),
+                torch.as_tensor(ds, dtype=torch.float64), None, None, None, None)
+
+
+def field_energy_forces(x, chi, kappa, sigma, pm, types, rho0, a):
+    """Differentiable field energy and forces (see FieldEnergyForces)."""
+    return FieldEnergyForces.apply(x, chi, kappa, sigma, pm, types, rho0, a)

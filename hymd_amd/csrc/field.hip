@@ -12,6 +12,7 @@
 // F_{t,d}(k) = -i k_d V_t(k).  That takes T r2c + 3T c2r per update instead
 // of the reference's 6T transforms (field.py:573-616).
 #include <cmath>
+#include <cstring>
 
 #include "fft.h"
 #include "field_dev.h"
@@ -595,6 +596,92 @@ __global__ void __launch_bounds__(kBlock) field_moments_kernel(const float2 *__r
                         partials + (int64_t)blockIdx.x * (2 * (MAXT * (MAXT + 1) / 2) + MAXT));
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the force chain F = readout(c2r(-i k_d H A H rho^)) with respect
+// to the coupling A and sigma.  With the upstream gradient g = dL/dF painted
+// per type t and component d (CIC paint is the adjoint of the CIC readout)
+// into P_td, and D_t = sum_d i k_d FFT(P_td):
+//   dL/dA_ts   = 1/G sum_k H(k)^2 Re(conj(D_t) rho^_s)           (raw FFTs)
+//   dL/dsigma  = -2 sigma 1/G sum_k |k|^2 H^2 Re(conj(D_t) (A rho^)_t)
+// summed over the full spectrum through the half-spectrum weights; the
+// k = 0 shift b has no gradient there, so it carries no force term.
+// ---------------------------------------------------------------------------
+
+// Binned slot j of the unweighted binning: ws[d * n + j] = g[perm[j], d];
+// the largest |g| goes to *maxbits as float bits (+inf for a non-finite g).
+__global__ void __launch_bounds__(kBlock) adjoint_gather_kernel(const float *__restrict__ g,
+                                                                 const int *__restrict__ perm, int64_t n,
+                                                                 float *__restrict__ ws, unsigned *maxbits) {
+    float mx = 0.f;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const int64_t i = perm[j];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const float v = g[3 * i + d];
+            ws[d * n + j] = v;
+            mx = isfinite(v) ? fmaxf(mx, fabsf(v)) : INFINITY;
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_down(mx, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(maxbits, __float_as_uint(mx));
+}
+
+struct AdjK {
+    double A[8][8];
+};
+
+// One partial row per block: [MAXT x MAXT] sum w H^2 Re(conj(D_t) rho_s), then
+// sum w |k|^2 H^2 Re(conj(D_t) (A rho)_t).  spec: T raw density spectra;
+// pspec: 3T raw spectra of the painted gradient, component-major [d][t].
+template <int MAXT>
+__global__ void __launch_bounds__(kBlock) adjoint_moments_kernel(const float2 *__restrict__ spec,
+                                                                  const float2 *__restrict__ pspec, KGeom g, int T,
+                                                                  double sigma2, AdjK p,
+                                                                  double *__restrict__ partials) {
+    constexpr int NV = MAXT * MAXT + 1;
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; kk < g.gc; kk += stride) {
+        double k[3], kd[3], wgt;
+        kvec(g, kk, k, kd, wgt);
+        const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        const double h2 = wgt * exp(-sigma2 * k2);
+        double rr[MAXT], ri[MAXT];
+#pragma unroll
+        for (int s = 0; s < MAXT; ++s) {
+            const float2 v = (s < T) ? spec[(int64_t)s * g.gc + kk] : make_float2(0.f, 0.f);
+            rr[s] = v.x;
+            ri[s] = v.y;
+        }
+        double sig = 0.0;
+#pragma unroll
+        for (int t = 0; t < MAXT; ++t) {
+            if (t >= T) continue;
+            // D_t = sum_d i k_d P_td
+            double dr = 0.0, di = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const float2 q = pspec[((int64_t)d * T + t) * g.gc + kk];
+                dr -= kd[d] * (double)q.y;
+                di += kd[d] * (double)q.x;
+            }
+            double arr = 0.0, ari = 0.0;
+#pragma unroll
+            for (int s = 0; s < MAXT; ++s) {
+                acc[t * MAXT + s] += h2 * (dr * rr[s] + di * ri[s]);
+                arr += p.A[t][s] * rr[s];
+                ari += p.A[t][s] * ri[s];
+            }
+            sig += dr * arr + di * ari;
+        }
+        acc[MAXT * MAXT] += k2 * h2 * sig;
+    }
+    block_reduce_store<NV>(acc, partials + (int64_t)blockIdx.x * NV);
+}
+
 // comp_pressure's field terms (pressure.py:9-179) from the filtered spectra
 // phi~_t (pmesh normalisation) by Parseval, summed over the full spectrum
 // through the half-spectrum weights, with V_t = (A phi~)_t:
@@ -783,6 +870,11 @@ struct hymd_pm {
     DevBuf counts, start, tiles;  // per mesh row (counts kept zero between sorts)
     int64_t sorted_n = 0;
     bool sorted_weights = false;
+    // spec holds the unnormalised spectra of the densities of the binned set
+    // (left by hymd_field_moments; any new binning clears it)
+    bool density_spectra = false;
+    int density_types = 0;
+    DevBuf adj;       // adjoint: 3 x n binned upstream gradient components, then their max |g| bits
     uint64_t box_version = 0;  // hymd_pm_set_box calls
 };
 
@@ -1052,6 +1144,7 @@ static int sort_particles(hymd_pm *pm, const float *x, const int32_t *types, con
                                 (int *)pm->perm.ptr, (float4 *)pm->xs.ptr, w ? (float *)pm->ws.ptr : nullptr, s));
     pm->sorted_n = n;
     pm->sorted_weights = (w != nullptr);
+    pm->density_spectra = false;
     return HYMD_OK;
 }
 
@@ -1321,6 +1414,7 @@ int row_order(hymd_pm *pm, const float *x, int64_t n, int *order, hipStream_t s)
     row_order_kernel<<<grid_for(n), kBlock, 0, s>>>((const int2 *)pm->keys.ptr, n, (const int *)pm->start.ptr, order);
     HYMD_HIP(hipGetLastError());
     pm->sorted_n = 0;  // the last binned set is gone
+    pm->density_spectra = false;
     return HYMD_OK;
 }
 }  // namespace hymd
@@ -1374,6 +1468,124 @@ extern "C" int hymd_field_moments(hymd_pm *pm, const float *x, const int32_t *ty
     }
     HYMD_HIP(hipStreamSynchronize(s));
     for (int j = 0; j < nv; ++j) moments[j] = ((double *)pm->host.ptr)[j];
+    pm->density_spectra = true;
+    pm->density_types = T;
+    return HYMD_OK;
+}
+
+// Force grids (and the forces of the binned particles) of a quadratic
+// Hamiltonian from the density spectra held by the last hymd_field_moments
+// call: the k-space pass and the inverse transforms, no binning or paint.
+extern "C" int hymd_field_forces_held(hymd_pm *pm, int32_t n_types, const double *coupling, const double *shift,
+                                      double sigma, float *force_mesh, float *force) {
+    HYMD_REQUIRE(pm && coupling && force_mesh, "null argument");
+    HYMD_REQUIRE(sigma >= 0.0 && std::isfinite(sigma), "sigma must be finite and >= 0");
+    HYMD_REQUIRE(pm->density_spectra && pm->density_types == n_types,
+                 "no density spectra of %d types are held (call hymd_field_moments first)", n_types);
+    const Mesh &m = pm->mesh;
+    const int T = n_types;
+    hipStream_t s = pm->stream;
+    FieldK p = field_params(T, coupling, shift, 0.0, sigma, m);
+    float2 *grad = (float2 *)pm->grad.ptr;
+    KGeom g = kgeom(pm);
+    if (T <= 4)
+        field_kspace_kernel<4><<<pm->kblocks, kBlock, 0, s>>>((float2 *)pm->spec.ptr, grad, nullptr, 0, g, p, nullptr);
+    else
+        field_kspace_kernel<8><<<pm->kblocks, kBlock, 0, s>>>((float2 *)pm->spec.ptr, grad, nullptr, 0, g, p, nullptr);
+    HYMD_HIP(hipGetLastError());
+    HYMD_TRY(fft_c2r(pm->fft, grad, force_mesh, 3 * T, 1.f, s));
+    if (force) HYMD_TRY(readout_sorted(pm, 3, T, force_mesh, force, false));
+    return HYMD_OK;
+}
+
+// Adjoint of hymd_compute_field_force o hymd_update_field (coupling A,
+// sigma): dL/dA and dL/dsigma for the upstream gradient grad_force = dL/dF.
+extern "C" int hymd_field_adjoint(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
+                                  const double *mass, double sigma, const double *coupling,
+                                  const float *grad_force, int32_t reuse, double *d_coupling, double *d_sigma) {
+    HYMD_REQUIRE(pm && coupling && d_coupling && (grad_force || n == 0) && n >= 0, "bad argument");
+    HYMD_REQUIRE(n_types >= 1 && n_types <= 8 && n_types <= pm->max_types, "the adjoint supports n_types <= 8");
+    HYMD_REQUIRE(sigma >= 0.0 && std::isfinite(sigma), "sigma must be finite and >= 0");
+    const Mesh &m = pm->mesh;
+    const int T = n_types;
+    hipStream_t s = pm->stream;
+    float2 *spec = (float2 *)pm->spec.ptr;
+    if (reuse) {
+        HYMD_REQUIRE(pm->density_spectra && pm->density_types == T && pm->sorted_n == n && !pm->sorted_weights,
+                     "reuse needs the binning and density spectra of a hymd_field_moments call on the same "
+                     "particles and types");
+    } else {
+        HYMD_REQUIRE((x || n == 0) && (types || n == 0), "null positions or types");
+        HYMD_TRY(sort_particles(pm, x, types, nullptr, n));
+        HYMD_TRY(paint_sorted(pm, T, mass, (float *)pm->paint.ptr, nullptr));
+        HYMD_TRY(fft_r2c_real(pm->fft, (const float *)pm->paint.ptr, spec, T, s));
+        pm->density_spectra = true;
+        pm->density_types = T;
+    }
+    for (int j = 0; j < T * T; ++j) d_coupling[j] = 0.0;
+    if (d_sigma) *d_sigma = 0.0;
+    if (n == 0) return HYMD_OK;
+    // 1. upstream gradient into binned order, and its largest modulus
+    HYMD_TRY(pm->adj.reserve(sizeof(float) * (3 * (size_t)n + 64)));
+    float *ws = (float *)pm->adj.ptr;
+    unsigned *maxbits = (unsigned *)(ws + 3 * n);
+    HYMD_TRY(zero_words(maxbits, 1, s));
+    adjoint_gather_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(grad_force, (const int *)pm->perm.ptr, n, ws,
+                                                                       maxbits);
+    HYMD_HIP(hipGetLastError());
+    unsigned hbits = 0;
+    HYMD_HIP(hipMemcpyAsync(&hbits, maxbits, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    HYMD_HIP(hipStreamSynchronize(s));
+    float gmax;
+    std::memcpy(&gmax, &hbits, sizeof(float));
+    HYMD_REQUIRE(std::isfinite(gmax), "grad_force holds a non-finite value");
+    if (gmax == 0.f) return HYMD_OK;
+    // 2. CIC paint of each component per type (the readout's adjoint), as
+    // int64 fixed point scaled so that the largest deposit is ~2^36
+    int ex = 0;
+    (void)std::frexp((double)gmax, &ex);
+    PaintArgs pa;
+    pa.n_types = T;
+    pa.scale = std::ldexp(1.0, 36 - ex);
+    for (int t = 0; t < HYMD_MAX_TYPES; ++t) pa.mass[t] = 1.0;
+    HYMD_TRY(pm->rscratch.reserve(sizeof(float) * 3 * (size_t)T * m.g));
+    float *pg = (float *)pm->rscratch.ptr;
+    const int tc = std::max(1, std::min(T, (int)(kRowLdsBytes / (sizeof(long long) * m.mz))));
+    dim3 grid((unsigned)(m.mx * m.my), (unsigned)((T + tc - 1) / tc));
+    for (int d = 0; d < 3; ++d)
+        paint_rows_kernel<<<grid, kBlock, sizeof(long long) * tc * m.mz, s>>>(
+            (const float4 *)pm->xs.ptr, ws + d * n, (const int *)pm->start.ptr, m, pa, tc, 1.0 / pa.scale,
+            pg + (int64_t)d * T * m.g, nullptr);
+    HYMD_HIP(hipGetLastError());
+    // 3. their spectra, 4. the adjoint moments
+    float2 *pspec = (float2 *)pm->grad.ptr;
+    HYMD_TRY(fft_r2c_real(pm->fft, pg, pspec, 3 * T, s));
+    KGeom g = kgeom(pm);
+    AdjK ak{};
+    for (int t = 0; t < T; ++t)
+        for (int u = 0; u < T; ++u) ak.A[t][u] = coupling[t * T + u];
+    const int nb = moments_blocks(m.gc);
+    const int mt = T <= 2 ? 2 : (T <= 4 ? 4 : 8);
+    const int nv = mt * mt + 1;
+    HYMD_TRY(pm->mpart.reserve(sizeof(double) * (size_t)nv * nb));
+    HYMD_TRY(pm->result.reserve(sizeof(double) * nv));
+    HYMD_TRY(pm->host.reserve(sizeof(double) * nv));
+    double *part = (double *)pm->mpart.ptr;
+    if (mt == 2)
+        adjoint_moments_kernel<2><<<nb, kBlock, 0, s>>>(spec, pspec, g, T, sigma * sigma, ak, part);
+    else if (mt == 4)
+        adjoint_moments_kernel<4><<<nb, kBlock, 0, s>>>(spec, pspec, g, T, sigma * sigma, ak, part);
+    else
+        adjoint_moments_kernel<8><<<nb, kBlock, 0, s>>>(spec, pspec, g, T, sigma * sigma, ak, part);
+    HYMD_HIP(hipGetLastError());
+    HYMD_TRY(reduce_partials(part, nb, nv, (double *)pm->result.ptr, s));
+    HYMD_HIP(hipMemcpyAsync(pm->host.ptr, pm->result.ptr, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+    HYMD_HIP(hipStreamSynchronize(s));
+    const double *r = (const double *)pm->host.ptr;
+    const double inv_g = 1.0 / (double)m.g;
+    for (int t = 0; t < T; ++t)
+        for (int u = 0; u < T; ++u) d_coupling[t * T + u] = r[t * mt + u] * inv_g;
+    if (d_sigma) *d_sigma = -2.0 * sigma * r[mt * mt] * inv_g;
     return HYMD_OK;
 }
 

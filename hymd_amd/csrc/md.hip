@@ -71,7 +71,7 @@ struct hymd_md {
     int64_t *order = nullptr, *where = nullptr;  // caller's: state row -> input row, input row -> state row
     DevBuf rperm, rtmp;
     // optional HIP-event timing of one stage in eager steps
-    int timed = 0;  // 0 none, 1 bonded forces, 2 kick+drift, 3 update_field, 4 readout
+    int timed = 0;  // 0 none, 1 bonded forces, 2 kick+drift, 3 update_field, 4 readout, 5 PME (+ dipoles)
     std::vector<hipEvent_t> ev;
     size_t ev_used = 0;
 };
@@ -172,6 +172,10 @@ static void mark(hymd_md *m, int stage) {
 
 extern "C" int hymd_md_set_fused(hymd_md *m, int32_t enable);
 
+// The slow forces the outer kicks apply: the field forces alone, or the
+// caller's sum with the PME (and redistributed dipole) forces.
+static float *slow_force(hymd_md *m) { return (m->charges || m->dih) ? m->f_slow : m->f_field; }
+
 // Slow forces beyond the field ones (main.py:1044-1095): f_slow = f_field
 // + PME forces of the charges + the dipole-charge PME forces redistributed
 // onto the backbone, in the order (and rounding) of the Python step.
@@ -211,7 +215,7 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
         // the inner loop also writes the field path's row keys of the new positions
         RowKeyTarget key;
         HYMD_TRY(field_key_target(m->pm, m->n, &key));
-        HYMD_TRY(fused_inner_impl(m->fused, m->x, m->v, m->x32, m->f_fast, m->f_field, half_in, half_out, m->dt,
+        HYMD_TRY(fused_inner_impl(m->fused, m->x, m->v, m->x32, m->f_fast, slow_force(m), half_in, half_out, m->dt,
                                   m->respa_inner, pending, m->box, energies ? eb : nullptr, nullptr, m->work, &key));
         mark(m, 1);
         if (energies) {
@@ -225,6 +229,12 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
         mark(m, 4);
         HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));
         mark(m, 4);
+        // PME of the charges (field.py:241-403) into f_slow = f_field + f_elec
+        if (m->charges) {
+            mark(m, 5);
+            HYMD_TRY(peptide_slow_forces(m, energies != nullptr));
+            mark(m, 5);
+        }
         return HYMD_OK;
     }
     const bool extra = m->charges || m->dih;
@@ -261,7 +271,11 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
     mark(m, 4);
     HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));
     mark(m, 4);
-    if (extra) HYMD_TRY(peptide_slow_forces(m, energies != nullptr));
+    if (extra) {
+        mark(m, 5);
+        HYMD_TRY(peptide_slow_forces(m, energies != nullptr));
+        mark(m, 5);
+    }
     HYMD_TRY(hymd_respa_kick(m->v, m->n, m->f_fast, half_in, f_slow, half_out, m->work));
     return HYMD_OK;
 }
@@ -384,7 +398,7 @@ static void check_pm(hymd_md *m) {
 static int finish_pending(hymd_md *m) {
     if (!m->fused || !m->pending) return HYMD_OK;
     const double half_out = 0.5 * m->respa_inner * m->dt / m->mass;
-    HYMD_TRY(hymd_respa_kick(m->v, m->n, m->f_field, half_out, nullptr, 0.0, m->work));
+    HYMD_TRY(hymd_respa_kick(m->v, m->n, slow_force(m), half_out, nullptr, 0.0, m->work));
     m->pending = 0;
     return HYMD_OK;
 }
@@ -404,7 +418,7 @@ static int couple_step(hymd_md *m) {
         const double *R = m->draws.data() + m->draw_next * 2 * G;
         const bool pend = m->fused && m->pending;
         const double half_out = 0.5 * m->respa_inner * m->dt / m->mass;
-        HYMD_TRY(csvr_enqueue(m->thermo, m->v, m->types, m->n, R, R + G, pend ? m->f_field : nullptr,
+        HYMD_TRY(csvr_enqueue(m->thermo, m->v, m->types, m->n, R, R + G, pend ? slow_force(m) : nullptr,
                               pend ? half_out : 0.0, m->thermo_work, m->thermo_ws, m->work));
         m->pending = 0;
         if (++m->draw_next * 2 * G >= m->draws.size()) {
@@ -465,8 +479,8 @@ extern "C" int hymd_md_step(hymd_md *m, int32_t n_steps, int32_t use_graph, doub
 
 extern "C" int hymd_md_set_fused(hymd_md *m, int32_t enable) {
     HYMD_REQUIRE(m, "null handle");
-    if (enable && (m->charges || m->dih)) {
-        set_error("the fused inner loop has no dihedral or PME terms");
+    if (enable && m->dih) {
+        set_error("the fused inner loop has no dihedral terms");
         return HYMD_ERR_UNSUPPORTED;
     }
     HYMD_HIP(hipStreamSynchronize(m->work));
@@ -562,6 +576,10 @@ extern "C" int hymd_md_set_reorder(hymd_md *m, int32_t every, int32_t *types, in
         set_error("reordering needs the fused inner loop (one-particle molecules without terms)");
         return HYMD_ERR_UNSUPPORTED;
     }
+    if (m->charges) {
+        set_error("reordering would leave the caller's charges in the old order");
+        return HYMD_ERR_UNSUPPORTED;
+    }
     HYMD_REQUIRE(types == m->types && order && where, "types must be the handle's array; order and where non-null");
     m->types_rw = types;
     m->order = order;
@@ -578,9 +596,10 @@ extern "C" int64_t hymd_md_single_tail(hymd_md *m) { return (m && m->fused) ? fu
 // Time one stage of the eager steps that follow with HIP events recorded on
 // the launch stream around every launch of that stage (graph replays are not
 // timed).  stage: 0 off, 1 bonded forces, 2 kick+drift, 3 update_field (all its
-// kernels), 4 readout.  Up to max_launches launches are recorded.
+// kernels), 4 readout, 5 PME of the charges (and dipoles).  Up to max_launches
+// launches are recorded.
 extern "C" int hymd_md_time_stage(hymd_md *m, int32_t stage, int32_t max_launches) {
-    HYMD_REQUIRE(m && stage >= 0 && stage <= 4 && max_launches >= 0, "bad argument");
+    HYMD_REQUIRE(m && stage >= 0 && stage <= 5 && max_launches >= 0, "bad argument");
     HYMD_HIP(hipStreamSynchronize(m->work));
     for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
     m->ev.clear();
@@ -615,7 +634,10 @@ extern "C" int hymd_md_set_peptide(hymd_md *m, const float *charges, double elec
     HYMD_REQUIRE(f_slow || (!charges && !dih), "f_slow is required with charges or dihedrals");
     HYMD_REQUIRE(!dih || n_dihedrals >= 0, "bad dihedral count");
     HYMD_HIP(hipStreamSynchronize(m->work));
-    if (charges || dih) HYMD_TRY(hymd_md_set_fused(m, 0));  // the fused inner loop has no dihedral terms
+    if (dih) HYMD_TRY(hymd_md_set_fused(m, 0));  // the fused inner loop has no dihedral terms
+    // the periodic reorder permutes the handle's own per-particle arrays,
+    // not the caller's charges or summed slow forces
+    if (charges) m->reorder_every = 0;
     drop_graphs(m);
     m->charges = charges;
     m->elec = elec_factor;

@@ -20,8 +20,9 @@ main.py:508-588, 802-887, 1044-1095 do: the dihedral forces (FP64 kernel,
 dipoles reconstructed on the last inner step) to the fast forces, and the
 PME forces of the charges plus the dipole-charge PME forces redistributed
 onto the backbone (force.py:855-877) to the slow forces, both in ``step()``
-and in the library's captured outer step (``run()``), which then uses the
-per-kernel inner loop.
+and in the library's captured outer step (``run()``).  With dihedrals that
+step uses the per-kernel inner loop; with charges alone it keeps the fused
+one (and no periodic reorder, which would leave the charges behind).
 """
 import ctypes
 
@@ -163,7 +164,7 @@ class RespaMD:
                  darr(self.type_mass), self.sigma, self.mass, self.dt, self.n_inner, self._box_c)
             self._native = h
             call("hymd_md_set_term_precision", h, _lib.HYMD_F32 if self.term_precision == "fp32" else _lib.HYMD_F64)
-            if self.reorder_every > 0 and _lib.lib().hymd_md_is_fused(h):
+            if self.reorder_every > 0 and self.charges is None and _lib.lib().hymd_md_is_fused(h):
                 call("hymd_md_set_reorder", h, self.reorder_every, dptr(self.types, torch.int32, "types"),
                      dptr(self._order, torch.int64, "order"), dptr(self._inverse, torch.int64, "where"))
             if self.extra_terms:
@@ -191,9 +192,10 @@ class RespaMD:
         e = (ctypes.c_double * 5)() if energies else None
         h = self._native_handle()
         call("hymd_md_step", h, int(n_steps), 1 if use_graph else 0, e, _lib.stream_ptr())
-        if self.dihedrals is not None:   # the library's dipoles / transfer matrices are the current ones
-            x2 = (ctypes.c_double * 2)()
-            call("hymd_md_peptide_state", h, dptr(self.dipoles), dptr(self.tmat), x2)
+        x2 = (ctypes.c_double * 2)()
+        if self.extra_terms:   # the library's dipoles / transfer matrices are the current ones
+            call("hymd_md_peptide_state", h, dptr(self.dipoles) if self.dihedrals is not None else None,
+                 dptr(self.tmat) if self.dihedrals is not None else None, x2)
         if energies:
             self.energies.update(field=e[0], bond=e[1], angle=e[2], kinetic=e[3])
             if self.dihedrals is not None:
@@ -255,7 +257,7 @@ class RespaMD:
         """Thermostat work of ``simulate`` so far (config.thermostat_work)."""
         return 0.0 if self._thermostat is None else float(self._thermostat.work.item())
 
-    STAGES = {"bonded": 1, "kick_drift": 2, "field": 3, "readout": 4}
+    STAGES = {"bonded": 1, "kick_drift": 2, "field": 3, "readout": 4, "pme": 5}
 
     def time_stage(self, name, max_launches=10000):
         """Record HIP events around every launch of one stage in the eager

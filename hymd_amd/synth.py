@@ -28,6 +28,17 @@ mass motion removed.  numpy PCG64 streams, seed = 1000 + workload index.
                      (side beads and solvent) carry +-1 with zero net charge;
                      dielectric 45; chi 4x4 symmetric U[0,30]; kappa 0.03;
                      sigma 0.5; dt 0.03; respa_inner 1.
+* ``markov_melt``    16,777,216 beads, 100 nm box, 256^3 mesh, 6 types in
+                     1,048,576 sixteen-bead linear chains, each chain's types a
+                     Markov sequence (first bead uniform; seeded 6x6 transition
+                     matrix U[0,1) + 3 I, rows normalised, so runs of one type
+                     are favoured); bonds r0 = 0.47, k = 1250 and angles
+                     theta0 = 180 deg, k = 25 (the values of the other chain
+                     workloads; the workload leaves them open); chi 6x6
+                     symmetric U[0,30]; kappa 0.02; sigma 0.6; 5 % of the beads
+                     carry +-1 with zero net charge, PME on the same mesh with
+                     dielectric 45 (left open, fixed as in peptide_solvent);
+                     dt 0.03, respa_inner 1, 200 steps.
 * ``replica_set``    64 independent A5B5 diblock melts for the batched
                      chi-gradient workload: N_r ~ U[50k, 400k] rounded down to
                      whole 10-bead chains, L_r = (N_r / 8.4)^(1/3) nm, mesh
@@ -206,6 +217,44 @@ def peptide_solvent(n_chains=8738, n_res=40, n_solvent=1048592, box=50.0, mesh=1
                0.03, 0.5, 0.03, 1, 500, charges=charges, dielectric=45.0)
     s.extra["dihedrals"] = dihedrals
     return s
+
+
+def markov_types(rng, n_chains, length, n_types, stay=3.0):
+    """(n_chains * length,) int32 bead types: per chain a Markov sequence with
+    a uniform first bead and the seeded transition matrix U[0,1) + stay I
+    (rows normalised)."""
+    P = rng.uniform(0.0, 1.0, (n_types, n_types)) + stay * np.eye(n_types)
+    cum = np.cumsum(P / P.sum(axis=1, keepdims=True), axis=1)
+    cum[:, -1] = 1.0
+    t = np.empty((n_chains, length), dtype=np.int32)
+    t[:, 0] = rng.integers(0, n_types, n_chains)
+    for j in range(1, length):
+        u = rng.uniform(0.0, 1.0, n_chains)
+        t[:, j] = np.minimum((u[:, None] >= cum[t[:, j - 1]]).sum(axis=1), n_types - 1)
+    return t.ravel()
+
+
+def markov_melt(n_chains=1048576, length=16, box=100.0, mesh=256, n_types=6, charged=0.05, seed=1004):
+    rng = np.random.default_rng(seed)
+    box = np.full(3, float(box))
+    x = random_walk_chains(rng, n_chains, length, box)
+    types = markov_types(rng, n_chains, length, n_types)
+    (ba, bb), (aa, ab, ac) = chain_terms(n_chains, length)
+    bonds = (ba, bb, np.full(len(ba), 0.47), np.full(len(ba), 1250.0))
+    angles = (aa, ab, ac, np.full(len(aa), np.pi), np.full(len(aa), 25.0))
+    c = rng.uniform(0.0, 30.0, (n_types, n_types))
+    chi = np.triu(c, 1)
+    chi = chi + chi.T
+    n = len(types)
+    n_q = int(n * charged) // 2 * 2
+    pick = rng.choice(n, n_q, replace=False)
+    charges = np.zeros(n, dtype=np.float32)
+    charges[pick[: n_q // 2]] = 1.0
+    charges[pick[n_q // 2:]] = -1.0
+    v = maxwell_velocities(rng, n, 72.0)
+    names = [chr(ord("A") + t) for t in range(n_types)]
+    return System("markov_melt", box, (mesh,) * 3, n_types, names, types, x, v, bonds, angles, chi, 0.02, 0.6, 0.03,
+                  1, 200, charges=charges if n_q else None, dielectric=45.0)
 
 
 @dataclass

@@ -117,6 +117,35 @@ int hymd_field_moments(hymd_pm *pm, const float *x, const int32_t *types, int64_
                        const double *mass, double sigma, double *moments, const double *coupling,
                        const double *shift, float *force_mesh);
 
+/* Field forces of the quadratic form (coupling, shift) and sigma from the
+ * density spectra and binning held by the last hymd_field_moments call on
+ * this handle (frozen positions, e.g. a parameter sweep; n_types <= 8 and
+ * equal to that call's): force_mesh device float32 (n_types,3,Mx,My,Mz) out,
+ * force device float32 (n,3) out or NULL, rows as that call's positions.
+ * The k-space pass and inverse transforms of update_field (field.py:580-613)
+ * and the readout of compute_field_force (:152-200), without binning or
+ * paint.  Asynchronous. */
+int hymd_field_forces_held(hymd_pm *pm, int32_t n_types, const double *coupling, const double *shift,
+                           double sigma, float *force_mesh, float *force);
+
+/* Adjoint of the force chain compute_field_force o update_field with respect
+ * to the coupling A and sigma (no reference counterpart: the backward pass of
+ * the differentiable field forces).  For a loss L of the field forces
+ * F = readout(c2r(-i k H A H rho^)) of (coupling, sigma), and its gradient
+ * grad_force = dL/dF (device float32 (n,3), the rows of x):
+ *   d_coupling host double (n_types, n_types) out: dL/dA_ts (t: the type of
+ *              the particles whose forces are read out, s: the density)
+ *   d_sigma    host double out or NULL: dL/dsigma at fixed A
+ * Chain: CIC paint of each gradient component per type (the readout's
+ * adjoint, int64 fixed point scaled to the largest |g|), r2c of the 3 n_types
+ * grids, one reduction against the density spectra.  n_types <= 8.
+ * reuse != 0: x and types are not read; the binning and density spectra of
+ * the last hymd_field_moments call on this handle are used (same particles,
+ * types and masses: frozen positions).  Synchronises the handle's stream. */
+int hymd_field_adjoint(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
+                       const double *mass, double sigma, const double *coupling, const float *grad_force,
+                       int32_t reuse, double *d_coupling, double *d_sigma);
+
 /* comp_pressure's field terms (pressure.py:9-179, comp_laplacian
  * pressure.py:182-200) from the filtered spectra of update_field:
  *   phi_fourier device complex64 (n_types,Mx,My,Mz/2+1), pmesh normalisation (n_types <= 8)
@@ -429,7 +458,7 @@ int hymd_md_set_reorder(hymd_md *md, int32_t every, int32_t *types, int64_t *ord
 /* first row of the one-particle tail that hymd_md_set_reorder permutes */
 int64_t hymd_md_single_tail(hymd_md *md);
 /* HIP-event timing of one stage (1 bonded forces, 2 kick+drift, 3 update_field,
- * 4 readout; 0 off) in the eager steps that follow, recorded on the launch
+ * 4 readout, 5 PME of the charges and dipoles; 0 off) in the eager steps that follow, recorded on the launch
  * stream around each launch; at most max_launches are recorded */
 int hymd_md_time_stage(hymd_md *md, int32_t stage, int32_t max_launches);
 /* total milliseconds and count of the timed launches so far (synchronises) */

@@ -59,12 +59,13 @@ def test_paint_empty(hip):
     assert int(pm.paint_fixed(x, t, 1).abs().sum()) == 0
 
 
-# (16, 200, 200): its y-z plane exceeds the LDS budget, so the separate line
-# passes run; the other meshes use the plane kernels, with the runtime plane
+# (16, 200, 200) and (256, 256, 256): their y-z planes exceed the LDS budget,
+# so the separate line passes run; the other meshes use the plane kernels, with the runtime plane
 # geometry or, for y-z planes of 48^2, 64^2, 96^2, 128^2, 160^2, the
 # compile-time one
 @pytest.mark.parametrize("mesh", [(8, 8, 8), (32, 32, 32), (48, 64, 96), (128, 128, 128), (160, 160, 160),
-                                  (20, 30, 36), (16, 200, 200), (12, 48, 48), (10, 64, 64), (6, 96, 96)])
+                                  (20, 30, 36), (16, 200, 200), (12, 48, 48), (10, 64, 64), (6, 96, 96),
+                                  (256, 256, 256)])
 def test_fft_vs_numpy(hip, mesh):
     rng = np.random.default_rng(sum(mesh))
     pm = _pm(mesh, (1.0, 1.0, 1.0), 1)
