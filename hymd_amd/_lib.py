@@ -35,6 +35,8 @@ SIGNATURES = {
     "hymd_pm_destroy": [_p],
     "hymd_pm_set_stream": [_p, _p],
     "hymd_pm_set_box": [_p, _pd],
+    "hymd_pm_set_tuning": [_p, _i32, _i32],
+    "hymd_pm_get_tuning": [_p, _i32, _pi32],
     "hymd_update_field": [_p, _p, _p, _i64, _i32, _pd, _pd, _d, _pd, _d, _p, _p, _p, _p, _pd],
     "hymd_compute_field_force": [_p, _p, _p, _i64, _i32, _p, _p],
     "hymd_update_field_force_q": [_p, _p, _p, _i64, _d, _d, _d, _p, _p, _p, _p, _pd],
@@ -54,6 +56,7 @@ SIGNATURES = {
     "hymd_batch_create": [ctypes.POINTER(_p), _i32, _pi32, _pd, _pi64, _i32, _pd],
     "hymd_batch_destroy": [_p],
     "hymd_batch_moments": [_p, _p, _p, _pd, _pd, _i32, _p],
+    "hymd_batch_forces": [_p, _p, _p, _pd, _pd, _pd, _p, _i32, _p],
     "hymd_paint_fixed": [_p, _p, _p, _p, _i64, _i32, _pd, _p, _i32],
     "hymd_fft_r2c": [_p, _p, _p, _i32],
     "hymd_fft_c2r": [_p, _p, _p, _i32],

@@ -187,6 +187,79 @@ __global__ void __launch_bounds__(kBlock) paint_rows_kernel(const float4 *__rest
     }
 }
 
+// The same row paint with one wavefront per mesh row and kPaintWaveRows rows
+// per workgroup: more rows in flight per CU for rows with few particles.  A
+// wavefront issues the record loads of up to four visits per lane before their
+// deposits.  Same deposits, so the same integer sums and bits.
+constexpr int kPaintWaveRows = kBlock / 64;
+
+__global__ void __launch_bounds__(kBlock) paint_rows_wave_kernel(const float4 *__restrict__ rec,
+                                                                  const float *__restrict__ ws,
+                                                                  const int *__restrict__ start, Mesh m, PaintArgs pa,
+                                                                  int tc, double out_scale, float *__restrict__ out,
+                                                                  long long *__restrict__ out_fixed) {
+#pragma clang fp contract(off)
+    extern __shared__ unsigned long long acc_all[];  // kPaintWaveRows x tc x mz
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nrows = m.mx * m.my;
+    const int row = (int)xcd_block(blockIdx.x, gridDim.x) * kPaintWaveRows + wave;
+    const bool active = row < nrows;
+    const int t0 = (int)blockIdx.y * tc;
+    const int nt = min(tc, pa.n_types - t0);
+    unsigned long long *acc = acc_all + (int64_t)wave * tc * m.mz;
+    const int ix = (active ? row : 0) / m.my, iy = (active ? row : 0) - ix * m.my;
+    for (int i = lane; i < nt * m.mz; i += 64) acc[i] = 0ull;
+    int rs[4], rn[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int a = q >> 1, b = q & 1;
+        const int bx = a ? (ix == 0 ? m.mx - 1 : ix - 1) : ix;
+        const int by = b ? (iy == 0 ? m.my - 1 : iy - 1) : iy;
+        const int br = bx * m.my + by;
+        rs[q] = active ? start[br] : 0;
+        rn[q] = active ? start[br + 1] - rs[q] : 0;
+    }
+    __syncthreads();
+    const int tot = rn[0] + rn[1] + rn[2] + rn[3];
+    for (int v0 = 0; v0 < tot; v0 += 4 * 64) {
+        float4 r[4];
+        int qq[4], pp[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int v = v0 + u * 64 + lane;
+            qq[u] = -1;
+            if (v < tot) {
+                int q = 0, off = v;
+                while (q < 3 && off >= rn[q]) {
+                    off -= rn[q];
+                    ++q;
+                }
+                qq[u] = q;
+                pp[u] = rs[q] + off;
+                r[u] = rec[pp[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (qq[u] < 0) continue;
+            const int t = __float_as_int(r[u].w) - t0;
+            if ((unsigned)t >= (unsigned)nt) continue;
+            const double mass = ws ? (double)ws[pp[u]] : pa.mass[t0 + t];
+            row_deposit(r[u], qq[u] >> 1, qq[u] & 1, mass, pa.scale, m.inv_h, m.mx, m.my, m.mz, acc + t * m.mz);
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    const int64_t rbase = (int64_t)row * m.mz;
+    for (int i = lane; i < nt * m.mz; i += 64) {
+        const int t = i / m.mz, z = i - t * m.mz;
+        const int64_t o = (int64_t)(t0 + t) * m.g + rbase + z;
+        const long long v = (long long)acc[i];
+        out[o] = (float)((double)v * out_scale);
+        if (out_fixed) out_fixed[o] = v;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // readout: out[i, c] = w_i * sum_corners W * grid[type_i * ncomp + c][corner]
 // ---------------------------------------------------------------------------
@@ -875,7 +948,11 @@ struct hymd_pm {
     bool density_spectra = false;
     int density_types = 0;
     DevBuf adj;       // adjoint: 3 x n binned upstream gradient components, then their max |g| bits
-    uint64_t box_version = 0;  // hymd_pm_set_box calls
+    // launch shapes chosen by hymd_pm_set_tuning (an autotuner's choices)
+    int paint_rows = 1;       // mesh rows per paint workgroup: 1 or 4 (one per wavefront)
+    int readout_block = kBlock;
+    int xline_w = 0;          // lines per x-line workgroup, 0: the widest that fits
+    uint64_t box_version = 0;  // hymd_pm_set_box and hymd_pm_set_tuning calls
 };
 
 uint64_t hymd::pm_generation(const hymd_pm *pm) {
@@ -931,16 +1008,20 @@ constexpr int kXlineGradThreads = 256;
 constexpr int kXlineGradElems = 16;  // values per thread and in-place stage
 
 // Lines per workgroup of field_xline_grad_kernel for T types (0: it does not
-// fit, or the y-z plane pass it feeds is unavailable).
-static int xline_grad_width(const FftPlan &f, int T) {
-    if (!f.lds_plane) return 0;
+// fit, or the y-z plane pass it feeds is unavailable): `want` when it fits,
+// else the widest that does.
+static bool xline_grad_fits(const FftPlan &f, int T, int W) {
     const int pmax = 16 / moments_maxt(T);
-    for (int W = 16; W >= 2; W >>= 1) {
-        if ((int64_t)W * f.ax.n > (int64_t)kXlineGradThreads * pmax) continue;
-        if (sizeof(float2) * ((size_t)f.ax.n + (size_t)T * W * f.ax.ld) > kXlineLdsBytes) continue;
-        if (!fft_inplace_fits(f.ax, T * W, kXlineGradThreads, kXlineGradElems)) continue;
-        return W;
-    }
+    if ((int64_t)W * f.ax.n > (int64_t)kXlineGradThreads * pmax) return false;
+    if (sizeof(float2) * ((size_t)f.ax.n + (size_t)T * W * f.ax.ld) > kXlineLdsBytes) return false;
+    return fft_inplace_fits(f.ax, T * W, kXlineGradThreads, kXlineGradElems);
+}
+
+static int xline_grad_width(const FftPlan &f, int T, int want = 0) {
+    if (!f.lds_plane) return 0;
+    if (want > 0 && xline_grad_fits(f, T, want)) return want;
+    for (int W = 16; W >= 2; W >>= 1)
+        if (xline_grad_fits(f, T, W)) return W;
     return 0;
 }
 
@@ -1096,6 +1177,43 @@ extern "C" int hymd_pm_set_stream(hymd_pm *pm, void *stream) {
     return HYMD_OK;
 }
 
+extern "C" int hymd_pm_set_tuning(hymd_pm *pm, int32_t knob, int32_t value) {
+    HYMD_REQUIRE(pm, "null handle");
+    switch (knob) {
+    case HYMD_TUNE_PAINT_ROWS:
+        HYMD_REQUIRE(value == 1 || value == kPaintWaveRows, "paint rows per workgroup must be 1 or %d", kPaintWaveRows);
+        pm->paint_rows = value;
+        break;
+    case HYMD_TUNE_READOUT_BLOCK:
+        HYMD_REQUIRE(value == 64 || value == 128 || value == 256, "readout block must be 64, 128 or 256");
+        pm->readout_block = value;
+        break;
+    case HYMD_TUNE_XLINE_WIDTH:
+        HYMD_REQUIRE(value == 0 || value == 2 || value == 4 || value == 8 || value == 16,
+                     "x-line width must be 0, 2, 4, 8 or 16");
+        pm->xline_w = value;
+        break;
+    default:
+        set_error("unknown tuning knob %d", knob);
+        return HYMD_ERR_INVALID;
+    }
+    ++pm->box_version;  // captured steps replay the old launch shapes: capture again
+    return HYMD_OK;
+}
+
+extern "C" int hymd_pm_get_tuning(hymd_pm *pm, int32_t knob, int32_t *value) {
+    HYMD_REQUIRE(pm && value, "null argument");
+    switch (knob) {
+    case HYMD_TUNE_PAINT_ROWS: *value = pm->paint_rows; break;
+    case HYMD_TUNE_READOUT_BLOCK: *value = pm->readout_block; break;
+    case HYMD_TUNE_XLINE_WIDTH: *value = pm->xline_w; break;
+    default:
+        set_error("unknown tuning knob %d", knob);
+        return HYMD_ERR_INVALID;
+    }
+    return HYMD_OK;
+}
+
 extern "C" int hymd_pm_set_box(hymd_pm *pm, const double box[3]) {
     HYMD_REQUIRE(pm && box, "null argument");
     int32_t mesh[3] = {pm->mesh.mx, pm->mesh.my, pm->mesh.mz};
@@ -1148,6 +1266,28 @@ static int sort_particles(hymd_pm *pm, const float *x, const int32_t *types, con
     return HYMD_OK;
 }
 
+// Row paint of the binned particles (weights ws per binned slot, or the
+// per-type masses) into T float grids sum * out_scale, optionally also the
+// int64 sums; one workgroup or one wavefront per mesh row (pm->paint_rows).
+static int launch_row_paint(hymd_pm *pm, int T, const float *ws, const PaintArgs &pa, double out_scale, float *out,
+                            long long *out_fixed) {
+    const Mesh &m = pm->mesh;
+    const int64_t nrows = (int64_t)m.mx * m.my;
+    if (pm->paint_rows == kPaintWaveRows) {
+        const int tc = std::max(1, std::min(T, (int)(kRowLdsBytes * 2 / (sizeof(long long) * kPaintWaveRows * m.mz))));
+        dim3 grid((unsigned)((nrows + kPaintWaveRows - 1) / kPaintWaveRows), (unsigned)((T + tc - 1) / tc));
+        paint_rows_wave_kernel<<<grid, kBlock, sizeof(long long) * kPaintWaveRows * tc * m.mz, pm->stream>>>(
+            (const float4 *)pm->xs.ptr, ws, (const int *)pm->start.ptr, m, pa, tc, out_scale, out, out_fixed);
+    } else {
+        const int tc = std::max(1, std::min(T, (int)(kRowLdsBytes / (sizeof(long long) * m.mz))));
+        dim3 grid((unsigned)nrows, (unsigned)((T + tc - 1) / tc));
+        paint_rows_kernel<<<grid, kBlock, sizeof(long long) * tc * m.mz, pm->stream>>>(
+            (const float4 *)pm->xs.ptr, ws, (const int *)pm->start.ptr, m, pa, tc, out_scale, out, out_fixed);
+    }
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
+}
+
 // Row-gather paint of the sorted particles into float grids rho_t = paint/dV
 // (T grids), optionally also the int64 fixed-point sums.
 static int paint_sorted(hymd_pm *pm, int T, const double *mass, float *out, long long *out_fixed) {
@@ -1156,15 +1296,8 @@ static int paint_sorted(hymd_pm *pm, int T, const double *mass, float *out, long
     pa.scale = std::ldexp(1.0, HYMD_FIXED_SHIFT);
     for (int t = 0; t < HYMD_MAX_TYPES; ++t) pa.mass[t] = (mass && t < T) ? mass[t] : 1.0;
     const double out_scale = std::ldexp(1.0, -HYMD_FIXED_SHIFT) / cell_volume(pm);
-    const Mesh &m = pm->mesh;
-    const int tc = std::max(1, std::min(T, (int)(kRowLdsBytes / (sizeof(long long) * m.mz))));
-    dim3 grid((unsigned)(m.mx * m.my), (unsigned)((T + tc - 1) / tc));
-    paint_rows_kernel<<<grid, kBlock, sizeof(long long) * tc * m.mz, pm->stream>>>(
-        (const float4 *)pm->xs.ptr, pm->sorted_weights ? (const float *)pm->ws.ptr : nullptr,
-        (const int *)pm->start.ptr, m, pa, tc, out_scale,
-        out, out_fixed);
-    HYMD_HIP(hipGetLastError());
-    return HYMD_OK;
+    return launch_row_paint(pm, T, pm->sorted_weights ? (const float *)pm->ws.ptr : nullptr, pa, out_scale, out,
+                            out_fixed);
 }
 
 // Readout of ncomp grids per type at the last sorted particle set, written to
@@ -1182,11 +1315,12 @@ static int readout_sorted(hymd_pm *pm, int ncomp, int n_types, const float *grid
             HYMD_TRY(zero_words(out, ncomp * n, pm->stream));
             kept = (const int *)pm->start.ptr + (int64_t)pm->mesh.mx * pm->mesh.my;
         }
+        const int bs = pm->readout_block;
         if (ncomp == 3)
-            readout_kernel<3><<<grid_for(n), kBlock, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh,
+            readout_kernel<3><<<grid_for(n, bs), bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh,
                                                                       out, pp, kept);
         else
-            readout_kernel<1><<<grid_for(n), kBlock, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh,
+            readout_kernel<1><<<grid_for(n, bs), bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh,
                                                                       out, pp, kept);
     }
     HYMD_HIP(hipGetLastError());
@@ -1283,7 +1417,7 @@ static int update_field_impl(hymd_pm *pm, const float *x, const int32_t *types, 
     FieldK p = field_params(T, coupling, shift, constant, sigma, m);
     p.want_energy = energy ? 1 : 0;
     KGeom g0 = kgeom(pm);
-    const int W2 = T <= 8 ? xline_grad_width(pm->fft, T) : 0;
+    const int W2 = T <= 8 ? xline_grad_width(pm->fft, T, pm->xline_w) : 0;
     if (!phi && !phi_fourier && !v_ext && W2 > 0) {
         // force grids only: z + y forward passes; one pass for the forward x
         // transform, the k-space pass and the 2T inverse x transforms; the
@@ -1550,13 +1684,8 @@ extern "C" int hymd_field_adjoint(hymd_pm *pm, const float *x, const int32_t *ty
     for (int t = 0; t < HYMD_MAX_TYPES; ++t) pa.mass[t] = 1.0;
     HYMD_TRY(pm->rscratch.reserve(sizeof(float) * 3 * (size_t)T * m.g));
     float *pg = (float *)pm->rscratch.ptr;
-    const int tc = std::max(1, std::min(T, (int)(kRowLdsBytes / (sizeof(long long) * m.mz))));
-    dim3 grid((unsigned)(m.mx * m.my), (unsigned)((T + tc - 1) / tc));
     for (int d = 0; d < 3; ++d)
-        paint_rows_kernel<<<grid, kBlock, sizeof(long long) * tc * m.mz, s>>>(
-            (const float4 *)pm->xs.ptr, ws + d * n, (const int *)pm->start.ptr, m, pa, tc, 1.0 / pa.scale,
-            pg + (int64_t)d * T * m.g, nullptr);
-    HYMD_HIP(hipGetLastError());
+        HYMD_TRY(launch_row_paint(pm, T, ws + d * n, pa, 1.0 / pa.scale, pg + (int64_t)d * T * m.g, nullptr));
     // 3. their spectra, 4. the adjoint moments
     float2 *pspec = (float2 *)pm->grad.ptr;
     HYMD_TRY(fft_r2c_real(pm->fft, pg, pspec, 3 * T, s));

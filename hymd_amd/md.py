@@ -50,6 +50,8 @@ class RespaMD:
     many outer steps, as they diffuse; ``x``, ``v``, ``types``, the forces and
     the order maps are permuted in place, the physics is unchanged (the field
     path's sums are order-independent and those rows carry no bonded terms).
+    ``autotune`` chooses the mesh kernels' launch shapes for this workload
+    (``hymd_amd.tune``; ``tuning`` holds the choice and its timings).
     ``term_precision`` sets the arithmetic of the bond and angle terms in the
     library's fused inner loop: "fp32" (default: FP32 term arithmetic, the
     FP32 forces this path allows; the reference's real(4) kernels difference
@@ -59,7 +61,8 @@ class RespaMD:
     ``step()`` path evaluates the terms in FP64.
     """
 
-    def __init__(self, system, stream=None, spatial_order=True, reorder_every=25, term_precision="fp32"):
+    def __init__(self, system, stream=None, spatial_order=True, reorder_every=25, term_precision="fp32",
+                 autotune=False):
         _lib.require_gpu()
         self.system = system
         if spatial_order:
@@ -106,6 +109,13 @@ class RespaMD:
         self._native = None
         self._thermostat = None
         self._com_ws = None
+        # launch shapes of the mesh kernels for this workload (hymd_amd.tune),
+        # from the tuning cache when this shape was tuned before
+        self.tuning = None
+        if autotune:
+            from . import tune
+            self.tuning = tune.autotune(self.pm, self.x32, self.types, self.T, self.A, self.b, self.c0, self.sigma,
+                                        mass=self.type_mass)
 
     @property
     def order(self):

@@ -45,6 +45,9 @@ mass motion removed.  numpy PCG64 streams, seed = 1000 + workload index.
                      edge the one of {48, 64, 96} nearest L_r / 0.6 nm (only
                      48 and 64 occur), chi_AB ~ U[5, 25], sigma ~ U[0.4, 0.7];
                      kappa = 0.05 (left open by the workload, fixed here).
+                     Each replica's chains are stored in the mesh order of
+                     their first bead (x-major cells, as an MD code keeps its
+                     particles; ``spatial=False`` keeps generation order).
 All accept smaller chain / solvent counts for fast parity cases.
 """
 from dataclasses import dataclass, field
@@ -278,8 +281,17 @@ class ReplicaSet:
         return self.positions[o[r]:o[r + 1]], self.types[o[r]:o[r + 1]]
 
 
+def mesh_ordered_chains(x, length, box, mesh):
+    """Chains of ``length`` beads (consecutive rows of x) reordered by the
+    x-major mesh cell of their first bead (stable)."""
+    c = x.reshape(-1, length, 3)
+    cell = np.floor(c[:, 0] / np.asarray(box) * mesh).astype(np.int64) % mesh
+    key = (cell[:, 0] * mesh + cell[:, 1]) * mesh + cell[:, 2]
+    return c[np.argsort(key, kind="stable")].reshape(-1, 3)
+
+
 def replica_set(n_replicas=64, n_min=50_000, n_max=400_000, meshes=(48, 64, 96), spacing=0.6, density=8.4,
-                seed=1003):
+                seed=1003, spatial=True):
     rng = np.random.default_rng(seed)
     counts = (rng.integers(n_min, n_max + 1, n_replicas) // 10) * 10
     L = (counts / density) ** (1.0 / 3.0)
@@ -290,7 +302,8 @@ def replica_set(n_replicas=64, n_min=50_000, n_max=400_000, meshes=(48, 64, 96),
     xs, ts = [], []
     for r in range(n_replicas):
         box = np.full(3, L[r])
-        xs.append(random_walk_chains(rng, int(counts[r]) // 10, 10, box))
+        xr = random_walk_chains(rng, int(counts[r]) // 10, 10, box)
+        xs.append(mesh_ordered_chains(xr, 10, box, int(mesh[r])) if spatial else xr)
         ts.append(np.tile(np.array([0] * 5 + [1] * 5, dtype=np.int32), int(counts[r]) // 10))
     chi = np.zeros((n_replicas, 2, 2))
     chi[:, 0, 1] = chi[:, 1, 0] = chi_ab

@@ -66,6 +66,22 @@ int hymd_pm_destroy(hymd_pm *pm);
 int hymd_pm_set_stream(hymd_pm *pm, void *stream);
 /* new box at fixed mesh (barostats re-create pmesh objects, barostat.py:163-164) */
 int hymd_pm_set_box(hymd_pm *pm, const double box[3]);
+/* Launch shapes of the handle's kernels (no reference counterpart: the
+ * autotuner's knobs; paint and readout shapes leave every result unchanged
+ * bit for bit, another x-line width may round the FP32 x transform
+ * differently):
+ *   HYMD_TUNE_PAINT_ROWS     mesh rows per paint workgroup, 1 (default) or 4
+ *                            (one per wavefront)
+ *   HYMD_TUNE_READOUT_BLOCK  threads per readout workgroup, 64, 128 or 256 (default)
+ *   HYMD_TUNE_XLINE_WIDTH    lines per x-line FFT + k-space workgroup, 0 (default:
+ *                            the widest that fits) or 2, 4, 8, 16 (used when it
+ *                            fits the update's type count, else the default)
+ * A change makes captured MD steps capture again. */
+#define HYMD_TUNE_PAINT_ROWS 1
+#define HYMD_TUNE_READOUT_BLOCK 2
+#define HYMD_TUNE_XLINE_WIDTH 3
+int hymd_pm_set_tuning(hymd_pm *pm, int32_t knob, int32_t value);
+int hymd_pm_get_tuning(hymd_pm *pm, int32_t knob, int32_t *value);
 
 /* update_field (field.py:428-616) for a quadratic Hamiltonian
  *   w(phi) = 1/2 phi^T A phi + b.phi + c0   (hamiltonian.py:145,259,393-412)
@@ -182,7 +198,8 @@ int hymd_dihedrals_redistribute(hymd_dihedrals *h, const double *f_dipoles, cons
 /* Batched replicas (no reference counterpart: the reference runs one system
  * per process).  n_replicas independent systems, replica r owning particles
  * [offsets[r], offsets[r+1]) of the caller's arrays, mesh[3r..3r+2],
- * box[3r..3r+2] and one mass per type (host arrays, fixed for the handle).
+ * box[3r..3r+2] and one mass per type (host arrays, fixed for the handle);
+ * n_types x the largest mesh edge along z at most 2048.
  * hymd_batch_moments paints, transforms and reduces every replica to the
  * moments of hymd_field_moments (n_types <= 8; row r of `moments` holds
  * replica r's 2 T(T+1)/2 + T doubles, bit-identical to hymd_field_moments on
@@ -196,6 +213,15 @@ int hymd_batch_create(hymd_batch **out, int32_t n_replicas, const int32_t *mesh,
 int hymd_batch_destroy(hymd_batch *b);
 int hymd_batch_moments(hymd_batch *b, const float *x, const int32_t *types, const double *sigma, double *moments,
                        int32_t use_graph, void *stream);
+/* hymd_batch_moments plus every replica's field forces in the same launch
+ * sequence: coupling host double (n_replicas, n_types, n_types), replica r's
+ * quadratic form A_r; forces device float32 (n,3) out, the forces of
+ * compute_field_force (field.py:152-200) of update_field with A_r and
+ * sigma[r] (field.py:580-613), bit-identical to hymd_field_moments with
+ * force grids + hymd_compute_field_force on that replica alone.  With
+ * use_graph the sequence is captured again when x, types or forces change. */
+int hymd_batch_forces(hymd_batch *b, const float *x, const int32_t *types, const double *sigma,
+                      const double *coupling, double *moments, float *forces, int32_t use_graph, void *stream);
 
 /* compute_self_energy_q (field.py:203-238): host double out */
 int hymd_self_energy_q(const float *charges, int64_t n, double sigma, double elec_factor, double *out,

@@ -192,7 +192,7 @@ def test_energy_forces_function_vs_fd(hip):
         return F.field_energy(out["phi"], ham, s.mesh, s.box) + np.sum(w * f)
 
     L0 = loss(s.chi, s.kappa, s.sigma)
-    assert float(L) == pytest.approx(L0, rel=1e-5)
+    assert float(L.detach()) == pytest.approx(L0, rel=1e-5)
     h = 1e-3
     t, u = 1, 3
     c = s.chi[t, u]

@@ -286,3 +286,56 @@ def test_readout_matches_oracle(hip, mesh, T):
     got = f.cpu().numpy()
     np.testing.assert_array_equal(got[~ok], 0.0)
     np.testing.assert_allclose(got, ref, rtol=0, atol=2e-6 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("mesh,T", [((32, 32, 32), 2), ((24, 20, 30), 5), ((128, 128, 128), 4)])
+def test_tuning_knobs_keep_the_bits(hip, mesh, T, tmp_path):
+    """The paint and readout launch shapes the autotuner may choose give the
+    same density, force grids, forces and energy bit for bit (integer paint
+    sums, unchanged per-element arithmetic); another x-line width may change
+    the FP32 rounding of the x transform (1e-6).  The autotuner runs, caches
+    its choice and restores it from the cache."""
+    from hymd_amd import tune
+    rng = np.random.default_rng(sum(mesh) + T)
+    box = np.array(mesh, dtype=np.float64) * 0.31
+    n = 30_000 if mesh[0] < 100 else 400_000
+    x = (rng.uniform(0, 1, (n, 3)) * box).astype(np.float32)
+    types = rng.integers(0, T, n).astype(np.int32)
+    A = rng.uniform(0.0, 1.0, (T, T))
+    A = A + A.T
+    b = rng.uniform(-1, 1, T)
+    pm = _pm(mesh, box, T)
+    xd, td = _dev(x, torch.float32), _dev(types, torch.int32)
+
+    def run():
+        fm = pm.create("real", batch=(T, 3))
+        e = pm.field_update(xd, td, T, A, b, 0.0, [1.0] * T, 0.5, fm, want_energy=True)
+        f = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+        pm.readout_forces(xd, td, T, fm, f)
+        grid = pm.paint_fixed(xd, td, T)
+        return fm.cpu(), f.cpu(), grid.cpu(), e
+
+    ref = run()
+    for name, (_, values) in tune.KNOBS.items():
+        for v in values:
+            tune.apply(pm, dict(tune.DEFAULTS, **{name: v}))
+            assert tune.get(pm)[name] == v
+            got = run()
+            assert torch.equal(got[2], ref[2])   # the integer density grid
+            if name == "xline_width":
+                # another x-line geometry may round the x transform differently
+                for a, c in zip(got[:2], ref[:2]):
+                    assert rms_rel(a.numpy(), c.numpy()) < 1e-6, (name, v)
+                assert got[3] == pytest.approx(ref[3], rel=1e-6)
+            else:
+                for a, c in zip(got[:2], ref[:2]):
+                    assert torch.equal(a, c), (name, v)
+                assert got[3] == ref[3]
+    path = str(tmp_path / "tune.json")
+    res = tune.autotune(pm, xd, td, T, A, b, 0.0, 0.5, reps=3, path=path)
+    assert not res["cached"] and res["tuned_ms"] <= res["default_ms"]
+    tune.apply(pm, tune.DEFAULTS)
+    again = tune.autotune(pm, xd, td, T, A, b, 0.0, 0.5, reps=3, path=path)
+    assert again["cached"] and tune.get(pm) == res["choice"]
+    got = run()
+    assert rms_rel(got[1].numpy(), ref[1].numpy()) < 1e-6
