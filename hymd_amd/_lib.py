@@ -62,6 +62,7 @@ SIGNATURES = {
     "hymd_fft_c2r": [_p, _p, _p, _i32],
     "hymd_readout": [_p, _p, _p, _i64, _p],
     "hymd_cbf": [_p, _p, _i64, _pd, _p, _p, _p, _p, _i64, _i32, _pd, _pd, _p],
+    "hymd_cbf_noalloc": [_p, _p, _i64, _pd, _p, _p, _p, _p, _i64, _i32, _pd, _pd, _p],
     "hymd_caf": [_p, _p, _i64, _pd, _p, _p, _p, _p, _p, _i64, _i32, _pd, _pd, _p],
     "hymd_cdf": [_p, _p, _p, _p, _i64, _pd, _p, _p, _p, _p, _p, _p, _p, _i64, _i32, _i32, _pd, _pi64, _p],
     "hymd_dipole_redistribution": [_p, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _i64, _i32, _p],

@@ -72,6 +72,28 @@ struct Csr {
     const uint32_t *ent() const { return (const uint32_t *)entries.ptr; }
 };
 
+// Builds the lists into caller-provided storage: counts (n ints, left holding
+// the offsets), off (n + 1 ints), entries (nroles * m words), tiles
+// (scan_tiles(n) + 2 ints, the last one the index error flag).  Clears with a
+// fill kernel, checks the indices with one synchronising read-back.
+static int build_csr_into(int *counts, int *off, uint32_t *entries, int *tiles, const TermIdx &ti, int64_t m,
+                          int64_t n, hipStream_t s) {
+    int *err = tiles + scan_tiles(n) + 1;
+    HYMD_TRY(zero_words(counts, std::max<int64_t>(n, 1), s));
+    HYMD_TRY(zero_words(err, 1, s));
+    if (m > 0) csr_count_kernel<<<grid_for(m), kBlock, 0, s>>>(ti, m, n, counts, err);
+    HYMD_HIP(hipGetLastError());
+    HYMD_TRY(exclusive_scan(counts, off, n, tiles, n > 0 ? counts : nullptr, false, s));
+    if (m > 0) csr_fill_kernel<<<grid_for(m), kBlock, 0, s>>>(ti, m, n, counts, entries);
+    if (n > 0) csr_sort_kernel<<<grid_for(n), kBlock, 0, s>>>(off, n, entries);
+    HYMD_HIP(hipGetLastError());
+    int herr = 0;
+    HYMD_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+    HYMD_HIP(hipStreamSynchronize(s));
+    HYMD_REQUIRE(herr == 0, "a bonded-term particle index is outside [0, %lld)", (long long)n);
+    return HYMD_OK;
+}
+
 static int build_csr(Csr &c, const TermIdx &ti, int64_t m, int64_t n, hipStream_t s) {
     HYMD_REQUIRE(m < (int64_t(1) << 29), "too many terms (%lld)", (long long)m);
     HYMD_REQUIRE(m * ti.nroles < (int64_t(1) << 31), "too many term members");
@@ -81,22 +103,8 @@ static int build_csr(Csr &c, const TermIdx &ti, int64_t m, int64_t n, hipStream_
     HYMD_TRY(c.offsets.reserve(sizeof(int) * (n + 1)));
     HYMD_TRY(c.entries.reserve(sizeof(uint32_t) * std::max<int64_t>(m * ti.nroles, 1)));
     HYMD_TRY(c.tiles.reserve(sizeof(int) * (scan_tiles(n) + 2)));
-    int *counts = (int *)c.counts.ptr, *off = (int *)c.offsets.ptr, *tiles = (int *)c.tiles.ptr;
-    int *err = tiles + scan_tiles(n) + 1;
-    HYMD_HIP(hipMemsetAsync(counts, 0, sizeof(int) * std::max<int64_t>(n, 1), s));
-    HYMD_HIP(hipMemsetAsync(err, 0, sizeof(int), s));
-    if (m > 0) csr_count_kernel<<<grid_for(m), kBlock, 0, s>>>(ti, m, n, counts, err);
-    HYMD_HIP(hipGetLastError());
-    HYMD_TRY(exclusive_scan(counts, off, n, tiles, nullptr, false, s));
-    if (n > 0) HYMD_HIP(hipMemcpyAsync(counts, off, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
-    if (m > 0) csr_fill_kernel<<<grid_for(m), kBlock, 0, s>>>(ti, m, n, counts, (uint32_t *)c.entries.ptr);
-    if (n > 0) csr_sort_kernel<<<grid_for(n), kBlock, 0, s>>>(off, n, (uint32_t *)c.entries.ptr);
-    HYMD_HIP(hipGetLastError());
-    int herr = 0;
-    HYMD_HIP(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
-    HYMD_HIP(hipStreamSynchronize(s));
-    HYMD_REQUIRE(herr == 0, "a bonded-term particle index is outside [0, %lld)", (long long)n);
-    return HYMD_OK;
+    return build_csr_into((int *)c.counts.ptr, (int *)c.offsets.ptr, (uint32_t *)c.entries.ptr, (int *)c.tiles.ptr, ti,
+                          m, n, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -450,6 +458,90 @@ extern "C" int hymd_cbf(void *f, const void *r, int64_t n, const double box[3], 
     if (red) {
         double out[4];
         HYMD_TRY(finish_reduce(w, nb, 4, out, s));
+        if (energy) *energy = out[0];
+        if (bond_pr)
+            for (int d = 0; d < 3; ++d) bond_pr[d] = out[1 + d];
+    }
+    return HYMD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free bonds: hymd_cbf's kernels with every scratch buffer (the
+// per-particle term lists, the energy / virial partials and their sums) in
+// one file-scope device array.  Each device holds its own copy of the array
+// for the life of the process; its address is looked up once per device and
+// kept.  All calls on one device share it, so calls that overlap on a device
+// race.
+// ---------------------------------------------------------------------------
+constexpr int64_t kScratchMaxParticles = HYMD_NOALLOC_MAX_PARTICLES;
+constexpr int64_t kScratchMaxBonds = HYMD_NOALLOC_MAX_BONDS;
+constexpr int kScratchMaxBlocks = 2048;  // the grid cap of the gather, so the partial rows of any n fit
+constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t kScrCounts = 0;
+constexpr size_t kScrOffsets = kScrCounts + align256(sizeof(int) * kScratchMaxParticles);
+constexpr size_t kScrEntries = kScrOffsets + align256(sizeof(int) * (kScratchMaxParticles + 1));
+constexpr size_t kScrTiles = kScrEntries + align256(sizeof(uint32_t) * 2 * kScratchMaxBonds);
+constexpr size_t kScrPartials =
+    kScrTiles + align256(sizeof(int) * ((kScratchMaxParticles + 1023) / 1024 + 2));
+constexpr size_t kScrResult = kScrPartials + align256(sizeof(double) * 4 * kScratchMaxBlocks);
+constexpr size_t kScratchBytes = kScrResult + align256(sizeof(double) * 4);
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static int scratch_base(unsigned char **out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static unsigned char *ptr[kMaxDevices] = {};
+    int dev = -1;
+    HYMD_HIP(hipGetDevice(&dev));
+    HYMD_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[dev]) {
+        void *p = nullptr;
+        HYMD_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        HYMD_REQUIRE(p != nullptr, "no device address for the bond scratch array");
+        ptr[dev] = (unsigned char *)p;
+    }
+    *out = ptr[dev];
+    return HYMD_OK;
+}
+
+extern "C" int hymd_cbf_noalloc(void *f, const void *r, int64_t n, const double box[3], const int32_t *a,
+                                const int32_t *b, const double *r0, const double *k, int64_t m, int32_t precision,
+                                double *energy, double *bond_pr, void *stream) {
+    HYMD_REQUIRE(f && box && n >= 0 && m >= 0 && (r || n == 0), "bad argument");
+    HYMD_REQUIRE(m == 0 || (a && b && r0 && k), "null term array");
+    HYMD_REQUIRE(precision == HYMD_F32 || precision == HYMD_F64, "precision must be HYMD_F32 or HYMD_F64");
+    HYMD_REQUIRE(n <= kScratchMaxParticles, "%lld particles exceed the allocation-free limit of %lld",
+                 (long long)n, (long long)kScratchMaxParticles);
+    HYMD_REQUIRE(m <= kScratchMaxBonds, "%lld bonds exceed the allocation-free limit of %lld", (long long)m,
+                 (long long)kScratchMaxBonds);
+    hipStream_t s = (hipStream_t)stream;
+    unsigned char *base = nullptr;
+    HYMD_TRY(scratch_base(&base));
+    HYMD_REQUIRE(scan_tiles(n) + 2 <= (int64_t)((kScrPartials - kScrTiles) / sizeof(int)), "scan tile scratch");
+    int *counts = (int *)(base + kScrCounts), *off = (int *)(base + kScrOffsets), *tiles = (int *)(base + kScrTiles);
+    uint32_t *entries = (uint32_t *)(base + kScrEntries);
+    double *partials = (double *)(base + kScrPartials), *result = (double *)(base + kScrResult);
+    TermIdx ti{{a, b, nullptr, nullptr}, 2};
+    HYMD_TRY(build_csr_into(counts, off, entries, tiles, ti, m, n, s));
+    const int nb = grid_for(n, kBlock, kScratchMaxBlocks);
+    const bool red = energy || bond_pr;
+    double *part = red ? partials : nullptr;
+    BondArgs B{a, b, r0, k};
+    Box bx = make_box(box);
+    if (precision == HYMD_F32)
+        bond_gather_kernel<float, float><<<nb, kBlock, 0, s>>>((const float *)r, (float *)f, n, off, entries, B, bx,
+                                                                part);
+    else
+        bond_gather_kernel<double, double><<<nb, kBlock, 0, s>>>((const double *)r, (double *)f, n, off, entries, B,
+                                                                  bx, part);
+    HYMD_HIP(hipGetLastError());
+    if (red) {
+        double out[4];
+        HYMD_TRY(reduce_partials(partials, nb, 4, result, s));
+        HYMD_HIP(hipMemcpyAsync(out, result, sizeof(out), hipMemcpyDeviceToHost, s));
+        HYMD_HIP(hipStreamSynchronize(s));
         if (energy) *energy = out[0];
         if (bond_pr)
             for (int d = 0; d < 3; ++d) bond_pr[d] = out[1 + d];

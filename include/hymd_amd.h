@@ -253,6 +253,19 @@ int hymd_readout(hymd_pm *pm, const float *grid, const float *x, int64_t n, floa
 int hymd_cbf(void *f, const void *r, int64_t n, const double box[3], const int32_t *a, const int32_t *b,
              const double *r0, const double *k, int64_t m, int32_t precision, double *energy, double *bond_pr,
              void *stream);
+/* Allocation-free variant of hymd_cbf, with no counterpart in the reference:
+ * for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, kernels and results as
+ * hymd_cbf; the term lists, partial sums and reduced sums live in one static
+ * device array of about 16.1 MB per device, held until the process ends.
+ * Rejects n > HYMD_NOALLOC_MAX_PARTICLES or m > HYMD_NOALLOC_MAX_BONDS with
+ * HYMD_ERR_INVALID.  Every call on a device shares that array, so calls that
+ * overlap on one device (other threads, other streams) race. */
+#define HYMD_NOALLOC_MAX_PARTICLES (1 << 20)
+#define HYMD_NOALLOC_MAX_BONDS (1 << 20)
+int hymd_cbf_noalloc(void *f, const void *r, int64_t n, const double box[3], const int32_t *a, const int32_t *b,
+                     const double *r0, const double *k, int64_t m, int32_t precision, double *energy,
+                     double *bond_pr, void *stream);
 int hymd_caf(void *f, const void *r, int64_t n, const double box[3], const int32_t *a, const int32_t *b,
              const int32_t *c, const double *t0, const double *k, int64_t m, int32_t precision, double *energy,
              double *angle_pr, void *stream);
