@@ -65,6 +65,11 @@ int fft_c2r(const FftPlan &plan, float2 *data, float *out, int batch, float scal
 // (k_y = kfy n_y, k_z = kfz n_z, zero on Nyquist planes); needs plan.lds_plane.
 int fft_c2r_yz_grad(const FftPlan &plan, const float2 *in, float *out, int T, double kfy, double kfz, float scale,
                     hipStream_t s);
+// The same 3T gradient grids with the separate y and z line passes (meshes
+// whose y-z planes do not fit in LDS): the y pass reads in[] and writes the
+// 3T y-transformed spectra to tmp (3T half spectra), the z pass writes out.
+int fft_c2r_lines_grad(const FftPlan &plan, const float2 *in, float2 *tmp, float *out, int T, double kfy, double kfz,
+                       float scale, hipStream_t s);
 // Whether fft_inplace<elems> can transform nlines lines of this axis with `threads` threads.
 bool fft_inplace_fits(const FftAxis &ax, int nlines, int threads, int elems);
 

@@ -154,10 +154,14 @@ __global__ void __launch_bounds__(kBlock) r2c_z_kernel(const float *__restrict__
 
 // z pass, half spectrum -> real:  Z'[k] = (X[k] + X*[Nh-k]) + i e^{2 pi i k/Mz} (X[k] - X*[Nh-k]),
 // inverse FFT of length Nh gives x[2n] + i x[2n+1] (unnormalised).
+// With kfz != 0 the grids by = 3t + 2 are multiplied by -i k_z (k_z = kfz kz,
+// zero on the Nyquist plane) as they are loaded: the z derivative of the
+// gradient passes.
 __global__ void __launch_bounds__(kBlock) c2r_z_kernel(const float2 *__restrict__ in, float *__restrict__ out,
                                                         int64_t rows, int mz, int zh, FftAxis ax,
                                                         const float2 *__restrict__ tw_glob,
-                                                        const float2 *__restrict__ post_glob, float scale, int W) {
+                                                        const float2 *__restrict__ post_glob, float scale, int W,
+                                                        float kfz) {
     extern __shared__ float2 smem[];
     const int nh = ax.n;
     float2 *tw = smem;
@@ -173,6 +177,7 @@ __global__ void __launch_bounds__(kBlock) c2r_z_kernel(const float2 *__restrict_
     const int wv = (int)(rows - row0 < W ? rows - row0 : W);
     const float2 *src = in + (bofs + row0) * zh;   // W contiguous half-spectrum rows (ld = zh)
     const int nvalid = wv * zh, total = W * zh;
+    const bool dz = kfz != 0.f && by % 3 == 2;
     for (int i0 = threadIdx.x; i0 < total; i0 += kBlock * kLoadBatch) {
         float2 v[kLoadBatch];
 #pragma unroll
@@ -183,7 +188,14 @@ __global__ void __launch_bounds__(kBlock) c2r_z_kernel(const float2 *__restrict_
 #pragma unroll
         for (int j = 0; j < kLoadBatch; ++j) {
             const int idx = i0 + j * kBlock;
-            if (idx < total) A[idx] = v[j];
+            if (idx < total) {
+                if (dz) {   // -i k_z v = (k_z v.y, -k_z v.x)
+                    const int kz = idx % zh;
+                    const float k = (2 * kz == mz) ? 0.f : kfz * (float)kz;
+                    v[j] = make_float2(k * v[j].y, -k * v[j].x);
+                }
+                A[idx] = v[j];
+            }
         }
     }
     __syncthreads();
@@ -248,6 +260,62 @@ __global__ void __launch_bounds__(kBlock) fft_strided_kernel(float2 *__restrict_
     float2 *r = fft_lds(A, B, W, ax, tw, sgn);
     if (mybase >= 0)
         for (int idx = threadIdx.x; idx < nel; idx += blockDim.x) d[mybase + (int64_t)(idx >> wshift) * es] = r[myl * ax.ld + (idx >> wshift)];
+}
+
+// Out-of-place inverse y pass of the gradient spectra: grid by = 3t + d of
+// out[] from in[t] (d = 0: in[t] = IFFT_x(-i k_x V_t)) or in[T + t]
+// (IFFT_x(V_t)), times -i k_y on load for d = 1 (k_y = kfy ny, zero on the
+// Nyquist plane); d = 2 gets its -i k_z in the z pass.  Lines (x, kz) along y
+// as fft_strided_kernel's y pass.
+__global__ void __launch_bounds__(kBlock) fft_y_grad_kernel(const float2 *__restrict__ in, float2 *__restrict__ out,
+                                                             int64_t gc, int nlines, int inner, int64_t outer,
+                                                             int64_t es, int T, FftAxis ax,
+                                                             const float2 *__restrict__ tw_glob, float kfy,
+                                                             int wshift) {
+    extern __shared__ float2 smem[];
+    const int n = ax.n;
+    const int W = 1 << wshift;
+    float2 *tw = smem;
+    float2 *A = tw + n;
+    float2 *B = A + W * ax.ld;
+    const unsigned flat = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const unsigned bx = flat % gridDim.x, by = flat / gridDim.x;
+    const int t = (int)by / 3, d = (int)by - 3 * t;
+    const float2 *src = in + (int64_t)(d == 0 ? t : T + t) * gc;
+    float2 *dst = out + (int64_t)by * gc;
+    const int line0 = (int)bx * W;
+    const int myl = (int)threadIdx.x & (W - 1);
+    const int myline = line0 + myl;
+    const int64_t mybase = myline < nlines ? (int64_t)(myline / inner) * outer + (myline % inner) : -1;
+    const int nel = W * n;
+    for (int i0 = threadIdx.x; i0 < nel; i0 += kBlock * kLoadBatch) {
+        float2 v[kLoadBatch];
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kBlock;
+            v[j] = make_float2(0.f, 0.f);
+            if (idx < nel && mybase >= 0) v[j] = src[mybase + (int64_t)(idx >> wshift) * es];
+        }
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kBlock;
+            if (idx < nel) {
+                const int e = idx >> wshift;
+                if (d == 1) {   // -i k_y v = (k_y v.y, -k_y v.x)
+                    const int ny = (2 * e < n) ? e : e - n;
+                    const float k = (2 * e == n) ? 0.f : kfy * (float)ny;
+                    v[j] = make_float2(k * v[j].y, -k * v[j].x);
+                }
+                A[myl * ax.ld + e] = v[j];
+            }
+        }
+    }
+    for (int i = threadIdx.x; i < n - 1; i += blockDim.x) tw[i] = tw_glob[i];
+    __syncthreads();
+    float2 *r = fft_lds(A, B, W, ax, tw, 1.f);
+    if (mybase >= 0)
+        for (int idx = threadIdx.x; idx < nel; idx += blockDim.x)
+            dst[mybase + (int64_t)(idx >> wshift) * es] = r[myl * ax.ld + (idx >> wshift)];
 }
 
 // ---------------------------------------------------------------------------
@@ -578,7 +646,7 @@ int fft_c2r_yz(const FftPlan &plan, float2 *in, float *out, int batch, float sca
     dim3 grid((unsigned)((rows + plan.wz - 1) / plan.wz), (unsigned)batch);
     c2r_z_kernel<<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, m.mz, m.zh, plan.azh,
                                                    (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr,
-                                                   scale, plan.wz);
+                                                   scale, plan.wz, 0.f);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }
@@ -600,6 +668,24 @@ int fft_c2r_yz_grad(const FftPlan &plan, const float2 *in, float *out, int T, do
         HYMD_HIP(hipGetLastError());
         return HYMD_OK;
     });
+}
+
+int fft_c2r_lines_grad(const FftPlan &plan, const float2 *in, float2 *tmp, float *out, int T, double kfy, double kfz,
+                       float scale, hipStream_t s) {
+    const Mesh &m = plan.mesh;
+    HYMD_REQUIRE(kfy != 0.0 && kfz != 0.0, "the gradient passes need non-zero k factors");
+    const int nlines = m.mx * m.zh;
+    dim3 gy((unsigned)((nlines + plan.wy - 1) / plan.wy), (unsigned)(3 * T));
+    fft_y_grad_kernel<<<gy, kBlock, plan.lds_y, s>>>(in, tmp, m.gc, nlines, m.zh, (int64_t)m.my * m.zh, m.zh, T,
+                                                     plan.ay, (const float2 *)plan.tw_y.ptr, (float)kfy,
+                                                     log2_exact(plan.wy));
+    HYMD_HIP(hipGetLastError());
+    const int64_t rows = (int64_t)m.mx * m.my;
+    dim3 gz((unsigned)((rows + plan.wz - 1) / plan.wz), (unsigned)(3 * T));
+    c2r_z_kernel<<<gz, kBlock, plan.lds_z, s>>>(tmp, out, rows, m.mz, m.zh, plan.azh, (const float2 *)plan.tw_zh.ptr,
+                                                (const float2 *)plan.post.ptr, scale, plan.wz, (float)kfz);
+    HYMD_HIP(hipGetLastError());
+    return HYMD_OK;
 }
 
 bool fft_inplace_fits(const FftAxis &ax, int nlines, int threads, int elems) {

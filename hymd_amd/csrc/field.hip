@@ -1008,8 +1008,7 @@ constexpr int kXlineGradThreads = 256;
 constexpr int kXlineGradElems = 16;  // values per thread and in-place stage
 
 // Lines per workgroup of field_xline_grad_kernel for T types (0: it does not
-// fit, or the y-z plane pass it feeds is unavailable): `want` when it fits,
-// else the widest that does.
+// fit): `want` when it fits, else the widest that does.
 static bool xline_grad_fits(const FftPlan &f, int T, int W) {
     const int pmax = 16 / moments_maxt(T);
     if ((int64_t)W * f.ax.n > (int64_t)kXlineGradThreads * pmax) return false;
@@ -1018,7 +1017,6 @@ static bool xline_grad_fits(const FftPlan &f, int T, int W) {
 }
 
 static int xline_grad_width(const FftPlan &f, int T, int want = 0) {
-    if (!f.lds_plane) return 0;
     if (want > 0 && xline_grad_fits(f, T, want)) return want;
     for (int W = 16; W >= 2; W >>= 1)
         if (xline_grad_fits(f, T, W)) return W;
@@ -1426,7 +1424,13 @@ static int update_field_impl(hymd_pm *pm, const float *x, const int32_t *types, 
         double *part = energy ? (double *)pm->partials.ptr : nullptr;
         int nb = 0;
         HYMD_TRY(launch_xline_grad(pm, spec, grad, g0, p, W2, part, &nb));
-        HYMD_TRY(fft_c2r_yz_grad(pm->fft, grad, force_mesh, T, g0.kfac[1], g0.kfac[2], 1.f, s));
+        if (pm->fft.lds_plane) {
+            HYMD_TRY(fft_c2r_yz_grad(pm->fft, grad, force_mesh, T, g0.kfac[1], g0.kfac[2], 1.f, s));
+        } else {  // y-z planes too large for LDS: the y and z line passes
+            HYMD_TRY(pm->cscratch.reserve(sizeof(float2) * m.gc * 3 * T));
+            HYMD_TRY(fft_c2r_lines_grad(pm->fft, grad, (float2 *)pm->cscratch.ptr, force_mesh, T, g0.kfac[1],
+                                        g0.kfac[2], 1.f, s));
+        }
         if (energy) {
             double e = 0.0;
             HYMD_TRY(read_result(pm, nb, 1, &e));
@@ -1802,24 +1806,24 @@ extern "C" int hymd_update_field_force_q(hymd_pm *pm, const float *x, const floa
     p.four_pi_elec = 4.0 * M_PI * elec_factor;
     KGeom g = kgeom(pm);
     double *partials = energy_q ? (double *)pm->partials.ptr : nullptr;
-    const int grad_form = pm->fft.lds_plane ? 1 : 0;
-    pme_kspace_kernel<<<pm->kblocks, kBlock, 0, s>>>(spec, out, psi ? 1 : 0, grad_form, g, p, partials);
+    pme_kspace_kernel<<<pm->kblocks, kBlock, 0, s>>>(spec, out, psi ? 1 : 0, 1, g, p, partials);
     HYMD_HIP(hipGetLastError());
     float *field = elec_field;
     if (!field) {
         HYMD_TRY(pm->rscratch.reserve(sizeof(float) * 3 * m.g));
         field = (float *)pm->rscratch.ptr;
     }
-    if (grad_form) {
-        // x-inverse of -i k_x psi and psi; the plane pass forms E_y, E_z
-        HYMD_TRY(fft_inverse_x(pm->fft, out, 2, s));
+    // x-inverse of -i k_x psi and psi; the plane pass (or the y and z line
+    // passes) forms E_y, E_z from the latter
+    HYMD_TRY(fft_inverse_x(pm->fft, out, 2, s));
+    if (pm->fft.lds_plane) {
         HYMD_TRY(fft_c2r_yz_grad(pm->fft, out, field, 1, g.kfac[1], g.kfac[2], 1.f, s));
-        if (psi) HYMD_TRY(fft_c2r_yz(pm->fft, out + m.gc, psi, 1, 1.f, s));
     } else {
-        HYMD_TRY(fft_inverse_x(pm->fft, out, psi ? 4 : 3, s));
-        HYMD_TRY(fft_c2r_yz(pm->fft, out, field, 3, 1.f, s));
-        if (psi) HYMD_TRY(fft_c2r_yz(pm->fft, out + 3 * m.gc, psi, 1, 1.f, s));
+        HYMD_TRY(pm->cscratch.reserve(sizeof(float2) * m.gc * 3));
+        HYMD_TRY(fft_c2r_lines_grad(pm->fft, out, (float2 *)pm->cscratch.ptr, field, 1, g.kfac[1], g.kfac[2], 1.f,
+                                    s));
     }
+    if (psi) HYMD_TRY(fft_c2r_yz(pm->fft, out + m.gc, psi, 1, 1.f, s));
     HYMD_TRY(readout_sorted(pm, 3, 1, field, elec_forces, true));
     if (energy_q) {
         double e = 0.0;

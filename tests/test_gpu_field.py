@@ -339,3 +339,47 @@ def test_tuning_knobs_keep_the_bits(hip, mesh, T, tmp_path):
     assert again["cached"] and tune.get(pm) == res["choice"]
     got = run()
     assert rms_rel(got[1].numpy(), ref[1].numpy()) < 1e-6
+
+
+@pytest.mark.parametrize("T", [1, 3, 6])
+def test_line_gradient_path_matches_oracle(hip, T):
+    """Meshes whose y-z plane does not fit in LDS (16 x 200 x 200 here, like
+    256^3) take the line-pass gradient path: the fused x-line pass, the
+    out-of-place y pass with -i k_y on load and the z pass with -i k_z on load.
+    Field forces, force grids and energy against the FP64 oracle; PME forces
+    and energy against the oracle on the same mesh."""
+    from hymd_amd.md import COULOMB_CONSTANT
+    mesh, box = (16, 200, 200), (3.2, 40.0, 40.0)
+    rng = np.random.default_rng(40 + T)
+    n = 60_000
+    x = (rng.uniform(0, 1, (n, 3)) * np.asarray(box)).astype(np.float32)
+    types = rng.integers(0, T, n).astype(np.int32)
+    A = rng.uniform(0.0, 1.0, (T, T))
+    A = A + A.T + T * np.eye(T)
+    b = rng.uniform(-1, 1, T)
+    ham = F.Quadratic(A, b, 0.3)
+    x64 = x.astype(np.float64)
+    ref = F.update_field(x64, types, T, mesh, box, 0.5, ham)
+    f_ref = F.compute_field_force(x64, types, ref["force_mesh"], mesh, box)
+    pm = _pm(mesh, box, T)
+    xd, td = _dev(x, torch.float32), _dev(types, torch.int32)
+    fm = pm.create("real", batch=(T, 3))
+    e = pm.field_update(xd, td, T, A, b, 0.3, [1.0] * T, 0.5, fm, want_energy=True)
+    f = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    pm.readout_forces(xd, td, T, fm, f)
+    fmg = fm.cpu().numpy()
+    for t in range(T):
+        for d in range(3):
+            assert rms_rel(fmg[t, d], ref["force_mesh"][t][d]) < 1e-5, (t, d)
+    assert rms_rel(f.cpu().numpy(), f_ref) < 1e-5
+    assert e == pytest.approx(F.field_energy(ref["phi"], ham, mesh, box), rel=1e-6)
+    q = np.zeros(n, dtype=np.float32)
+    q[: n // 10] = 1.0
+    q[n // 10: n // 5] = -1.0
+    rq = F.update_field_force_q(x64, q.astype(np.float64), mesh, box, 0.5, COULOMB_CONSTANT, 45.0)
+    fq = torch.empty_like(f)
+    psi = pm.create("real")
+    eq = pm.pme(xd, _dev(q, torch.float32), 0.5, COULOMB_CONSTANT / 45.0, 0.0, fq, psi=psi, want_energy=True)
+    assert rms_rel(fq.cpu().numpy(), rq["forces"]) < 1e-5
+    assert rms_rel(psi.cpu().numpy(), rq["psi"]) < 1e-5
+    assert eq == pytest.approx(F.field_q_energy(rq["phi_q"], rq["psi"], 0.0, mesh, box), rel=1e-5)
