@@ -55,16 +55,17 @@ def algorithmic_bytes(kernel, n, n_bonds, n_angles, T, mesh):
     raise ValueError(kernel)
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` on this workload from the committed
-    calibrated FETCH_SIZE + WRITE_SIZE summary in profiles/, or None."""
+def profiled(kernel, key):
+    """A figure of `kernel` on this workload from the committed counter
+    summary in profiles/ (HBM bytes per launch from calibrated FETCH_SIZE +
+    WRITE_SIZE, or the VALU issue fraction), or None."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_lipid_water_1M_128_hbm_traffic.md")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         for line in fh:
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1])
+            if line.startswith(key + ":"):
+                return float(line.split(":")[1])
     return None
 
 
@@ -581,12 +582,16 @@ def main():
     kname = "fused" if (args.roofline_kernel == "bonded" and md.fused) else args.roofline_kernel
     nbytes = algorithmic_bytes(kname, s.n, len(s.bonds[0]), len(s.angles[0]), s.n_types, s.mesh)
     achieved = nbytes / (dur_ms * 1e-3) / 1e9
-    traffic = None if args.small else measured_traffic(KERNEL_NAMES[kname])
+    traffic = None if args.small else profiled(KERNEL_NAMES[kname], "traffic_bytes_per_launch")
+    traffic = None if traffic is None else int(traffic)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": KERNEL_NAMES[kname], "avg_launch_us": round(dur_ms * 1e3, 2),
                 "launches_timed": launches, "algorithmic_bytes": int(nbytes),
-                "eager_ms_per_step": round(eager_ms, 4)}
+                "eager_ms_per_step": round(eager_ms, 4),
+                # from the committed counter run: VALU wave-instructions per second
+                # against 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles (the kernel's real bound)
+                "valu_issue_frac_profiled": None if args.small else profiled(KERNEL_NAMES[kname], "valu_issue_frac")}
 
     log(f"roofline stage {args.roofline_kernel}: {dur_ms * 1e3:.2f} us/launch over {launches} launches")
     press = None

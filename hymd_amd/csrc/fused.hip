@@ -34,7 +34,7 @@ struct FusedArgs {
     const uint32_t *bond_ab;     // a | b << 16 (block-local)
     const double2 *bond_par;     // r0, k
     const int *blk_ang_off;      // nb+1
-    const uint64_t *ang_abc;     // a | b << 16 | c << 32 (block-local)
+    const uint32_t *ang_abc;     // a | b << 10 | c << 20 (block-local, < 1024)
     const double2 *ang_par;      // theta0, k
     const int *csr_off;          // N+1 (global)
     const uint32_t *csr_ent;     // (term << 3) | (angle << 2) | role, term block-local
@@ -62,9 +62,9 @@ inline size_t fused_lds_bytes(int particles, int bonds, int angles) {
 // F32: bonds and angles evaluated in FP32 (slots and the gather in FP32),
 // kicks, drifts and positions in FP64 as always.
 // Occupancy hints: a 26 KB block fits six times in a CU's LDS, i.e. six waves
-// per SIMD; the FP32 one-particle-per-thread form is held to 80 VGPRs so that
-// registers allow the same six (116 us against 127 us at 96 VGPRs on the
-// 1,048,576-bead lipid/water outer step).
+// per SIMD; the FP32 one-particle-per-thread form fits 80 VGPRs without
+// spills (float2 term parameters, 32-bit angle indices), so registers allow
+// the same six.
 template <int PPT, bool F32>
 __global__ void __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(PPT == 1 ? (F32 ? 6 : 4) : 3))) fused_inner_kernel(FusedArgs A) {
@@ -112,21 +112,27 @@ __attribute__((amdgpu_waves_per_eu(PPT == 1 ? (F32 ? 6 : 4) : 3))) fused_inner_k
             }
         }
     }
-    // own terms in registers
+    // own terms in registers, their parameters in the terms' precision (the
+    // FP64 energies at the end read them again)
+    using P2 = typename std::conditional<F32, float2, double2>::type;
     uint32_t bab[PPT];
-    double2 bpar[PPT];
-    uint64_t aabc[PPT];
-    double2 apar[PPT];
+    P2 bpar[PPT];
+    uint32_t aabc[PPT];
+    P2 apar[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         const int tb = threadIdx.x + k * kBlock;
         if (tb < nb) {
             bab[k] = A.bond_ab[b0 + tb];
-            bpar[k] = A.bond_par[b0 + tb];
+            const double2 q = A.bond_par[b0 + tb];
+            bpar[k].x = q.x;
+            bpar[k].y = q.y;
         }
         if (tb < na) {
             aabc[k] = A.ang_abc[a0 + tb];
-            apar[k] = A.ang_par[a0 + tb];
+            const double2 q = A.ang_par[a0 + tb];
+            apar[k].x = q.x;
+            apar[k].y = q.y;
         }
     }
 
@@ -156,7 +162,7 @@ __attribute__((amdgpu_waves_per_eu(PPT == 1 ? (F32 ? 6 : 4) : 3))) fused_inner_k
                                       xs[3 * ib + 2] - xs[3 * ia + 2]),
                                    box);
                 if constexpr (F32) {
-                    const F3 fa = bond_term_f(to_f3(rab), (float)bpar[k].x, (float)bpar[k].y);
+                    const F3 fa = bond_term_f(to_f3(rab), bpar[k].x, bpar[k].y);
                     sb[3 * tb] = fa.x;
                     sb[3 * tb + 1] = fa.y;
                     sb[3 * tb + 2] = fa.z;
@@ -169,14 +175,14 @@ __attribute__((amdgpu_waves_per_eu(PPT == 1 ? (F32 ? 6 : 4) : 3))) fused_inner_k
                 }
             }
             if (tb < na) {
-                const int ia = (int)(aabc[k] & 0xffff), ib = (int)((aabc[k] >> 16) & 0xffff),
-                          ic = (int)(aabc[k] >> 32);
+                const int ia = (int)(aabc[k] & 0x3ff), ib = (int)((aabc[k] >> 10) & 0x3ff),
+                          ic = (int)(aabc[k] >> 20);
                 const V3 rb = v3(xs[3 * ib], xs[3 * ib + 1], xs[3 * ib + 2]);
                 const V3 ra = mic(v3(xs[3 * ia], xs[3 * ia + 1], xs[3 * ia + 2]) - rb, box);
                 const V3 rc = mic(v3(xs[3 * ic], xs[3 * ic + 1], xs[3 * ic + 2]) - rb, box);
                 if constexpr (F32) {
                     F3 fa, fc;
-                    angle_term_f(ra, rc, (float)apar[k].x, (float)apar[k].y, fa, fc);
+                    angle_term_f(ra, rc, apar[k].x, apar[k].y, fa, fc);
                     sa[6 * tb] = fa.x;
                     sa[6 * tb + 1] = fa.y;
                     sa[6 * tb + 2] = fa.z;
@@ -261,21 +267,23 @@ __attribute__((amdgpu_waves_per_eu(PPT == 1 ? (F32 ? 6 : 4) : 3))) fused_inner_k
                                       xs[3 * ib + 2] - xs[3 * ia + 2]),
                                    box);
                 double en;
-                const V3 fa = bond_term(rab, bpar[k].x, bpar[k].y, en);
+                const double2 q = A.bond_par[b0 + tb];
+                const V3 fa = bond_term(rab, q.x, q.y, en);
                 ev[0] += en;
                 ev[1] += fa.x * rab.x;
                 ev[2] += fa.y * rab.y;
                 ev[3] += fa.z * rab.z;
             }
             if (tb < na) {
-                const int ia = (int)(aabc[k] & 0xffff), ib = (int)((aabc[k] >> 16) & 0xffff),
-                          ic = (int)(aabc[k] >> 32);
+                const int ia = (int)(aabc[k] & 0x3ff), ib = (int)((aabc[k] >> 10) & 0x3ff),
+                          ic = (int)(aabc[k] >> 20);
                 const V3 rb = v3(xs[3 * ib], xs[3 * ib + 1], xs[3 * ib + 2]);
                 const V3 ra = mic(v3(xs[3 * ia], xs[3 * ia + 1], xs[3 * ia + 2]) - rb, box);
                 const V3 rc = mic(v3(xs[3 * ic], xs[3 * ic + 1], xs[3 * ic + 2]) - rb, box);
                 V3 fa, fc;
                 double en;
-                angle_term(ra, rc, apar[k].x, apar[k].y, fa, fc, en);
+                const double2 q = A.ang_par[a0 + tb];
+                angle_term(ra, rc, q.x, q.y, fa, fc, en);
                 ev[4] += en;
                 ev[5] -= fa.x * ra.x + fc.x * rc.x;
                 ev[6] -= fa.y * ra.y + fc.y * rc.y;
@@ -425,7 +433,7 @@ extern "C" int hymd_fused_create(hymd_fused **out, int64_t n, int64_t m2, const 
     std::partial_sum(acount.begin(), acount.end(), acount.begin());
     std::vector<uint32_t> bab(m2);
     std::vector<double2> bpar(m2);
-    std::vector<uint64_t> aabc(m3);
+    std::vector<uint32_t> aabc(m3);
     std::vector<double2> apar(m3);
     std::vector<int> bfill(bcount.begin(), bcount.end() - 1), afill(acount.begin(), acount.end() - 1);
     std::vector<int> bond_local(m2), ang_local(m3);
@@ -437,7 +445,7 @@ extern "C" int hymd_fused_create(hymd_fused **out, int64_t n, int64_t m2, const 
     }
     for (int64_t t = 0; t < m3; ++t) {
         const int b = blk_of[A3[t]], s = blk_start[b], slot = afill[b]++;
-        aabc[slot] = (uint64_t)(A3[t] - s) | ((uint64_t)(B3[t] - s) << 16) | ((uint64_t)(C3[t] - s) << 32);
+        aabc[slot] = (uint32_t)(A3[t] - s) | ((uint32_t)(B3[t] - s) << 10) | ((uint32_t)(C3[t] - s) << 20);
         apar[slot] = make_double2(T0[t], K3[t]);
         ang_local[t] = slot - acount[b];
     }
@@ -539,7 +547,7 @@ int fused_inner_impl(hymd_fused *f, double *x, double *v, float *x32, float *f_f
     A.bond_ab = (const uint32_t *)f->bond_ab.ptr;
     A.bond_par = (const double2 *)f->bond_par.ptr;
     A.blk_ang_off = (const int *)f->blk_ang_off.ptr;
-    A.ang_abc = (const uint64_t *)f->ang_abc.ptr;
+    A.ang_abc = (const uint32_t *)f->ang_abc.ptr;
     A.ang_par = (const double2 *)f->ang_par.ptr;
     A.csr_off = (const int *)f->csr_off.ptr;
     A.csr_ent = (const uint32_t *)f->csr_ent.ptr;
