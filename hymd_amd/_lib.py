@@ -40,6 +40,8 @@ SIGNATURES = {
     "hymd_update_field": [_p, _p, _p, _i64, _i32, _pd, _pd, _d, _pd, _d, _p, _p, _p, _p, _pd],
     "hymd_compute_field_force": [_p, _p, _p, _i64, _i32, _p, _p],
     "hymd_update_field_force_q": [_p, _p, _p, _i64, _d, _d, _d, _p, _p, _p, _p, _pd],
+    "hymd_update_field_force_q_gpe": [_p, _p, _p, _p, _i64, _i32, _p, _pd, _pd, _d, _d, _d, _d, _i32, _i32, _p, _p,
+                                      _p, _p, _p, _p, _p, _p, _pd, _pi32, _pd],
     "hymd_field_energy": [_p, _p, _i32, _pd, _pd, _d, _pd],
     "hymd_field_q_energy": [_p, _p, _p, _d, _pd],
     "hymd_self_energy_q": [_p, _i64, _d, _d, _pd, _p],

@@ -53,6 +53,13 @@ class Config:
     dielectric_const: Optional[float] = None
     self_energy: Optional[float] = None
     type_charges: Optional[list] = None
+    # PIC_Spectral_GPE (input_parser.py:195-200, 1256-1262): one dielectric per
+    # type (after sort_dielectric_by_type_id), mixing 0.6 and tolerance 1e-6
+    # by default, convergence "max_diff" by default (main.py:137-165)
+    dielectric_type: Optional[list] = None
+    pol_mixing: Optional[float] = None
+    conv_crit: Optional[float] = None
+    convergence_type: Optional[str] = None
     barostat: Optional[str] = None           # "isotropic" | "semiisotropic"
     rho0: Optional[float] = None
     a: Optional[float] = None

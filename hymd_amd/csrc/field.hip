@@ -13,6 +13,7 @@
 // of the reference's 6T transforms (field.py:573-616).
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "fft.h"
 #include "field_dev.h"
@@ -1881,5 +1882,273 @@ extern "C" int hymd_self_energy_q(const float *charges, int64_t n, double sigma,
     HYMD_HIP(hipMemcpyAsync(&q2, part + nb, sizeof(double), hipMemcpyDeviceToHost, s));
     HYMD_HIP(hipStreamSynchronize(s));
     *out = elec_factor * std::sqrt(1.0 / (2.0 * M_PI * sigma * sigma)) * q2;
+    return HYMD_OK;
+}
+
+// ===========================================================================
+// Variable-dielectric electrostatics: update_field_force_q_GPE
+// (field.py:764-1112) and compute_field_energy_q_GPE (field.py:706-761)
+// ===========================================================================
+namespace hymd {
+
+struct GpeK {
+    int T;
+    double eps[HYMD_MAX_TYPES];  // relative dielectric per type (config.dielectric_type)
+    double q[HYMD_MAX_TYPES];    // charge per type (config.type_charges)
+};
+
+// den = sum_t phi_t, eps = sum_t eps_t phi_t / den where den > 1e-6 (the
+// caller's previous value elsewhere, as np.divide(..., where=) leaves it);
+// then phi_q /= eps where eps > 1e-6 (field.py:982-992)
+__global__ void __launch_bounds__(kBlock) gpe_eps_kernel(const float *__restrict__ phi, int64_t g, GpeK p,
+                                                          float *__restrict__ eps, float *__restrict__ phi_q,
+                                                          float *__restrict__ den_out) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < g; c += stride) {
+        double den = 0.0, num = 0.0;
+        for (int t = 0; t < p.T; ++t) {
+            const double v = phi[(int64_t)t * g + c];
+            num += p.eps[t] * v;
+            den += v;
+        }
+        float e = eps[c];
+        if (den > 1e-6) e = (float)(num / den);
+        eps[c] = e;
+        den_out[c] = (float)den;
+        if (e > 1e-6f) phi_q[c] = phi_q[c] / e;
+    }
+}
+
+// The GPE's k-space multiplies of unnormalised spectra (scale = 1/G folds the
+// r2c normalisation in; |k|^2 := 1 at k = 0, k_d = 0 on Nyquist planes):
+//   0: out[d] = i k_d in            (gradient of the dielectric)
+//   1: out[d] = -i k_d in / |k|^2   (field of a charge density, without 4 pi k_e)
+//   2: out[d] = -i k_d in / |k|^2, out[3] = in / |k|^2   (field and potential)
+//   3: out[3t + d] = -i k_d H in[t], t < T                 (force grids of V_t)
+//   4: out[0] = H in                                       (the filter)
+__global__ void __launch_bounds__(kBlock) gpe_kspace_kernel(const float2 *__restrict__ in, float2 *__restrict__ out,
+                                                             KGeom g, int mode, int T, double sigma2, double scale) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; kk < g.gc; kk += stride) {
+        double k[3], kd[3], wgt;
+        kvec(g, kk, k, kd, wgt);
+        double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        if (kk == 0) k2 = 1.0;
+        if (mode == 3) {
+            const double h = exp(-0.5 * sigma2 * (kk == 0 ? 0.0 : k2)) * scale;
+            for (int t = 0; t < T; ++t) {
+                const float2 v = in[(int64_t)t * g.gc + kk];
+                const double re = h * v.x, im = h * v.y;
+                for (int d = 0; d < 3; ++d)
+                    out[(int64_t)(3 * t + d) * g.gc + kk] = make_float2((float)(kd[d] * im), (float)(-kd[d] * re));
+            }
+            continue;
+        }
+        const float2 v = in[kk];
+        if (mode == 4) {
+            const double h = exp(-0.5 * sigma2 * (kk == 0 ? 0.0 : k2)) * scale;
+            out[kk] = make_float2((float)(h * v.x), (float)(h * v.y));
+            continue;
+        }
+        const double s = (mode == 0) ? scale : scale / k2;
+        const double re = s * v.x, im = s * v.y;
+        for (int d = 0; d < 3; ++d) {
+            if (mode == 0)  // i k v = (-k v.y, k v.x)
+                out[(int64_t)d * g.gc + kk] = make_float2((float)(-kd[d] * im), (float)(kd[d] * re));
+            else            // -i k v = (k v.y, -k v.x)
+                out[(int64_t)d * g.gc + kk] = make_float2((float)(kd[d] * im), (float)(-kd[d] * re));
+        }
+        if (mode == 2) out[(int64_t)3 * g.gc + kk] = make_float2((float)re, (float)im);
+    }
+}
+
+// phi_eta_d /= eps where eps > 1e-6 (field.py:1007)
+__global__ void __launch_bounds__(kBlock) gpe_eta_kernel(float *__restrict__ eta, const float *__restrict__ eps,
+                                                          int64_t g) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < g; c += stride) {
+        const float e = eps[c];
+        if (e > 1e-6f)
+            for (int d = 0; d < 3; ++d) eta[(int64_t)d * g + c] = eta[(int64_t)d * g + c] / e;
+    }
+}
+
+// out = a (phi_q + pol)
+__global__ void __launch_bounds__(kBlock) gpe_sum_kernel(const float *__restrict__ phi_q, const float *__restrict__ pol,
+                                                          float a, float *__restrict__ out, int64_t g) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < g; c += stride)
+        out[c] = a * (phi_q[c] + pol[c]);
+}
+
+// One polarisation update (field.py:1026-1040): pol <- w (-eta . E) +
+// (1 - w) pol, per block the sum, sum of squares and maximum of |change|.
+__global__ void __launch_bounds__(kBlock) gpe_pol_kernel(const float *__restrict__ eta, const float *__restrict__ E,
+                                                          float *__restrict__ pol, int64_t g, float w,
+                                                          double *__restrict__ partials) {
+    double acc[2] = {0.0, 0.0};
+    double mx = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < g; c += stride) {
+        float p = -(eta[c] * E[c] + eta[g + c] * E[g + c] + eta[2 * g + c] * E[2 * g + c]);
+        const float prev = pol[c];
+        p = w * p + (1.0f - w) * prev;
+        const double diff = fabs((double)(p - prev));
+        acc[0] += diff;
+        acc[1] += diff * diff;
+        mx = fmax(mx, diff);
+        pol[c] = p;
+    }
+    block_reduce_store<2>(acc, partials + 3 * (int64_t)blockIdx.x);
+    // the block maximum: wave shuffles, then the waves' maxima in LDS
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_down(mx, o, 64));
+    __shared__ double wmax[kBlock / 64];
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int k = 0; k < kBlock / 64; ++k) m = fmax(m, wmax[k]);
+        partials[3 * (int64_t)blockIdx.x + 2] = m;
+    }
+}
+
+// elec_dot = sum_d E_d^2, contrib = elec_dot / den where den > 1e-6 (kept
+// elsewhere), V_t = q_t psi - inv2e (eps_t - eps) contrib (field.py:1068-1086);
+// per block sum eps elec_dot (the energy, field.py:757-759)
+__global__ void __launch_bounds__(kBlock) gpe_vbar_kernel(const float *__restrict__ E, const float *__restrict__ psi,
+                                                           const float *__restrict__ eps, const float *__restrict__ den,
+                                                           GpeK p, float inv2e, int64_t g,
+                                                           float *__restrict__ contrib, float *__restrict__ elec_dot,
+                                                           float *__restrict__ vbar, double *__restrict__ partials) {
+    double acc[1] = {0.0};
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < g; c += stride) {
+        const float e0 = E[c], e1 = E[g + c], e2 = E[2 * g + c];
+        const float dot = e0 * e0 + e1 * e1 + e2 * e2;
+        if (elec_dot) elec_dot[c] = dot;
+        const float d = den[c];
+        float cb = contrib[c];
+        if (d > 1e-6f) cb = dot / d;
+        contrib[c] = cb;
+        const float ec = eps[c], ps = psi[c];
+        for (int t = 0; t < p.T; ++t)
+            vbar[(int64_t)t * g + c] = (float)p.q[t] * ps - inv2e * ((float)p.eps[t] - ec) * cb;
+        acc[0] += (double)ec * (double)dot;
+    }
+    block_reduce_store<1>(acc, partials + blockIdx.x);
+}
+
+}  // namespace hymd
+
+extern "C" int hymd_update_field_force_q_gpe(hymd_pm *pm, const float *x, const int32_t *types, const float *charges,
+                                             int64_t n, int32_t n_types, const float *phi,
+                                             const double *dielectric_type, const double *type_charges, double sigma,
+                                             double coulomb_constant, double conv_crit, double pol_mixing,
+                                             int32_t conv_type, int32_t max_iter, float *phi_eps,
+                                             const float *phi_pol_prev, float *phi_pol, float *elec_field_contrib,
+                                             float *psi, float *elec_dot, float *vbar, float *elec_forces,
+                                             double *energy, int32_t *iterations, double *delta_out) {
+    HYMD_REQUIRE(pm && phi && dielectric_type && type_charges && phi_eps && elec_field_contrib && elec_forces,
+                 "null argument");
+    HYMD_REQUIRE(n >= 0 && (n == 0 || (x && types && charges)), "bad particle arrays");
+    HYMD_REQUIRE(n_types >= 1 && n_types <= pm->max_types, "n_types out of range");
+    HYMD_REQUIRE(conv_type >= 0 && conv_type <= 2, "conv_type must be 0 (max), 1 (sum) or 2 (sum of squares)");
+    HYMD_REQUIRE(max_iter >= 0 && sigma >= 0.0 && coulomb_constant > 0.0, "bad parameters");
+    const Mesh &m = pm->mesh;
+    const int T = n_types;
+    const int64_t G = m.g;
+    hipStream_t s = pm->stream;
+    GpeK p{};
+    p.T = T;
+    for (int t = 0; t < T; ++t) {
+        p.eps[t] = dielectric_type[t];
+        p.q[t] = type_charges[t];
+    }
+    // real scratch: phi_q, den, eta (3), E (3), psi, tmp, pol, V (T), force grids (3T)
+    HYMD_TRY(pm->rscratch.reserve(sizeof(float) * (size_t)G * (11 + 4 * (size_t)T)));
+    float *phq = (float *)pm->rscratch.ptr, *den = phq + G, *eta = den + G, *E = eta + 3 * G, *psi_i = E + 3 * G;
+    float *tmp = psi_i + G, *pol_i = tmp + G, *vb_i = pol_i + G, *fm = vb_i + (int64_t)T * G;
+    float *pol = phi_pol ? phi_pol : pol_i;
+    float *vb = vbar ? vbar : vb_i;
+    float2 *spec = (float2 *)pm->spec.ptr, *grad = (float2 *)pm->grad.ptr;
+    const KGeom g = kgeom(pm);
+    const double inv_g = 1.0 / (double)G;
+    const double s2 = sigma * sigma;
+    const int kb = pm->kblocks;
+    const int eb = grid_for(G, kBlock, 4096);
+    HYMD_TRY(pm->partials.reserve(sizeof(double) * 3 * (size_t)eb));
+    double *part = (double *)pm->partials.ptr;
+    // 1. smeared charge density (field.py:971-980)
+    HYMD_TRY(sort_particles(pm, x, nullptr, charges, n));
+    HYMD_TRY(paint_sorted(pm, 1, nullptr, phq, nullptr));
+    HYMD_TRY(fft_r2c_real(pm->fft, phq, spec, 1, s));
+    gpe_kspace_kernel<<<kb, kBlock, 0, s>>>(spec, grad, g, 4, 1, s2, inv_g);
+    HYMD_HIP(hipGetLastError());
+    HYMD_TRY(fft_c2r(pm->fft, grad, phq, 1, 1.f, s));
+    // 2-3. dielectric, phi_q / eps
+    gpe_eps_kernel<<<eb, kBlock, 0, s>>>(phi, G, p, phi_eps, phq, den);
+    HYMD_HIP(hipGetLastError());
+    // 4. grad(eps) / eps
+    HYMD_TRY(fft_r2c_real(pm->fft, phi_eps, spec, 1, s));
+    gpe_kspace_kernel<<<kb, kBlock, 0, s>>>(spec, grad, g, 0, 1, s2, inv_g);
+    HYMD_HIP(hipGetLastError());
+    HYMD_TRY(fft_c2r(pm->fft, grad, eta, 3, 1.f, s));
+    gpe_eta_kernel<<<eb, kBlock, 0, s>>>(eta, phi_eps, G);
+    HYMD_HIP(hipGetLastError());
+    // 5. polarisation charges from the caller's previous grid
+    if (phi_pol_prev)
+        HYMD_HIP(hipMemcpyAsync(pol, phi_pol_prev, sizeof(float) * G, hipMemcpyDeviceToDevice, s));
+    else
+        HYMD_TRY(zero_words(pol, G, s));
+    std::vector<double> hp(3 * (size_t)eb);
+    int it = 0;
+    double delta = 1.0;
+    while (it < max_iter && delta > conv_crit) {
+        gpe_sum_kernel<<<eb, kBlock, 0, s>>>(phq, pol, 1.f, tmp, G);
+        HYMD_TRY(fft_r2c_real(pm->fft, tmp, spec, 1, s));
+        gpe_kspace_kernel<<<kb, kBlock, 0, s>>>(spec, grad, g, 1, 1, s2, inv_g);
+        HYMD_HIP(hipGetLastError());
+        HYMD_TRY(fft_c2r(pm->fft, grad, E, 3, 1.f, s));
+        gpe_pol_kernel<<<eb, kBlock, 0, s>>>(eta, E, pol, G, (float)pol_mixing, part);
+        HYMD_HIP(hipGetLastError());
+        HYMD_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * 3 * eb, hipMemcpyDeviceToHost, s));
+        HYMD_HIP(hipStreamSynchronize(s));
+        double sum = 0.0, sq = 0.0, mx = 0.0;
+        for (int b = 0; b < eb; ++b) {
+            sum += hp[3 * b];
+            sq += hp[3 * b + 1];
+            mx = std::max(mx, hp[3 * b + 2]);
+        }
+        delta = conv_type == 0 ? mx : (conv_type == 1 ? sum : sq);
+        ++it;
+    }
+    // 6. potential and field: psi = 4 pi k_e (phi_q + pol) / |k|^2
+    const double eps0_inv = 4.0 * M_PI * coulomb_constant;
+    gpe_sum_kernel<<<eb, kBlock, 0, s>>>(phq, pol, (float)eps0_inv, tmp, G);
+    HYMD_TRY(fft_r2c_real(pm->fft, tmp, spec, 1, s));
+    gpe_kspace_kernel<<<kb, kBlock, 0, s>>>(spec, grad, g, 2, 1, s2, inv_g);
+    HYMD_HIP(hipGetLastError());
+    HYMD_TRY(fft_c2r(pm->fft, grad, E, 4, 1.f, s));  // E (3 grids), then psi_i
+    if (psi) HYMD_HIP(hipMemcpyAsync(psi, psi_i, sizeof(float) * G, hipMemcpyDeviceToDevice, s));
+    // 7. |E|^2, contrib, V_t and the energy partials
+    gpe_vbar_kernel<<<eb, kBlock, 0, s>>>(E, psi_i, phi_eps, den, p, (float)(0.5 / eps0_inv), G, elec_field_contrib,
+                                          elec_dot, vb, part);
+    HYMD_HIP(hipGetLastError());
+    double e_sum = 0.0;
+    if (energy) {
+        HYMD_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * eb, hipMemcpyDeviceToHost, s));
+        HYMD_HIP(hipStreamSynchronize(s));
+        for (int b = 0; b < eb; ++b) e_sum += hp[b];
+    }
+    // 8. force grids -i k_d H V_t and the per-type readout
+    HYMD_TRY(fft_r2c_real(pm->fft, vb, spec, T, s));
+    gpe_kspace_kernel<<<kb, kBlock, 0, s>>>(spec, grad, g, 3, T, s2, inv_g);
+    HYMD_HIP(hipGetLastError());
+    HYMD_TRY(fft_c2r(pm->fft, grad, fm, 3 * T, 1.f, s));
+    HYMD_TRY(sort_particles(pm, x, types, nullptr, n));
+    HYMD_TRY(readout_sorted(pm, 3, T, fm, elec_forces, false));
+    if (energy) *energy = cell_volume(pm) * (0.5 / eps0_inv) * e_sum;
+    if (iterations) *iterations = it;
+    if (delta_out) *delta_out = delta;
     return HYMD_OK;
 }

@@ -278,6 +278,23 @@ def initialize_pm(pmesh, config, comm=None):
         elec_common_list = [phi_q, phi_q_fourier, psi, elec_field]
         eff_all = pm.create("complex", batch=3)
         coulomb_list = [[eff_all[d] for d in range(3)], pm.create("complex")]
+    elif config.coulombtype == "PIC_Spectral_GPE":
+        phi_q = pm.create("real")
+        phi_q_fourier = pm.create("complex")
+        psi = pm.create("real")
+        ef_all = pm.create("real", batch=3)
+        elec_common_list = [phi_q, phi_q_fourier, psi, [ef_all[d] for d in range(3)]]
+        eta_all = pm.create("real", batch=3)
+        etaf_all = pm.create("complex", batch=3)
+        vb_all = pm.create("real", batch=T)
+        vbf_all = pm.create("complex", batch=T)
+        fme_all = pm.create("real", batch=(T, 3))
+        fmef_all = pm.create("complex", batch=(T, 3))
+        coulomb_list = [pm.create("real"), pm.create("complex"), [eta_all[d] for d in range(3)],
+                        [etaf_all[d] for d in range(3)], pm.create("real"), pm.create("real"), pm.create("real"),
+                        pm.create("real"), [vb_all[t] for t in range(T)], [vbf_all[t] for t in range(T)],
+                        [[fme_all[t, d] for d in range(3)] for t in range(T)],
+                        [[fmef_all[t, d] for d in range(3)] for t in range(T)]]
     elif config.coulombtype is not None:
         raise NotImplementedError(f"coulombtype {config.coulombtype!r} is not on the HIP path")
     field_list = [phi, phi_fourier, force_on_grid, v_ext_fourier, v_ext, phi_transfer, phi_laplacian]
@@ -363,6 +380,71 @@ def update_field_force_q(charges, phi_q, phi_q_fourier, psi, psi_fourier, elec_f
             elec_forces.copy_(out)
         else:
             np.copyto(elec_forces, out.cpu().numpy().astype(elec_forces.dtype))
+
+
+CONVERGENCE_TYPES = {"max_diff": 0, "csum": 1, "euclidean_norm": 2}
+
+
+def update_field_force_q_GPE(conv_fun, phi, types, charges, phi_q, phi_q_fourier, phi_eps, phi_eps_fourier,
+                             phi_eta, phi_eta_fourier, phi_pol_prev, phi_pol, elec_field, elec_forces,
+                             elec_field_contrib, psi, Vbar_elec, Vbar_elec_fourier, force_mesh_elec,
+                             force_mesh_elec_fourier, hamiltonian, layout_q, layouts, pm, positions, config,
+                             comm=None):
+    """field.py:764-1112 on the device (``hymd_update_field_force_q_gpe``):
+    the polarisation iteration runs in the library, with the convergence
+    measure of ``config.convergence_type`` ("max_diff", the default, "csum"
+    or "euclidean_norm"; ``conv_fun`` is accepted for the signature).  Writes
+    ``phi_eps``, ``elec_field_contrib``, ``psi``, ``phi_pol`` (the converged
+    polarisation; ``phi_pol_prev`` is read, not changed, as the reference's
+    loop rebinds the name), ``Vbar_elec`` and ``elec_forces``; returns
+    (Vbar_elec, phi_eps, elec_dot), elec_dot a new grid.  The iteration count
+    and last convergence measure are kept as ``pm.gpe_iterations`` /
+    ``pm.gpe_delta``, the energy of compute_field_energy_q_GPE as
+    ``pm.last_field_q_energy``."""
+    T = int(config.n_types)
+    x = _positions32(positions)
+    ty = _device_tensor(types, torch.int32, "types")
+    q = _device_tensor(charges, torch.float32, "charges")
+    dev = pm.device
+    phi_t = _Out(phi, pm.real_shape, torch.float32, dev)
+    if phi_t.flat is None:   # an unpacked list: gather it
+        phi_t.tensor.copy_(torch.stack([g for g in _flatten(phi)]))
+    vb = _Out(Vbar_elec, pm.real_shape, torch.float32, dev)
+    out = elec_forces if (isinstance(elec_forces, torch.Tensor) and elec_forces.is_cuda
+                          and elec_forces.dtype == torch.float32 and elec_forces.is_contiguous()) else \
+        torch.empty((x.shape[0], 3), dtype=torch.float32, device=dev)
+    elec_dot = pm.create("real")
+    kind = getattr(config, "convergence_type", None) or "max_diff"
+    it = ctypes.c_int32(0)
+    delta = ctypes.c_double(0.0)
+    energy = ctypes.c_double(0.0)
+    eps = [float(v) for v in config.dielectric_type]
+    tq = [float(v) for v in (config.type_charges if config.type_charges is not None else [0.0] * T)]
+    call("hymd_update_field_force_q_gpe", pm.handle, dptr(x, torch.float32), dptr(ty, torch.int32),
+         dptr(q, torch.float32), int(x.shape[0]), T, dptr(phi_t.tensor, torch.float32), darr(eps, T), darr(tq, T),
+         float(config.sigma), float(config.coulomb_constant), float(config.conv_crit or 1e-6),
+         float(config.pol_mixing if config.pol_mixing is not None else 0.6), CONVERGENCE_TYPES[kind], 100,
+         dptr(phi_eps, torch.float32, "phi_eps"), dptr(phi_pol_prev, torch.float32, "phi_pol_prev"),
+         dptr(phi_pol, torch.float32, "phi_pol"), dptr(elec_field_contrib, torch.float32, "elec_field_contrib"),
+         dptr(psi, torch.float32, "psi"), dptr(elec_dot, torch.float32), dptr(vb.tensor, torch.float32),
+         dptr(out, torch.float32), ctypes.byref(energy), ctypes.byref(it), ctypes.byref(delta))
+    vb.finish()
+    if out is not elec_forces:
+        if isinstance(elec_forces, torch.Tensor):
+            elec_forces.copy_(out)
+        else:
+            np.copyto(elec_forces, out.cpu().numpy().astype(elec_forces.dtype))
+    pm.gpe_iterations, pm.gpe_delta, pm.last_field_q_energy = it.value, delta.value, energy.value
+    return Vbar_elec, phi_eps, elec_dot
+
+
+def compute_field_energy_q_GPE(config, phi_eps, field_q_energy, dot_elec, comm=None):
+    """field.py:706-761: dV (eps_0 / 2) sum phi_eps |E|^2 with eps_0 =
+    1/(4 pi k_e), an FP64 sum over the cells."""
+    V = float(np.prod(config.box_size))
+    G = int(np.prod(np.broadcast_to(np.asarray(config.mesh_size), (3,))))
+    eps_0 = 1.0 / (config.coulomb_constant * 4.0 * np.pi)
+    return V / G * (0.5 * eps_0) * float(torch.sum(phi_eps.double() * dot_elec.double()))
 
 
 def compute_field_and_kinetic_energy(phi, phi_q, psi, velocity, hamiltonian, positions, types, v_ext, config,

@@ -114,6 +114,31 @@ int hymd_update_field_force_q(hymd_pm *pm, const float *x, const float *charges,
                               double elec_factor, double self_energy, float *elec_forces, float *phi_q,
                               float *psi, float *elec_field, double *energy_q);
 
+/* update_field_force_q_GPE (field.py:764-1112): electrostatics with the
+ * dielectric eps(r) = sum_t eps_t phi~_t / sum_t phi~_t of the filtered
+ * densities phi (device float32 (n_types,Mx,My,Mz), as update_field writes
+ * them), solved by the reference's polarisation-charge iteration (mixing
+ * pol_mixing, at most max_iter iterations while the convergence measure of
+ * |change| exceeds conv_crit: conv_type 0 its maximum (max_diff), 1 its sum
+ * (csum), 2 its sum of squares (euclidean_norm)).
+ *   charges device float32 (n) painted, types device int32 (n) read out
+ *   dielectric_type, type_charges host double (n_types)
+ *   phi_eps, elec_field_contrib device float32 grids in/out: kept where the
+ *     density (resp. its sum) is <= 1e-6, as the reference's masked divisions
+ *   phi_pol_prev device grid in or NULL (zero): the iteration's start
+ *   phi_pol, psi, elec_dot device grids out or NULL; vbar (n_types grids) out or NULL
+ *   elec_forces device float32 (n,3) out: readout of c2r(-i k H r2c(V_t)) per type
+ *   energy host out or NULL: compute_field_energy_q_GPE (field.py:706-761)
+ *   iterations, delta host out or NULL
+ * Synchronises the handle's stream once per iteration. */
+int hymd_update_field_force_q_gpe(hymd_pm *pm, const float *x, const int32_t *types, const float *charges, int64_t n,
+                                  int32_t n_types, const float *phi, const double *dielectric_type,
+                                  const double *type_charges, double sigma, double coulomb_constant,
+                                  double conv_crit, double pol_mixing, int32_t conv_type, int32_t max_iter,
+                                  float *phi_eps, const float *phi_pol_prev, float *phi_pol,
+                                  float *elec_field_contrib, float *psi, float *elec_dot, float *vbar,
+                                  float *elec_forces, double *energy, int32_t *iterations, double *delta);
+
 /* compute_field_and_kinetic_energy (field.py:688-703), real-space sums in FP64:
  * field energy  dV sum_cells w(phi~) from filtered densities phi (n_types,Mx,My,Mz);
  * charge energy dV sum_cells (1/2 phi_q psi - self_energy/V). host double out. */

@@ -383,3 +383,60 @@ def test_line_gradient_path_matches_oracle(hip, T):
     assert rms_rel(fq.cpu().numpy(), rq["forces"]) < 1e-5
     assert rms_rel(psi.cpu().numpy(), rq["psi"]) < 1e-5
     assert eq == pytest.approx(F.field_q_energy(rq["phi_q"], rq["psi"], 0.0, mesh, box), rel=1e-5)
+
+
+@pytest.mark.parametrize("conv", ["max_diff", "csum", "euclidean_norm"])
+def test_gpe_matches_oracle(hip, conv):
+    """update_field_force_q_GPE on the device (through the reference's call
+    surface: initialize_pm's grid set, update_field's filtered densities)
+    against the FP64 oracle: dielectric, potential, |E|^2, V_t and forces to
+    FP32 rounding of the iteration (1e-4 relative RMS), the same iteration
+    count to within one, the energy to 1e-4; the masked cells keep their
+    previous values."""
+    from hymd_amd import config as C
+    from hymd_amd import field as HF
+    from oracle import gpe as OG
+    rng = np.random.default_rng(17)
+    M, L, T = (24, 24, 24), np.array([6.0, 6.0, 6.0]), 3
+    n = 9000
+    x = rng.uniform(0, 1, (n, 3)) * L
+    types = np.where(x[:, 0] < 2.0, 0, np.where(x[:, 0] < 4.0, 1, 2)).astype(np.int32)
+    q = np.zeros(n, dtype=np.float32)
+    q[:450], q[450:900] = 1.0, -1.0
+    cfg = C.Config(n_steps=1, time_step=0.03, mesh_size=M, sigma=0.6, kappa=0.05, box_size=L, n_types=T,
+                   coulombtype="PIC_Spectral_GPE", dielectric_type=[4.0, 30.0, 80.0], type_charges=[0.5, 0.0, -0.5],
+                   conv_crit={"max_diff": 1e-6, "csum": 1e-3, "euclidean_norm": 1e-9}[conv], convergence_type=conv)
+    pm, fields, elec, coul = HF.initialize_pm(None, cfg)
+    x32 = x.astype(np.float32)
+    ham = F.Quadratic(np.eye(T), np.zeros(T), 0.0)
+    A, b, c0 = np.eye(T), np.zeros(T), 0.0
+    fm = pm.create("real", batch=(T, 3))
+    ph = pm.create("real", batch=T)
+    pm.field_update(_dev(x32, torch.float32), _dev(types, torch.int32), T, A, b, c0, [1.0] * T, cfg.sigma, fm, phi=ph)
+    phi_q, phi_q_f, psi, efield = elec
+    (phi_eps, phi_eps_f, phi_eta, phi_eta_f, phi_pol, phi_pol_prev, elec_dot, contrib, vbar, vbar_f, fme,
+     fme_f) = coul
+    phi_eps.fill_(7.0)
+    contrib.fill_(-3.0)
+    forces = torch.empty((n, 3), dtype=torch.float32, device="cuda")
+    vb, pe, ed = HF.update_field_force_q_GPE(None, [ph[t] for t in range(T)], _dev(types, torch.int32),
+                                             _dev(q, torch.float32), phi_q, phi_q_f, phi_eps, phi_eps_f, phi_eta,
+                                             phi_eta_f, phi_pol_prev, phi_pol, efield, forces, contrib, psi, vbar,
+                                             vbar_f, fme, fme_f, None, None, None, pm, _dev(x32, torch.float32), cfg)
+    e_dev = HF.compute_field_energy_q_GPE(cfg, pe, 0.0, ed)
+    x64 = x32.astype(np.float64)
+    ref_phi = F.update_field(x64, types, T, M, L, cfg.sigma, ham)["phi"]
+    ref = OG.update_field_force_q_gpe(x64, types, q.astype(np.float64), ref_phi, M, L, cfg.sigma, cfg.dielectric_type,
+                                      cfg.type_charges, cfg.coulomb_constant, conv_crit=cfg.conv_crit, conv=conv,
+                                      phi_eps=np.full(M, 7.0), elec_field_contrib=np.full(M, -3.0))
+    assert abs(pm.gpe_iterations - ref["iterations"]) <= 1
+    assert rms_rel(pe.cpu().numpy(), ref["phi_eps"]) < 1e-6
+    assert rms_rel(psi.cpu().numpy(), ref["psi"]) < 1e-4
+    assert rms_rel(ed.cpu().numpy(), ref["elec_dot"]) < 1e-4
+    assert rms_rel(phi_pol.cpu().numpy(), ref["phi_pol"]) < 1e-3
+    for t in range(T):
+        assert rms_rel(vb[t].cpu().numpy(), ref["Vbar"][t]) < 1e-4
+    assert rms_rel(forces.cpu().numpy(), ref["forces"]) < 1e-4
+    assert e_dev == pytest.approx(ref["energy"], rel=1e-4)
+    assert pm.last_field_q_energy == pytest.approx(ref["energy"], rel=1e-4)
+    assert torch.all(phi_pol_prev == 0)     # read, not changed
