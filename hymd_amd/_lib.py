@@ -78,6 +78,8 @@ SIGNATURES = {
     "hymd_md_create": [ctypes.POINTER(_p), _p, _p, _i64, _i32, _p, _p, _p, _p, _p, _p, _p, _pd, _pd, _d, _pd, _d,
                        _d, _d, _i32, _pd],
     "hymd_md_step": [_p, _i32, _i32, _pd, _p],
+    "hymd_md_step_half": [_p, _i32, _pd, _pd, _p],
+    "hymd_md_set_spectra": [_p, _p, _p, _p],
     "hymd_md_destroy": [_p],
     "hymd_md_time_stage": [_p, _i32, _i32],
     "hymd_md_set_fused": [_p, _i32],

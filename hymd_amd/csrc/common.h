@@ -338,7 +338,7 @@ int row_order(hymd_pm *pm, const float *x, int64_t n, int *order, hipStream_t s)
 // positions x already produced into field_key_target()'s arrays
 int update_field_keyed(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
                        const double *coupling, const double *shift, double constant, const double *mass,
-                       double sigma, float *force_mesh, double *energy);
+                       double sigma, float *force_mesh, double *energy, void *phi_fourier = nullptr);
 // force arrays of the outer step: mode 0 out = a + (float)b, 1 out += a,
 // 2 out += (float)b, 3 out = a; and float64 <-> float32 copies
 int force_combine(float *out, const float *a, const double *b, int64_t n3, int mode, hipStream_t s);

@@ -1522,9 +1522,9 @@ int field_key_target(hymd_pm *pm, int64_t n, RowKeyTarget *t) {
 
 int update_field_keyed(hymd_pm *pm, const float *x, const int32_t *types, int64_t n, int32_t n_types,
                        const double *coupling, const double *shift, double constant, const double *mass,
-                       double sigma, float *force_mesh, double *energy) {
+                       double sigma, float *force_mesh, double *energy, void *phi_fourier) {
     return update_field_impl(pm, x, types, n, n_types, coupling, shift, constant, mass, sigma, force_mesh, nullptr,
-                             nullptr, nullptr, energy, true);
+                             phi_fourier, nullptr, energy, true);
 }
 }  // namespace hymd
 

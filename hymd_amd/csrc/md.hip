@@ -37,6 +37,10 @@ struct hymd_md {
     int respa_inner = 1;
     hymd_fused *fused = nullptr;  // fused inner loop (null: per-kernel path)
     int pending = 0;              // fused: outer half kick of the last step not yet applied
+    // outputs for a pressure between the step's halves (hymd_md_set_spectra):
+    // every field update writes the filtered spectra (and PME psi, phi_q)
+    void *spectra = nullptr;
+    float *psi = nullptr, *phi_q = nullptr;
     int term_precision = HYMD_F64; // fused inner loop: bond / angle terms in FP64 or FP32
     // coupling after outer steps (main.py:1314-1321): CSVR thermostat every
     // n_b steps, COM momentum removal every com_every steps; `step` counts the
@@ -185,7 +189,7 @@ static int peptide_slow_forces(hymd_md *m, bool energy) {
     HYMD_TRY(force_combine(m->f_slow, m->f_field, nullptr, n3, 3, s));
     if (m->charges) {
         HYMD_TRY(hymd_update_field_force_q(m->pm, m->x32, m->charges, m->n, m->sigma, m->elec, m->e_self,
-                                           (float *)m->f_elec.ptr, nullptr, nullptr, nullptr,
+                                           (float *)m->f_elec.ptr, m->phi_q, m->psi, nullptr,
                                            energy ? &m->extra_e[1] : nullptr));
         HYMD_TRY(force_combine(m->f_slow, (const float *)m->f_elec.ptr, nullptr, n3, 1, s));
     }
@@ -224,7 +228,7 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
         }
         mark(m, 3);
         HYMD_TRY(update_field_keyed(m->pm, m->x32, m->types, m->n, m->T, m->A, m->b, m->c0, m->mass_t, m->sigma,
-                                    m->force_mesh, energies ? &energies[0] : nullptr));
+                                    m->force_mesh, energies ? &energies[0] : nullptr, m->spectra));
         mark(m, 3);
         mark(m, 4);
         HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));
@@ -266,7 +270,7 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
     }
     mark(m, 3);
     HYMD_TRY(hymd_update_field(m->pm, m->x32, m->types, m->n, m->T, m->A, m->b, m->c0, m->mass_t, m->sigma,
-                               m->force_mesh, nullptr, nullptr, nullptr, energies ? &energies[0] : nullptr));
+                               m->force_mesh, nullptr, m->spectra, nullptr, energies ? &energies[0] : nullptr));
     mark(m, 3);
     mark(m, 4);
     HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));
@@ -278,6 +282,68 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
     }
     HYMD_TRY(hymd_respa_kick(m->v, m->n, m->f_fast, half_in, f_slow, half_out, m->work));
     return HYMD_OK;
+}
+
+// The two halves of an outer step, eager, for a coupling between them (the
+// barostat of main.py:889-932 acts after the inner loop, before the slow
+// forces).  part 0: the pending outer half kick, the first outer half kick and
+// the inner loop (the per-kernel path also applies the last inner half kick
+// here); virials (host double[6] or NULL): bond and angle virials of the last
+// inner step.  part 1: the slow forces at the current positions (no row keys
+// from the inner loop: the barostat may have moved the positions), with the
+// filtered spectra when hymd_md_set_spectra asked for them, then the second
+// outer half kick (left pending on the fused path).  energies as enqueue_step.
+static int enqueue_half(hymd_md *m, int part, double *virials, double *energies) {
+    const double half_in = 0.5 * m->dt / m->mass;
+    const double half_out = 0.5 * m->respa_inner * m->dt / m->mass;
+    const bool extra = m->charges || m->dih;
+    float *f_slow = extra ? m->f_slow : m->f_field;
+    const int64_t n3 = 3 * m->n;
+    HYMD_TRY(hymd_pm_set_stream(m->pm, m->work));
+    if (part == 0) {
+        if (m->fused) {
+            double eb[2];
+            HYMD_TRY(fused_inner_impl(m->fused, m->x, m->v, m->x32, m->f_fast, f_slow, half_in, half_out, m->dt,
+                                      m->respa_inner, m->pending, m->box, energies ? eb : nullptr, virials, m->work,
+                                      nullptr));
+            m->pending = 0;
+            if (energies) {
+                energies[1] = eb[0];
+                energies[2] = eb[1];
+            }
+            return HYMD_OK;
+        }
+        for (int i = 0; i < m->respa_inner; ++i) {
+            const bool first = (i == 0), last_inner = (i == m->respa_inner - 1);
+            HYMD_TRY(hymd_respa_kick_drift(m->x, m->v, m->x32, m->n, m->f_fast, half_in, first ? f_slow : m->f_fast,
+                                           first ? half_out : half_in, m->dt, m->box, m->work));
+            const bool last_e = (energies || virials) && last_inner;
+            double eb[2];
+            HYMD_TRY(hymd_topology_forces(m->topo, m->x, m->dih ? (float *)m->f_ba.ptr : m->f_fast, m->box,
+                                          last_e ? eb : nullptr, last_inner ? virials : nullptr, m->work));
+            if (m->dih) {
+                HYMD_TRY(hymd_dihedrals_forces(m->dih, m->x, (double *)m->f_dih.ptr, (double *)m->dipoles.ptr,
+                                               (double *)m->tmat.ptr, m->box, (last_inner && m->protein) ? 1 : 0,
+                                               (energies && last_inner) ? &m->extra_e[0] : nullptr, m->work));
+                HYMD_TRY(force_combine(m->f_fast, (const float *)m->f_ba.ptr, (const double *)m->f_dih.ptr, n3, 0,
+                                       m->work));
+            }
+            if (energies && last_inner) {
+                energies[1] = eb[0];
+                energies[2] = eb[1];
+            }
+        }
+        return hymd_respa_kick(m->v, m->n, m->f_fast, half_in, nullptr, 0.0, m->work);
+    }
+    HYMD_TRY(hymd_update_field(m->pm, m->x32, m->types, m->n, m->T, m->A, m->b, m->c0, m->mass_t, m->sigma,
+                               m->force_mesh, nullptr, m->spectra, nullptr, energies ? &energies[0] : nullptr));
+    HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));
+    if (extra) HYMD_TRY(peptide_slow_forces(m, energies != nullptr));
+    if (m->fused) {
+        m->pending = 1;
+        return HYMD_OK;
+    }
+    return hymd_respa_kick(m->v, m->n, f_slow, half_out, nullptr, 0.0, m->work);
 }
 
 extern "C" int hymd_md_create(hymd_md **out, hymd_pm *pm, hymd_topology *topo, int64_t n, int32_t n_types,
@@ -474,6 +540,42 @@ extern "C" int hymd_md_step(hymd_md *m, int32_t n_steps, int32_t use_graph, doub
     HYMD_TRY(finish_pending(m));
     HYMD_HIP(hipEventRecord(m->ev_out, m->work));
     HYMD_HIP(hipStreamWaitEvent(caller, m->ev_out, 0));
+    return HYMD_OK;
+}
+
+extern "C" int hymd_md_step_half(hymd_md *m, int32_t part, double *virials, double *energies, void *stream) {
+    HYMD_REQUIRE(m && (part == 0 || part == 1), "bad argument");
+    HYMD_REQUIRE(part == 0 || !virials, "virials come with part 0");
+    if (part == 1 && m->n_b > 0) {
+        const int G = m->thermo.gp.n_groups;
+        const int64_t have = (int64_t)(m->draws.size() / (2 * G)) - (int64_t)m->draw_next;
+        const int64_t need = (m->step + 1) % m->n_b == 0 ? 1 : 0;
+        HYMD_REQUIRE(need <= have, "the thermostat needs a draw set for this step, none was pushed");
+    }
+    hipStream_t caller = (hipStream_t)stream;
+    HYMD_HIP(hipEventRecord(m->ev_in, caller));
+    HYMD_HIP(hipStreamWaitEvent(m->work, m->ev_in, 0));
+    check_pm(m);  // a barostat between the halves moves the pm's box
+    if (energies) std::memset(energies, 0, 5 * sizeof(double));
+    HYMD_TRY(enqueue_half(m, part, virials, energies));
+    if (part == 1) {  // velocities are complete when part 1 returns, as after hymd_md_step
+        HYMD_TRY(couple_step(m));
+        HYMD_TRY(finish_pending(m));
+        if (energies) HYMD_TRY(hymd_kinetic_energy(m->v, m->n, HYMD_F64, m->mass, &energies[3], m->work));
+    }
+    HYMD_HIP(hipEventRecord(m->ev_out, m->work));
+    HYMD_HIP(hipStreamWaitEvent(caller, m->ev_out, 0));
+    return HYMD_OK;
+}
+
+extern "C" int hymd_md_set_spectra(hymd_md *m, void *phi_fourier, float *psi, float *phi_q) {
+    HYMD_REQUIRE(m, "null handle");
+    HYMD_REQUIRE(!phi_q || psi, "phi_q comes with psi");
+    HYMD_HIP(hipStreamSynchronize(m->work));
+    m->spectra = phi_fourier;
+    m->psi = psi;
+    m->phi_q = phi_q;
+    drop_graphs(m);
     return HYMD_OK;
 }
 

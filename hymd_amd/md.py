@@ -240,11 +240,8 @@ class RespaMD:
                  float(config.gas_constant), float(config.target_temperature), c, 1 if th.remove_com else 0, n_b,
                  dptr(th.work), dptr(th._ws), th._ws.numel() * 8)
             calls = (step0 + n_steps) // n_b - step0 // n_b
-            if calls:
-                draws = [th.draw(prng) for _ in range(calls)]
-                R = np.ascontiguousarray([d[0] for d in draws], dtype=np.float64)
-                S = np.ascontiguousarray([d[1] for d in draws], dtype=np.float64)
-                call("hymd_md_push_draws", h, R.ctypes.data_as(_lib._pd), S.ctypes.data_as(_lib._pd), calls)
+            if calls and not config.barostat:
+                self._push_draws(h, th, prng, calls)
         else:
             call("hymd_md_set_thermostat", h, self.T, None, 0, None, 0.0, 0.0, 0.0, 0, 0, None, None, 0)
         if n_com:
@@ -256,11 +253,84 @@ class RespaMD:
             call("hymd_md_set_com_removal", h, 0, 0, None, 0)
         call("hymd_md_set_step", h, step0)
         try:
-            self.run(n_steps, use_graph=use_graph)
+            if config.barostat:
+                self._npt_steps(h, n_steps, config, prng, step0)
+            else:
+                self.run(n_steps, use_graph=use_graph)
         finally:   # plain run() calls stay uncoupled
             call("hymd_md_set_thermostat", h, self.T, None, 0, None, 0.0, 0.0, 0.0, 0, 0, None, None, 0)
             call("hymd_md_set_com_removal", h, 0, 0, None, 0)
         return step0 + n_steps
+
+    @staticmethod
+    def _push_draws(h, th, prng, calls):
+        draws = [th.draw(prng) for _ in range(calls)]
+        R = np.ascontiguousarray([d[0] for d in draws], dtype=np.float64)
+        S = np.ascontiguousarray([d[1] for d in draws], dtype=np.float64)
+        call("hymd_md_push_draws", h, R.ctypes.data_as(_lib._pd), S.ctypes.data_as(_lib._pd), calls)
+
+    def _npt_steps(self, h, n_steps, config, prng, step0):
+        """Outer steps with the barostat of main.py:889-932 between the inner
+        loop and the slow forces: each step is the library's two halves; in
+        between, on every n_b-th step, the barostat (hymd_amd.barostat) takes
+        the pressure from the velocities after the inner loop, the bonded
+        virials of the last inner step and the filtered spectra of the last
+        field update, scales the positions (FP64 state and float32 mirror) and
+        config.box_size, and moves the mesh to the new box; the step's field
+        update then runs in that box.  Random numbers are taken from ``prng``
+        in the reference's order: the barostat's (SCR) within the step, then
+        the thermostat's.  The semi-isotropic coupling scales the x, y columns
+        and the z column (``rows_quirk=False``: the state's rows are in the
+        spatial order, so the reference's rows 0-2 would be other particles).
+        With charges, ``config.type_charges`` gives the electrostatic
+        pressure terms (the reference's per-type charges)."""
+        from .barostat import isotropic, semiisotropic
+        from .hamiltonian import Hamiltonian
+        kind = config.barostat.lower()
+        if kind not in ("isotropic", "semiisotropic"):
+            raise ValueError(f"barostat {config.barostat!r} is not isotropic or semiisotropic")
+        baro = isotropic if kind == "isotropic" else semiisotropic
+        if getattr(self, "_phi_fourier", None) is None:
+            self._phi_fourier = self.pm.create("complex", batch=self.T)
+            self._psi = self.pm.create("real") if self.charges is not None else None
+            self._phi_q = self.pm.create("real") if self.charges is not None else None
+            call("hymd_md_set_spectra", h, dptr(self._phi_fourier), dptr(self._psi), dptr(self._phi_q))
+        # the spectra of the current positions, as the last field update left them
+        fm = self.pm.create("real", batch=(self.T, 3))
+        self.pm.field_update(self.x32, self.types, self.T, self.A, self.b, self.c0, self.type_mass, self.sigma, fm,
+                             phi_fourier=self._phi_fourier)
+        del fm
+        if self.charges is not None:
+            f = torch.empty_like(self.f_field)
+            self.pm.pme(self.x32, self.charges, self.sigma, self.elec, self.e_self, f, phi_q=self._phi_q, psi=self._psi)
+
+        class _Form(Hamiltonian):
+            def __init__(self, A, b, c0):
+                self._q = (A, b, c0)
+
+            def quadratic(self):
+                return self._q
+
+        ham = _Form(self.A, self.b, self.c0)
+        n_b = int(config.n_b or 1)
+        config.box_size = np.asarray(self.box, dtype=np.float64).copy()
+        if self.charges is not None:
+            if config.type_charges is None:
+                raise ValueError("the pressure of a charged system needs config.type_charges")
+            if not config.coulombtype:
+                config.coulombtype = "PIC_Spectral"
+        stream = _lib.stream_ptr()
+        vir = (ctypes.c_double * 6)()
+        for step in range(step0 + 1, step0 + n_steps + 1):
+            call("hymd_md_step_half", h, 0, vir, None, stream)
+            kw = {} if kind == "isotropic" else {"rows_quirk": False}
+            baro(None, self.pm, None, self._phi_q, self._psi, ham, self.x, self.v, config, self._phi_fourier, None,
+                 None, np.array(vir[:3]), np.array(vir[3:6]), step, prng, x32=self.x32, **kw)
+            self.box = np.asarray(config.box_size, dtype=np.float64).copy()
+            self._box_c = darr(self.box, 3)
+            if config.target_temperature and step % n_b == 0:
+                self._push_draws(h, self._thermostat, prng, 1)
+            call("hymd_md_step_half", h, 1, None, None, stream)
 
     @property
     def thermostat_work(self):

@@ -486,6 +486,20 @@ int hymd_md_peptide_state(hymd_md *md, double *dipoles, double *tmat, double *ex
 int hymd_md_destroy(hymd_md *md);
 /* use the fused inner loop (1) or the per-kernel inner loop (0); returns
  * HYMD_ERR_UNSUPPORTED if the topology does not allow fusing */
+/* One half of an outer step, launched eagerly, for a coupling between the
+ * halves (the barostat, main.py:889-932): part 0 = the pending and the first
+ * outer half kick and the inner loop (velocities complete through the last
+ * inner half kick); virials host double[6] or NULL = bond then angle virials
+ * (bond_pr, angle_pr) of the last inner step.  part 1 = the slow forces at
+ * the current positions in the pm's current box, the second outer half kick,
+ * then the thermostat / COM coupling of hymd_md_step (one outer step
+ * counted); velocities are complete when it returns.  energies as hymd_md_step (part 0: bond, angle; part 1: field,
+ * kinetic).  Synchronises only for host outputs. */
+int hymd_md_step_half(hymd_md *md, int32_t part, double *virials, double *energies, void *stream);
+/* Field updates of the step write the filtered spectra phi_fourier (device
+ * complex64 (n_types,Mx,My,Mz/2+1), the pmesh normalisation, or NULL) and the
+ * PME psi, phi_q grids (or NULL): the inputs of comp_pressure. */
+int hymd_md_set_spectra(hymd_md *md, void *phi_fourier, float *psi, float *phi_q);
 int hymd_md_set_fused(hymd_md *md, int32_t enable);
 /* Coupling after outer steps, as main.py:1314-1321: after outer step s
  * (counted by the handle; hymd_md_set_step sets the count of steps done) the

@@ -117,3 +117,71 @@ def test_field_after_box_move_matches_oracle(hip, geom):
         assert rms_rel(fl[0][t].cpu().numpy(), ref["phi"][t]) < 1e-6
         for d in range(3):
             assert rms_rel(fl[2][t][d].cpu().numpy(), ref["force_mesh"][t][d]) < 1e-5
+
+
+@pytest.mark.parametrize("kind", ["isotropic", "semiisotropic"])
+def test_npt_steps_match_the_oracle(hip, oracle_built, kind):
+    """RespaMD.simulate with a Berendsen barostat every outer step, called
+    between the inner loop and the field update as main.py:889-932, against
+    the same steps composed from the FP64 oracle: kicks and inner loop, the
+    pressure from the velocities after the inner loop, the last inner step's
+    bonded virials and the previous field update's spectra, the barostat's
+    box and position scaling, the field update in the new box, the second
+    kick.  Box to 1e-6 relative, positions to 1e-5 nm, velocities to 1e-4."""
+    from hymd_amd import synth
+    from hymd_amd.config import Config, Target_pressure
+    from hymd_amd.md import RespaMD
+    from oracle import bonded as OBd
+    from oracle import field as F
+    from oracle import md as OM
+    from oracle import pressure as OP
+    s = synth.diblock_melt(n_chains=300, box=6.5, mesh=24, seed=1000)
+    s.respa_inner = 3
+    A, b, c0 = synth.hamiltonian_form(s)
+    ham = F.Quadratic(A, b, c0)
+    T, m, n_in, dt = s.n_types, s.mass, s.respa_inner, s.time_step
+    # target below the current pressure: the box grows by about 0.5 % over the three steps
+    st = OM.State(s, ham)
+    out = F.update_field(st.x, s.types, T, s.mesh, s.box, s.sigma, ham)
+    _, _, bp = OBd.cbf(st.x, s.box, *s.bonds)
+    _, _, ap = OBd.caf(st.x, s.box, *s.angles)
+    p0 = OP.comp_pressure(out["phi"], out["phi_fourier"], ham, st.v, m, s.mesh, s.box, s.sigma, bp, ap)
+    P_t = float(np.mean(p0[-3:])) * OB.BAR * 0.7
+    cfg = Config(n_steps=3, time_step=dt, mesh_size=list(s.mesh), sigma=s.sigma, kappa=s.kappa, mass=m,
+                 respa_inner=n_in, n_types=T, n_particles=s.n, box_size=s.box.copy(), barostat=kind,
+                 barostat_type="berendsen", tau_p=1.0, n_b=1, target_pressure=Target_pressure(P_L=P_t, P_N=P_t))
+    md = RespaMD(s)
+    md.simulate(3, cfg, np.random.default_rng(0))
+    torch.cuda.synchronize()
+    # the oracle's steps
+    box = s.box.copy()
+    last = out
+    for _ in range(3):
+        for f in st.f_slow:
+            st.v = OM.integrate_velocity(st.v, f / m, n_in * dt)
+        for i in range(n_in):
+            st.v = OM.integrate_velocity(st.v, st.f_fast / m, dt)
+            st.x = np.mod(OM.integrate_position(st.x, st.v, dt), box[None, :])
+            fb, _, bp = OBd.cbf(st.x, box, *s.bonds)
+            fa, _, ap = OBd.caf(st.x, box, *s.angles)
+            st.f_fast = fb + fa
+            st.v = OM.integrate_velocity(st.v, st.f_fast / m, dt)
+        p = OP.comp_pressure(last["phi"], last["phi_fourier"], ham, st.v, m, s.mesh, box, s.sigma, bp, ap)
+        if kind == "isotropic":
+            OB.berendsen_isotropic(p, st.x, box, n_in * dt, 1, 1.0, P_t)
+        else:
+            OB.berendsen_semiisotropic(p, st.x, box, n_in * dt, 1, 1.0, P_t, P_t, rows_quirk=False)
+        last = F.update_field(st.x, s.types, T, s.mesh, box, s.sigma, ham)
+        f_field = F.compute_field_force(st.x, s.types, last["force_mesh"], s.mesh, box)
+        st.v = OM.integrate_velocity(st.v, f_field / m, n_in * dt)
+        st.f_slow = [f_field]
+    assert np.max(np.abs(box / s.box - 1.0)) > 2e-3          # the barostat moved the box
+    np.testing.assert_allclose(md.box, box, rtol=1e-6)
+    np.testing.assert_allclose(cfg.box_size, box, rtol=1e-6)
+    np.testing.assert_allclose(md.pm.BoxSize, box, rtol=1e-6)
+    io = md.in_input_order
+    dx = io(md.x).cpu().numpy() - st.x
+    dx -= box * np.round(dx / box)
+    assert np.abs(dx).max() < 1e-5
+    v = io(md.v).cpu().numpy()
+    assert np.abs(v - st.v).max() < 1e-4 * np.abs(st.v).max()
