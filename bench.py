@@ -501,6 +501,8 @@ def main():
                     help="device prepare_bonds builds of the bench system (0 = skip)")
     ap.add_argument("--large-steps", type=int, default=200,
                     help="outer steps of the 16,777,216-bead / 256^3 Markov melt with PME (0 = skip)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes (profiling runs; the bench line is the tuned step)")
     ap.add_argument("--sweep-points", type=int, default=32,
                     help="chi-sweep points on the large system's frozen final positions")
     args = ap.parse_args()
@@ -555,19 +557,25 @@ def main():
     # the default launch shapes, then the autotuner's choice for this
     # workload (the headline is the tuned step, the library's normal set-up)
     default_elapsed = timed_steps()
-    from hymd_amd import tune
-    tuning = tune.autotune(md.pm, md.x32, md.types, md.T, md.A, md.b, md.c0, md.sigma, mass=md.type_mass,
-                           use_cache=False)
-    md.run(args.warmup, use_graph=not args.no_graph)   # captures the step again with the tuned shapes
-    torch.cuda.synchronize()
-    elapsed = timed_steps()
+    autotune = None
+    if args.no_autotune:
+        elapsed = default_elapsed
+    else:
+        from hymd_amd import tune
+        tuning = tune.autotune(md.pm, md.x32, md.types, md.T, md.A, md.b, md.c0, md.sigma, mass=md.type_mass,
+                               use_cache=False)
+        md.run(args.warmup, use_graph=not args.no_graph)   # captures the step again with the tuned shapes
+        torch.cuda.synchronize()
+        elapsed = timed_steps()
+        autotune = {"choice": tuning["choice"], "default_ms_per_step": round(1e3 * default_elapsed / args.steps, 4),
+                    "tuned_ms_per_step": round(1e3 * elapsed / args.steps, 4),
+                    "update_readout_default_ms": tuning["default_ms"], "update_readout_tuned_ms": tuning["tuned_ms"],
+                    "trials_ms": tuning["trials"]}
     ms_per_step = 1e3 * elapsed / args.steps
     value = s.n * args.steps / elapsed
-    autotune = {"choice": tuning["choice"], "default_ms_per_step": round(1e3 * default_elapsed / args.steps, 4),
-                "tuned_ms_per_step": round(ms_per_step, 4), "update_readout_default_ms": tuning["default_ms"],
-                "update_readout_tuned_ms": tuning["tuned_ms"], "trials_ms": tuning["trials"]}
-    log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step tuned ({autotune['default_ms_per_step']:.4f} default), "
-        f"{value:.4g} particle-steps/s")
+    log(f"timed {args.steps} steps: {ms_per_step:.4f} ms/step"
+        + ("" if autotune is None else f" tuned ({autotune['default_ms_per_step']:.4f} default)")
+        + f", {value:.4g} particle-steps/s")
 
     # roofline: the same outer steps launched eagerly by the library with HIP
     # events recorded on the launch stream around every launch of one stage

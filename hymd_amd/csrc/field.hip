@@ -1007,41 +1007,75 @@ static int launch_xline(hymd_pm *pm, const float2 *spec, float2 *grad, const KGe
 
 constexpr int kXlineGradThreads = 256;
 constexpr int kXlineGradElems = 16;  // values per thread and in-place stage
+// Workgroups of 512 or 1024 threads (one per CU at up to 150 KB of LDS) for
+// long x lines with many types, where 256 threads hold only 2 lines: a
+// workgroup's lines are neighbouring (y, k_z) columns, so W lines give W * 8
+// contiguous bytes per x row, and at 16 bytes the partial HBM bursts double
+// the traffic of the pass.
+constexpr size_t kXlineWideLdsBytes = 150 * 1024;
 
-// Lines per workgroup of field_xline_grad_kernel for T types (0: it does not
-// fit): `want` when it fits, else the widest that does.
-static bool xline_grad_fits(const FftPlan &f, int T, int W) {
+struct XlineShape {
+    int W = 0, NT = kXlineGradThreads;
+};
+
+// Lines per workgroup W of field_xline_grad_kernel with NT threads for T
+// types: the registers hold PMAX positions per thread (W n <= NT PMAX), the
+// in-place stages kXlineGradElems values per thread, LDS the T W lines.
+static bool xline_grad_fits(const FftPlan &f, int T, int W, int NT) {
     const int pmax = 16 / moments_maxt(T);
-    if ((int64_t)W * f.ax.n > (int64_t)kXlineGradThreads * pmax) return false;
-    if (sizeof(float2) * ((size_t)f.ax.n + (size_t)T * W * f.ax.ld) > kXlineLdsBytes) return false;
-    return fft_inplace_fits(f.ax, T * W, kXlineGradThreads, kXlineGradElems);
+    if ((int64_t)W * f.ax.n > (int64_t)NT * pmax) return false;
+    const size_t lim = NT == kXlineGradThreads ? kXlineLdsBytes : kXlineWideLdsBytes;
+    if (sizeof(float2) * ((size_t)f.ax.n + (size_t)T * W * f.ax.ld) > lim) return false;
+    return fft_inplace_fits(f.ax, T * W, NT, kXlineGradElems);
 }
 
-static int xline_grad_width(const FftPlan &f, int T, int want = 0) {
-    if (want > 0 && xline_grad_fits(f, T, want)) return want;
-    for (int W = 16; W >= 2; W >>= 1)
-        if (xline_grad_fits(f, T, W)) return W;
-    return 0;
+// The x-line shape: `want` lines with the fewest threads that fit it, else
+// the widest W that fits 256 threads when that is at least 4 lines, else 8
+// or 16 lines with 512 or 1024 threads, else 2 lines (W = 0: the fused pass
+// does not fit).
+static XlineShape xline_grad_shape(const FftPlan &f, int T, int want = 0) {
+    const int nts[3] = {kXlineGradThreads, 2 * kXlineGradThreads, 4 * kXlineGradThreads};
+    if (want > 0)
+        for (int NT : nts)
+            if (xline_grad_fits(f, T, want, NT)) return XlineShape{want, NT};
+    for (int W = 16; W >= 4; W >>= 1)
+        if (xline_grad_fits(f, T, W, kXlineGradThreads)) return XlineShape{W, kXlineGradThreads};
+    for (int W = 16; W >= 8; W >>= 1)
+        for (int NT : {2 * kXlineGradThreads, 4 * kXlineGradThreads})
+            if (xline_grad_fits(f, T, W, NT)) return XlineShape{W, NT};
+    if (xline_grad_fits(f, T, 2, kXlineGradThreads)) return XlineShape{2, kXlineGradThreads};
+    return XlineShape{};
 }
 
-template <int MAXT, class G>
+template <int MAXT, int NT, class G>
 static int launch_xline_grad_g(hymd_pm *pm, const float2 *spec, float2 *out, const KGeom &g, const FieldK &p, int W,
                                const G &geo, double *partials, int *nblocks) {
     const FftPlan &f = pm->fft;
     const int plane = g.my * g.zh;
     const int nb = (plane + W - 1) / W;
     const size_t lds = sizeof(float2) * ((size_t)f.ax.n + (size_t)p.T * W * f.ax.ld);
+    const size_t lim = NT == kXlineGradThreads ? kXlineLdsBytes : kXlineWideLdsBytes;
     static bool attr_set = false;  // the opt-in is per kernel and process
     if (!attr_set && lds > 64 * 1024) {
-        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&field_xline_grad_kernel<MAXT, kXlineGradThreads, G>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kXlineLdsBytes));
+        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&field_xline_grad_kernel<MAXT, NT, G>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim));
         attr_set = true;
     }
-    field_xline_grad_kernel<MAXT, kXlineGradThreads, G><<<nb, kXlineGradThreads, lds, pm->stream>>>(
-        spec, out, g, p, geo, (const float2 *)f.tw_x.ptr, partials);
+    field_xline_grad_kernel<MAXT, NT, G><<<nb, NT, lds, pm->stream>>>(spec, out, g, p, geo,
+                                                                      (const float2 *)f.tw_x.ptr, partials);
     HYMD_HIP(hipGetLastError());
     *nblocks = nb;
     return HYMD_OK;
+}
+
+template <int MAXT, class G>
+static int launch_xline_grad_nt(hymd_pm *pm, const float2 *spec, float2 *out, const KGeom &g, const FieldK &p,
+                                const XlineShape &sh, const G &geo, double *partials, int *nblocks) {
+    if (sh.NT == 4 * kXlineGradThreads)
+        return launch_xline_grad_g<MAXT, 4 * kXlineGradThreads>(pm, spec, out, g, p, sh.W, geo, partials, nblocks);
+    if (sh.NT == 2 * kXlineGradThreads)
+        return launch_xline_grad_g<MAXT, 2 * kXlineGradThreads>(pm, spec, out, g, p, sh.W, geo, partials, nblocks);
+    return launch_xline_grad_g<MAXT, kXlineGradThreads>(pm, spec, out, g, p, sh.W, geo, partials, nblocks);
 }
 
 template <int N>
@@ -1053,16 +1087,17 @@ static bool xline_ct_plan(const FftPlan &f) {
     return true;
 }
 
-// The x-line pass for T types with W lines per workgroup: a compile-time
-// geometry for Mx = 128 or 160 at the W that xline_grad_width picks there,
-// the runtime one otherwise.
-static int launch_xline_grad(hymd_pm *pm, const float2 *spec, float2 *out, const KGeom &g, const FieldK &p, int W,
-                             double *partials, int *nblocks) {
+// The x-line pass for T types in the shape sh: a compile-time geometry for
+// Mx = 128 or 160 at the 256-thread W that xline_grad_shape picks there, the
+// runtime one otherwise.
+static int launch_xline_grad(hymd_pm *pm, const float2 *spec, float2 *out, const KGeom &g, const FieldK &p,
+                             const XlineShape &sh, double *partials, int *nblocks) {
     const FftPlan &f = pm->fft;
-    const int T = p.T;
+    const int T = p.T, W = sh.W;
 #define HYMD_XLINE_CT(NX, TT, WW, MT)                                                                      \
-    if (f.ax.n == NX && T == TT && W == WW && xline_ct_plan<NX>(f))                                       \
-        return launch_xline_grad_g<MT>(pm, spec, out, g, p, W, XlineCt<TT, NX, WW>{}, partials, nblocks);
+    if (sh.NT == kXlineGradThreads && f.ax.n == NX && T == TT && W == WW && xline_ct_plan<NX>(f))          \
+        return launch_xline_grad_g<MT, kXlineGradThreads>(pm, spec, out, g, p, W, XlineCt<TT, NX, WW>{}, partials, \
+                                                          nblocks);
     HYMD_XLINE_CT(128, 1, 16, 2)
     HYMD_XLINE_CT(128, 2, 16, 2)
     HYMD_XLINE_CT(128, 3, 8, 4)
@@ -1075,9 +1110,9 @@ static int launch_xline_grad(hymd_pm *pm, const float2 *spec, float2 *out, const
     int wshift = 0;
     while ((1 << wshift) < W) ++wshift;
     const XlineRt<kXlineGradElems> geo{f.ax.n, W, wshift, T, f.ax.ld, f.ax.ndiv, make_fastdiv((uint32_t)(T * W)), f.ax};
-    if (T <= 2) return launch_xline_grad_g<2>(pm, spec, out, g, p, W, geo, partials, nblocks);
-    if (T <= 4) return launch_xline_grad_g<4>(pm, spec, out, g, p, W, geo, partials, nblocks);
-    return launch_xline_grad_g<8>(pm, spec, out, g, p, W, geo, partials, nblocks);
+    if (T <= 2) return launch_xline_grad_nt<2>(pm, spec, out, g, p, sh, geo, partials, nblocks);
+    if (T <= 4) return launch_xline_grad_nt<4>(pm, spec, out, g, p, sh, geo, partials, nblocks);
+    return launch_xline_grad_nt<8>(pm, spec, out, g, p, sh, geo, partials, nblocks);
 }
 
 // LDS budget of one row-paint workgroup (int64 accumulators of tc types x Mz)
@@ -1416,15 +1451,15 @@ static int update_field_impl(hymd_pm *pm, const float *x, const int32_t *types, 
     FieldK p = field_params(T, coupling, shift, constant, sigma, m);
     p.want_energy = energy ? 1 : 0;
     KGeom g0 = kgeom(pm);
-    const int W2 = T <= 8 ? xline_grad_width(pm->fft, T, pm->xline_w) : 0;
-    if (!phi && !phi_fourier && !v_ext && W2 > 0) {
+    const XlineShape sh = T <= 8 ? xline_grad_shape(pm->fft, T, pm->xline_w) : XlineShape{};
+    if (!phi && !phi_fourier && !v_ext && sh.W > 0) {
         // force grids only: z + y forward passes; one pass for the forward x
         // transform, the k-space pass and the 2T inverse x transforms; the
         // y-z plane pass writes the 3T force grids
         HYMD_TRY(fft_r2c_zy(pm->fft, rho, spec, T, s));
         double *part = energy ? (double *)pm->partials.ptr : nullptr;
         int nb = 0;
-        HYMD_TRY(launch_xline_grad(pm, spec, grad, g0, p, W2, part, &nb));
+        HYMD_TRY(launch_xline_grad(pm, spec, grad, g0, p, sh, part, &nb));
         if (pm->fft.lds_plane) {
             HYMD_TRY(fft_c2r_yz_grad(pm->fft, grad, force_mesh, T, g0.kfac[1], g0.kfac[2], 1.f, s));
         } else {  // y-z planes too large for LDS: the y and z line passes

@@ -232,7 +232,9 @@ def test_drop_in_field_surface(hip):
                                     ((128, 96, 80), 6), ((64, 64, 64), 1), ((40, 30, 18), 8),
                                     # compile-time x-line (Mx 128, 160) and plane (128^2, 160^2) geometries
                                     ((128, 40, 24), 4), ((128, 24, 20), 2), ((160, 20, 16), 3), ((160, 16, 24), 1),
-                                    ((16, 128, 128), 4), ((8, 160, 160), 2)])
+                                    ((16, 128, 128), 4), ((8, 160, 160), 2),
+                                    # long x lines with many types: 8 lines per workgroup of 1024 threads
+                                    ((192, 16, 16), 6), ((256, 12, 10), 5), ((512, 8, 8), 4), ((160, 24, 20), 7)])
 def test_fused_xline_pass_equals_unfused(hip, mesh, T):
     """The force-grid-only update (forward x, k-space and inverse x passes in
     one kernel, -i k_y and -i k_z applied in the y-z plane pass) against the
@@ -288,7 +290,8 @@ def test_readout_matches_oracle(hip, mesh, T):
     np.testing.assert_allclose(got, ref, rtol=0, atol=2e-6 * np.abs(ref).max())
 
 
-@pytest.mark.parametrize("mesh,T", [((32, 32, 32), 2), ((24, 20, 30), 5), ((128, 128, 128), 4)])
+@pytest.mark.parametrize("mesh,T", [((32, 32, 32), 2), ((24, 20, 30), 5), ((128, 128, 128), 4),
+                                    ((128, 24, 24), 6)])   # x-line widths on 256, 512 and 1024 threads
 def test_tuning_knobs_keep_the_bits(hip, mesh, T, tmp_path):
     """The paint and readout launch shapes the autotuner may choose give the
     same density, force grids, forces and energy bit for bit (integer paint
@@ -298,7 +301,7 @@ def test_tuning_knobs_keep_the_bits(hip, mesh, T, tmp_path):
     from hymd_amd import tune
     rng = np.random.default_rng(sum(mesh) + T)
     box = np.array(mesh, dtype=np.float64) * 0.31
-    n = 30_000 if mesh[0] < 100 else 400_000
+    n = 30_000 if mesh[0] * mesh[1] < 10_000 else 400_000
     x = (rng.uniform(0, 1, (n, 3)) * box).astype(np.float32)
     types = rng.integers(0, T, n).astype(np.int32)
     A = rng.uniform(0.0, 1.0, (T, T))
