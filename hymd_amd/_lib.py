@@ -55,6 +55,7 @@ SIGNATURES = {
     "hymd_dihedrals_destroy": [_p],
     "hymd_dihedrals_forces": [_p, _p, _p, _p, _p, _pd, _i32, _pd, _p],
     "hymd_dihedrals_redistribute": [_p, _p, _p, _p, _p],
+    "hymd_dihedrals_dipoles": [_p, _p, _p, _p, _pd, _p],
     "hymd_batch_create": [ctypes.POINTER(_p), _i32, _pi32, _pd, _pi64, _i32, _pd],
     "hymd_batch_destroy": [_p],
     "hymd_batch_moments": [_p, _p, _p, _pd, _pd, _i32, _p],
@@ -93,8 +94,11 @@ SIGNATURES = {
     "hymd_md_set_reorder": [_p, _i32, _p, _p, _p],
     "hymd_md_single_tail": [_p],
     "hymd_fused_create": [ctypes.POINTER(_p), _i64, _i64, _p, _p, _p, _p, _i64, _p, _p, _p, _p, _p],
+    "hymd_fused_create_dihedrals": [ctypes.POINTER(_p), _i64, _i64, _p, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _i64,
+                                    _p, _p, _p, _p, _p, _p, _p],
     "hymd_fused_destroy": [_p],
     "hymd_fused_inner": [_p, _p, _p, _p, _p, _p, _d, _d, _d, _i32, _i32, _pd, _pd, _pd, _p],
+    "hymd_fused_inner_dihedrals": [_p, _p, _p, _p, _p, _p, _d, _d, _d, _i32, _i32, _pd, _pd, _pd, _p],
     "hymd_md_stage_time": [_p, _pd, _pi32],
     "hymd_thermostat_workspace_bytes": [],
     "hymd_csvr_thermostat": [_p, _p, _i64, _i32, _pi32, _i32, _pi64, _pd, _pd, _d, _d, _d, _d, _i32, _p, _p, _i64,

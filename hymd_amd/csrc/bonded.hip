@@ -222,98 +222,23 @@ struct DihArgs {
     int dipole_flag;
 };
 
-// One dihedral term (compute_dihedral_forces.f90:35-134): the force on each of
-// its four particles (roles 0..3 = a, b, c, d), its energy and clamp count,
-// and its dipoles / transfer matrices (zero where the term has none, as the
-// reference's zeroed outputs).
+// One dihedral term t of the caller's arrays (dihedral_core on its four
+// positions; dipoles and transfer matrices at rows 12 t and 54 t).
 template <typename PosT>
 __device__ void dihedral_term(const PosT *__restrict__ r, const DihArgs &D, int64_t t, const Box &box, double *ev,
                               PosT *__restrict__ dipoles, PosT *__restrict__ tmat, V3 (&out)[4]) {
-    const V3 ra = load3(r, D.a[t]), rb = load3(r, D.b[t]), rc = load3(r, D.c[t]), rd = load3(r, D.d[t]);
-    const V3 f = mic(ra - rb, box), g = mic(rb - rc, box), h = mic(rd - rc, box);
-    const V3 v = cross(f, g), w = cross(h, g);
-    const double v_sq = dot(v, v), w_sq = dot(w, w), g_norm = norm(g);
-    const double cos_phi = dot(v, w);
-    const double sin_phi = dot(w, f) * g_norm;
-    const double phi = atan2(sin_phi, cos_phi);
-    const double f_dot_g = dot(f, g), h_dot_g = dot(h, g);
-    const float *cf = D.coeff + 30 * t;
-    const int dt = D.dtype[t];
-    double energy = 0.0, df = 0.0;
-    for (int q = 0; q < 4; ++q) out[q] = v3(0, 0, 0);
-    if (dt == 0 || dt == 1) {
-        cosine_series(cf, cf + 5, phi, energy, df);
-        int zc = 0, zd = 0;
-        for (int i = 0; i < 5; ++i) {
-            zc += (cf[10 + i] == 0.f);
-            zd += (cf[15 + i] == 0.f);
-        }
-        if (zc != 5 && zd != 5) cosine_series(cf + 10, cf + 15, phi, energy, df);
-    }
-    bool dip0 = false, dip1 = false;
-    if (dt == 1) {
-        Recon rec;
-        reconstruct(f, rb, -g, box, cf + 20, cf + 25, phi, D.dipole_flag, rec);
-        energy += rec.energy;
-        df += rec.df;
-        out[0] = out[0] - rec.fa;
-        out[1] = out[1] - rec.fb;
-        out[2] = out[2] - rec.fc;
-        ev[1] += rec.clamped ? 1.0 : 0.0;
-        if (rec.has_dipole) {
-            dip0 = true;
-            for (int p = 0; p < 2; ++p) {
-                dipoles[12 * t + 3 * p + 0] = (PosT)rec.dip[p].x;
-                dipoles[12 * t + 3 * p + 1] = (PosT)rec.dip[p].y;
-                dipoles[12 * t + 3 * p + 2] = (PosT)rec.dip[p].z;
-            }
-            for (int x = 0; x < 3; ++x)
-                for (int i = 0; i < 3; ++i)
-                    for (int j = 0; j < 3; ++j) tmat[54 * t + 9 * x + 3 * i + j] = (PosT)rec.tm[x][i][j];
-        }
-        if (D.last[t] == 1) {
-            Recon rec2;
-            reconstruct(g, rc, h, box, cf + 20, cf + 25, phi, D.dipole_flag, rec2);
-            energy += rec2.energy;
-            df += rec2.df;
-            out[1] = out[1] - rec2.fa;
-            out[2] = out[2] - rec2.fb;
-            out[3] = out[3] - rec2.fc;
-            ev[1] += rec2.clamped ? 1.0 : 0.0;
-            if (rec2.has_dipole) {
-                dip1 = true;
-                for (int p = 0; p < 2; ++p) {
-                    dipoles[12 * t + 6 + 3 * p + 0] = (PosT)rec2.dip[p].x;
-                    dipoles[12 * t + 6 + 3 * p + 1] = (PosT)rec2.dip[p].y;
-                    dipoles[12 * t + 6 + 3 * p + 2] = (PosT)rec2.dip[p].z;
-                }
-                for (int x = 0; x < 3; ++x)
-                    for (int i = 0; i < 3; ++i)
-                        for (int j = 0; j < 3; ++j) tmat[54 * t + 27 + 9 * x + 3 * i + j] = (PosT)rec2.tm[x][i][j];
-            }
-        }
-    }
-    if (!dip0) {
-        for (int k = 0; k < 6; ++k) dipoles[12 * t + k] = (PosT)0;
-        for (int k = 0; k < 27; ++k) tmat[54 * t + k] = (PosT)0;
-    }
-    if (!dip1) {
-        for (int k = 6; k < 12; ++k) dipoles[12 * t + k] = (PosT)0;
-        for (int k = 27; k < 54; ++k) tmat[54 * t + k] = (PosT)0;
-    }
-    if (dt == 2) {
-        const double eq = (double)cf[0], kf = (double)cf[1];
-        df = kf * (phi - eq);
-        energy += 0.5 * kf * ((phi - eq) * (phi - eq));
-    }
-    const V3 sc = (f_dot_g / (v_sq * g_norm)) * v - (h_dot_g / (w_sq * g_norm)) * w;
-    const V3 fa = (-df * g_norm / v_sq) * v;
-    const V3 fd = (df * g_norm / w_sq) * w;
-    out[0] = out[0] + fa;
-    out[1] = out[1] + (df * sc - fa);
-    out[2] = out[2] + (-df * sc - fd);
-    out[3] = out[3] + fd;
-    ev[0] += energy;
+    dihedral_core(load3(r, D.a[t]), load3(r, D.b[t]), load3(r, D.c[t]), load3(r, D.d[t]), D.coeff + 30 * t,
+                  D.dtype[t], D.last[t], box, D.dipole_flag, ev, dipoles + 12 * t, tmat + 54 * t, out);
+}
+
+// One thread per dihedral: its dipoles and transfer matrices alone.
+__global__ void __launch_bounds__(kBlock) dihedral_dipole_kernel(const double *__restrict__ r, int64_t m, DihArgs D,
+                                                                  Box box, double *__restrict__ dipoles,
+                                                                  double *__restrict__ tmat) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < m; t += stride)
+        dihedral_dipoles_core(load3(r, D.a[t]), load3(r, D.b[t]), load3(r, D.c[t]), load3(r, D.d[t]), D.dtype[t],
+                              D.last[t], box, dipoles + 12 * t, tmat + 54 * t);
 }
 
 // One thread per dihedral: its four particle forces into slots (term-major,
@@ -767,6 +692,21 @@ struct hymd_dihedrals {
     int nblocks = 1;
 };
 
+namespace hymd {
+DihTerms dihedral_terms(const hymd_dihedrals *h) {
+    DihTerms v;
+    v.m = h->m;
+    v.a = h->D.a;
+    v.b = h->D.b;
+    v.c = h->D.c;
+    v.d = h->D.d;
+    v.dtype = h->D.dtype;
+    v.last = h->D.last;
+    v.coeff = h->D.coeff;
+    return v;
+}
+}  // namespace hymd
+
 extern "C" int hymd_dihedrals_create(hymd_dihedrals **out, int64_t n, int64_t m, const int32_t *a, const int32_t *b,
                                      const int32_t *c, const int32_t *d, const float *coeff, const int32_t *dtype,
                                      const int32_t *last, void *stream) {
@@ -817,6 +757,16 @@ extern "C" int hymd_dihedrals_forces(hymd_dihedrals *h, const double *x, double 
             *energy = ((const double *)h->host.ptr)[0];
         }
     }
+    return HYMD_OK;
+}
+
+extern "C" int hymd_dihedrals_dipoles(hymd_dihedrals *h, const double *x, double *dipoles, double *tmat,
+                                      const double box[3], void *stream) {
+    HYMD_REQUIRE(h && box && (h->m == 0 || (x && dipoles && tmat)), "bad argument");
+    if (h->m > 0)
+        dihedral_dipole_kernel<<<h->nblocks, kBlock, 0, (hipStream_t)stream>>>(x, h->m, h->D, make_box(box), dipoles,
+                                                                               tmat);
+    HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }
 

@@ -344,10 +344,24 @@ int update_field_keyed(hymd_pm *pm, const float *x, const int32_t *types, int64_
 int force_combine(float *out, const float *a, const double *b, int64_t n3, int mode, hipStream_t s);
 int convert_f64_f32(float *f32, double *f64, int64_t n, bool to_f32, hipStream_t s);
 
+// Dihedral outputs of a fused inner loop whose handle holds dihedral terms:
+// the dihedral energy and clamp count of the last sub-step into energy[2]
+// when energy != NULL.
+struct FusedDihOut {
+    double *energy;
+};
 // hymd_fused_inner that also produces the row keys of the final positions
+// (key) and the dihedral outputs (dih)
 int fused_inner_impl(hymd_fused *f, double *x, double *v, float *x32, float *f_fast, const float *f_slow, double h_in,
                      double h_out, double dt, int32_t n_inner, int32_t pending, const double box[3], double *energies,
-                     double *virials, void *stream, const RowKeyTarget *key);
+                     double *virials, void *stream, const RowKeyTarget *key, const FusedDihOut *dih = nullptr);
+// The term arrays of a persistent dihedral handle (device pointers).
+struct DihTerms {
+    int64_t m;
+    const int32_t *a, *b, *c, *d, *dtype, *last;
+    const float *coeff;
+};
+DihTerms dihedral_terms(const hymd_dihedrals *h);
 
 // Pinned host staging for small results.
 // Device scratch of at least n doubles for reductions that synchronise

@@ -206,6 +206,29 @@ static int peptide_slow_forces(hymd_md *m, bool energy) {
     return HYMD_OK;
 }
 
+// The fused inner loop of one outer step; with dihedrals also their energy of
+// the last sub-step (when energies) and, for a protein, the dipoles and
+// transfer matrices of the final positions (main.py:863-881 reconstructs them
+// on the last inner step; the kernel leaves them to hymd_dihedrals_dipoles).
+static int fused_inner(hymd_md *m, int pending, double *energies, double *virials, const RowKeyTarget *key) {
+    const double half_in = 0.5 * m->dt / m->mass;
+    const double half_out = 0.5 * m->respa_inner * m->dt / m->mass;
+    double eb[2], ed[2];
+    FusedDihOut dout{ed};
+    HYMD_TRY(fused_inner_impl(m->fused, m->x, m->v, m->x32, m->f_fast, slow_force(m), half_in, half_out, m->dt,
+                              m->respa_inner, pending, m->box, energies ? eb : nullptr, virials, m->work, key,
+                              (m->dih && energies) ? &dout : nullptr));
+    if (m->protein)
+        HYMD_TRY(hymd_dihedrals_dipoles(m->dih, m->x, (double *)m->dipoles.ptr, (double *)m->tmat.ptr, m->box,
+                                        m->work));
+    if (energies) {
+        energies[1] = eb[0];
+        energies[2] = eb[1];
+        if (m->dih) m->extra_e[0] = ed[0];
+    }
+    return HYMD_OK;
+}
+
 // energies (host, 5): field, bond, angle, kinetic, (unused = 0)
 static int enqueue_step(hymd_md *m, double *energies, int pending) {
     const double half_in = 0.5 * m->dt / m->mass;
@@ -214,18 +237,12 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
     if (m->fused) {
         // fused: the step ends after the slow forces; its outer half kick is
         // applied by the next step's kernel (or by finish_pending)
-        double eb[2];
         mark(m, 1);
         // the inner loop also writes the field path's row keys of the new positions
         RowKeyTarget key;
         HYMD_TRY(field_key_target(m->pm, m->n, &key));
-        HYMD_TRY(fused_inner_impl(m->fused, m->x, m->v, m->x32, m->f_fast, slow_force(m), half_in, half_out, m->dt,
-                                  m->respa_inner, pending, m->box, energies ? eb : nullptr, nullptr, m->work, &key));
+        HYMD_TRY(fused_inner(m, pending, energies, nullptr, &key));
         mark(m, 1);
-        if (energies) {
-            energies[1] = eb[0];
-            energies[2] = eb[1];
-        }
         mark(m, 3);
         HYMD_TRY(update_field_keyed(m->pm, m->x32, m->types, m->n, m->T, m->A, m->b, m->c0, m->mass_t, m->sigma,
                                     m->force_mesh, energies ? &energies[0] : nullptr, m->spectra));
@@ -302,15 +319,8 @@ static int enqueue_half(hymd_md *m, int part, double *virials, double *energies)
     HYMD_TRY(hymd_pm_set_stream(m->pm, m->work));
     if (part == 0) {
         if (m->fused) {
-            double eb[2];
-            HYMD_TRY(fused_inner_impl(m->fused, m->x, m->v, m->x32, m->f_fast, f_slow, half_in, half_out, m->dt,
-                                      m->respa_inner, m->pending, m->box, energies ? eb : nullptr, virials, m->work,
-                                      nullptr));
+            HYMD_TRY(fused_inner(m, m->pending, energies, virials, nullptr));
             m->pending = 0;
-            if (energies) {
-                energies[1] = eb[0];
-                energies[2] = eb[1];
-            }
             return HYMD_OK;
         }
         for (int i = 0; i < m->respa_inner; ++i) {
@@ -581,10 +591,6 @@ extern "C" int hymd_md_set_spectra(hymd_md *m, void *phi_fourier, float *psi, fl
 
 extern "C" int hymd_md_set_fused(hymd_md *m, int32_t enable) {
     HYMD_REQUIRE(m, "null handle");
-    if (enable && m->dih) {
-        set_error("the fused inner loop has no dihedral terms");
-        return HYMD_ERR_UNSUPPORTED;
-    }
     HYMD_HIP(hipStreamSynchronize(m->work));
     drop_graphs(m);
     if (!enable) {
@@ -595,7 +601,13 @@ extern "C" int hymd_md_set_fused(hymd_md *m, int32_t enable) {
     if (m->fused) return HYMD_OK;
     const TopoTerms t = topology_terms(m->topo);
     HYMD_REQUIRE(t.n == m->n, "topology and system sizes differ");
-    HYMD_TRY(hymd_fused_create(&m->fused, t.n, t.m2, t.a2, t.b2, t.r0, t.k2, t.m3, t.a3, t.b3, t.c3, t.t0, t.k3));
+    if (m->dih) {  // bonds, angles and the dihedrals in one block layout
+        const DihTerms d = dihedral_terms(m->dih);
+        HYMD_TRY(hymd_fused_create_dihedrals(&m->fused, t.n, t.m2, t.a2, t.b2, t.r0, t.k2, t.m3, t.a3, t.b3, t.c3,
+                                             t.t0, t.k3, d.m, d.a, d.b, d.c, d.d, d.coeff, d.dtype, d.last));
+    } else {
+        HYMD_TRY(hymd_fused_create(&m->fused, t.n, t.m2, t.a2, t.b2, t.r0, t.k2, t.m3, t.a3, t.b3, t.c3, t.t0, t.k3));
+    }
     return hymd_fused_set_precision(m->fused, m->term_precision);
 }
 
@@ -736,7 +748,10 @@ extern "C" int hymd_md_set_peptide(hymd_md *m, const float *charges, double elec
     HYMD_REQUIRE(f_slow || (!charges && !dih), "f_slow is required with charges or dihedrals");
     HYMD_REQUIRE(!dih || n_dihedrals >= 0, "bad dihedral count");
     HYMD_HIP(hipStreamSynchronize(m->work));
-    if (dih) HYMD_TRY(hymd_md_set_fused(m, 0));  // the fused inner loop has no dihedral terms
+    // a fused inner loop is rebuilt with the dihedral terms below (the
+    // per-kernel loop stays when they do not fit its blocks)
+    const bool fuse_again = dih && m->fused;
+    if (dih) HYMD_TRY(hymd_md_set_fused(m, 0));
     // the periodic reorder permutes the handle's own per-particle arrays,
     // not the caller's charges or summed slow forces
     if (charges) m->reorder_every = 0;
@@ -762,6 +777,10 @@ extern "C" int hymd_md_set_peptide(hymd_md *m, const float *charges, double elec
         HYMD_TRY(m->dip32.reserve(sizeof(float) * 3 * nd));
         HYMD_TRY(m->f_dip64.reserve(sizeof(double) * 3 * nd));
         HYMD_TRY(m->f_recon.reserve(sizeof(double) * n3));
+    }
+    if (fuse_again) {
+        const int st = hymd_md_set_fused(m, 1);
+        if (st != HYMD_OK && st != HYMD_ERR_UNSUPPORTED) return st;
     }
     return HYMD_OK;
 }

@@ -219,6 +219,13 @@ int hymd_dihedrals_forces(hymd_dihedrals *h, const double *x, double *f, double 
                           const double box[3], int32_t dipole_flag, double *energy, void *stream);
 int hymd_dihedrals_redistribute(hymd_dihedrals *h, const double *f_dipoles, const double *tmat, double *f,
                                 void *stream);
+/* The dipoles (M,4,3) and transfer matrices (M,6,3,3) alone at positions x:
+ * the dipole outputs of hymd_dihedrals_forces with dipole_flag 1
+ * (compute_dihedral_forces__plus__dipoles.f90 via dipole_reconstruction.f90
+ * :148-213), without the forces; the fused inner loop's dipoles of a protein
+ * come from here after its launch. */
+int hymd_dihedrals_dipoles(hymd_dihedrals *h, const double *x, double *dipoles, double *tmat, const double box[3],
+                           void *stream);
 
 /* Batched replicas (no reference counterpart: the reference runs one system
  * per process).  n_replicas independent systems, replica r owning particles
@@ -322,14 +329,26 @@ int hymd_topology_forces(hymd_topology *top, const double *r, float *f, const do
                          double *virials, void *stream);
 
 /* Molecule-blocked fused inner loop: all respa_inner sub-steps (kicks, drift,
- * wrap, bonds + angles) of main.py:830-887 in one launch, one workgroup per
- * block of whole molecules.  Creation fails with HYMD_ERR_UNSUPPORTED when a
- * molecule is not a contiguous index range or does not fit a workgroup.
- * The term arrays are copied at creation. */
+ * wrap, bonds + angles (+ dihedrals)) of main.py:830-887 in one launch, one
+ * workgroup per block of whole molecules.  Creation fails with
+ * HYMD_ERR_UNSUPPORTED when a molecule is not a contiguous index range or does
+ * not fit a workgroup.  The term arrays are copied at creation. */
 typedef struct hymd_fused hymd_fused;
 int hymd_fused_create(hymd_fused **fused, int64_t n, int64_t m2, const int32_t *a2, const int32_t *b2,
                       const double *r0, const double *k2, int64_t m3, const int32_t *a3, const int32_t *b3,
                       const int32_t *c3, const double *t0, const double *k3);
+/* The same with the m4 dihedral terms of hymd_cdf / hymd_dihedrals_create
+ * (indices a4..d4, coeff (m4,6,5), dtype, last) evaluated every sub-step as
+ * main.py:863-881 does: each sub-step's dihedral forces (FP64 arithmetic of
+ * hymd_dihedrals_forces, stored in the gather's precision) join the fast
+ * forces after the bonds and angles.  The dipoles of a protein are not
+ * written by the loop: hymd_dihedrals_dipoles at the final positions gives
+ * those of its last sub-step. */
+int hymd_fused_create_dihedrals(hymd_fused **fused, int64_t n, int64_t m2, const int32_t *a2, const int32_t *b2,
+                                const double *r0, const double *k2, int64_t m3, const int32_t *a3, const int32_t *b3,
+                                const int32_t *c3, const double *t0, const double *k3, int64_t m4, const int32_t *a4,
+                                const int32_t *b4, const int32_t *c4, const int32_t *d4, const float *coeff,
+                                const int32_t *dtype, const int32_t *last);
 int hymd_fused_destroy(hymd_fused *fused);
 /* v += [pending ? h_out f_slow : 0] + h_out f_slow; then n_inner times
  * { v += h_in f_fast; x += dt v; wrap; f_fast = bonds + angles; v += h_in f_fast }
@@ -345,6 +364,13 @@ int hymd_fused_set_precision(hymd_fused *fused, int32_t precision);
 int hymd_fused_inner(hymd_fused *fused, double *x, double *v, float *x32, float *f_fast, const float *f_slow,
                      double h_in, double h_out, double dt, int32_t n_inner, int32_t pending, const double box[3],
                      double *energies, double *virials, void *stream);
+/* hymd_fused_inner with energies host double[4]: bond, angle, dihedral energy
+ * and the count of clamped sin(gamma) of the last sub-step (zero without
+ * dihedral terms), or NULL. */
+int hymd_fused_inner_dihedrals(hymd_fused *fused, double *x, double *v, float *x32, float *f_fast,
+                               const float *f_slow, double h_in, double h_out, double dt, int32_t n_inner,
+                               int32_t pending, const double box[3], double *energies, double *virials,
+                               void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* rRESPA integrator (FP64 x, v): main.py:802-887,1143-1169.                 */

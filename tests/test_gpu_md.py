@@ -161,13 +161,14 @@ def test_peptide_md_matches_oracle_loop(hip, oracle_built):
 
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_peptide_native_step_equals_python_step(hip, use_graph):
-    """The library's outer step with PME, dihedrals and dipoles (eager and
-    hipGraph replay) runs the kernels of the Python step in its order:
-    identical results; the fused inner loop is refused for this system."""
-    from hymd_amd._lib import HymdError
+    """The library's outer step with PME, dihedrals and dipoles on the
+    per-kernel inner loop (eager and hipGraph replay) runs the kernels of the
+    Python step in its order: identical results."""
     from hymd_amd.md import RespaMD
     a = RespaMD(_small_peptide())
     b = RespaMD(_small_peptide())
+    assert b.fused           # dihedrals join the fused inner loop by default
+    b.set_fused(False)
     for _ in range(3):
         a.step()
     b.run(3, use_graph=use_graph)
@@ -179,8 +180,84 @@ def test_peptide_native_step_equals_python_step(hip, use_graph):
     eb = b.run(1, energies=True)
     for k in ("field", "bond", "angle", "kinetic", "dihedral", "field_q"):
         assert ea[k] == pytest.approx(eb[k], rel=1e-12), k
-    with pytest.raises(HymdError):
-        b.set_fused(True)
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_fused_peptide_loop_matches_per_kernel_path(hip, use_graph):
+    """Dihedrals and CBT in the molecule-blocked fused inner loop against the
+    per-kernel inner loop (bond + angle gather, dihedral term kernel and slot
+    gather, force sum), FP64 terms: the same kicks and per-term FP64 values;
+    the fast force is rounded to float32 once instead of as two partial sums
+    (one float32 ulp), so trajectories agree to that rounding.  The dipoles
+    of the final positions and the energies, dihedral included, agree."""
+    from hymd_amd.md import RespaMD
+    a = RespaMD(_small_peptide(), term_precision="fp64")
+    b = RespaMD(_small_peptide(), term_precision="fp64")
+    a.set_fused(False)
+    assert b.fused and not a.fused and b.protein
+    a.run(1, use_graph=use_graph)
+    b.run(1, use_graph=use_graph)
+    torch.cuda.synchronize()
+    assert rel_err(b.f_fast.cpu().numpy(), a.f_fast.cpu().numpy()) < 1e-6
+    dx = (a.x - b.x).abs().max().item()
+    dv = (a.v - b.v).abs().max().item() / a.v.abs().max().item()
+    assert dx < 1e-9 and dv < 1e-7, (dx, dv)
+    # the fused step's dipoles are those of the dihedral kernel at its final positions
+    f = torch.empty_like(b.x)
+    dip = torch.empty_like(b.dipoles)
+    tm = torch.empty_like(b.tmat)
+    b.dihedrals.forces(b.x, f, dip, tm, b.box, 1)
+    torch.cuda.synchronize()
+    assert (dip - b.dipoles).abs().max().item() < 1e-12
+    assert (tm - b.tmat).abs().max().item() < 1e-12 * tm.abs().max().item()
+    # and follow the position differences of the two paths
+    assert (a.dipoles - b.dipoles).abs().max().item() < 1e-7
+    assert (a.tmat - b.tmat).abs().max().item() < 1e-6 * a.tmat.abs().max().item()
+    a.run(2, use_graph=use_graph)
+    b.run(2, use_graph=use_graph)
+    ea = a.run(1, energies=True)
+    eb = b.run(1, energies=True)
+    torch.cuda.synchronize()
+    dx = (a.x - b.x).abs().max().item()
+    dv = (a.v - b.v).abs().max().item() / a.v.abs().max().item()
+    assert dx < 1e-7 and dv < 1e-5, (dx, dv)
+    for k in ("field", "bond", "angle", "kinetic", "dihedral", "field_q"):
+        assert ea[k] == pytest.approx(eb[k], rel=1e-7), k
+    a.close()
+    b.close()
+
+
+def test_fused_peptide_run_matches_oracle(hip, oracle_built):
+    """The library's default outer step for the peptide/solvent system (fused
+    inner loop with FP32 bond / angle terms and FP64 dihedral terms, PME of
+    the charges and of the dipole charges) against the FP64 oracle loop: two
+    outer steps of two inner steps, then a third with energies."""
+    from hymd_amd import synth
+    from hymd_amd.md import RespaMD
+    from oracle import field as F
+    from oracle import md as OM
+    s = _small_peptide()
+    A, b, c0 = synth.hamiltonian_form(s)
+    ref = OM.State(s, F.Quadratic(A, b, c0))
+    md = RespaMD(s)
+    assert md.fused and md.protein
+    io = md.in_input_order
+    md.run(2, use_graph=True)
+    for _ in range(2):
+        ref.outer_step()
+    torch.cuda.synchronize()
+    dx = io(md.x).cpu().numpy() - ref.x
+    dx -= s.box * np.round(dx / s.box)
+    assert np.abs(dx).max() < 1e-5
+    assert rel_err(io(md.v).cpu().numpy(), ref.v) < 1e-4
+    np.testing.assert_allclose(md.dipoles.cpu().numpy(), ref.dipoles, atol=1e-6)
+    e = md.run(1, energies=True)
+    ref.outer_step()
+    assert e["dihedral"] == pytest.approx(ref.e_fast["dihedral"], rel=1e-6)
+    assert e["bond"] == pytest.approx(ref.e_fast["bond"], rel=1e-6)
+    assert e["angle"] == pytest.approx(ref.e_fast["angle"], rel=1e-6)
+    assert e["kinetic"] == pytest.approx(ref.kinetic_energy(), rel=1e-6)
+    md.close()
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
