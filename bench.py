@@ -227,15 +227,21 @@ def peptide_solvent_stages(reps, check):
         ev1.record(st)
         torch.cuda.synchronize()
         ms[name] = ev0.elapsed_time(ev1) / reps
-    # the whole outer step (respa_inner 1): the library's captured step
+    # the whole outer step (respa_inner 1): the library's captured step, with
+    # its default inner loop and with the other one
     from hymd_amd.md import RespaMD
     md = RespaMD(s)
-    md.run(2)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    md.run(reps)
-    torch.cuda.synchronize()
-    step_ms = 1e3 * (time.perf_counter() - t0) / reps
+    default_fused = md.fused
+    paths = {}
+    for fused in (default_fused, not default_fused):
+        md.set_fused(fused)
+        md.run(2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        md.run(reps)
+        torch.cuda.synchronize()
+        paths["fused" if fused else "per_kernel"] = 1e3 * (time.perf_counter() - t0) / reps
+    step_ms = paths["fused" if default_fused else "per_kernel"]
     md.close()
     out = {"workload": "peptide_solvent: %d beads (%d chains x 40 residues x 3 beads + solvent), %.0f nm box, "
                        "%d^3 mesh for density and PME, %d types, %d charges, %d bonds, %d angles, %d CBT "
@@ -243,8 +249,13 @@ def peptide_solvent_stages(reps, check):
                                                    len(s.bonds[0]), len(s.angles[0]), m4),
            "device_ms": ms, "device_ms_outer_step_forces": sum(ms.values()),
            "md_ms_per_outer_step": step_ms, "md_particle_steps_per_s": n / (step_ms * 1e-3),
-           "md_path": "RespaMD.run(): one captured outer step (per-kernel inner loop with dihedrals, PME of "
-                      "the charges and of the dipoles) replayed as a hipGraph"}
+           "md_ms_per_outer_step_by_inner_loop": {k: round(v, 4) for k, v in paths.items()},
+           "md_path": "RespaMD.run(): one captured outer step replayed as a hipGraph (%s inner loop: %s; PME of "
+                      "the charges and of the dipoles)" % (
+                          "fused" if default_fused else "per-kernel",
+                          "bonds, angles and FP64 dihedrals per sub-step in one kernel, dipoles after it"
+                          if default_fused else "the last sub-step's bonded and dihedral kernels on a second "
+                                                "stream beside the field update and the charges' PME")}
     if check:
         from oracle import bonded as OB
         from oracle import field as OF

@@ -19,6 +19,10 @@ struct hymd_md {
     hymd_topology *topo = nullptr;
     hipStream_t work = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    // per-kernel path: the last inner step's bonded forces run on `side`
+    // beside the field update (both read the positions after the last drift)
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // captured outer steps: [1] starts with the last step's outer half kick
     // pending (fused path, replayed back to back), [0] without (per-kernel
     // path, or the fused path after the velocities were completed)
@@ -183,7 +187,8 @@ static float *slow_force(hymd_md *m) { return (m->charges || m->dih) ? m->f_slow
 // Slow forces beyond the field ones (main.py:1044-1095): f_slow = f_field
 // + PME forces of the charges + the dipole-charge PME forces redistributed
 // onto the backbone, in the order (and rounding) of the Python step.
-static int peptide_slow_forces(hymd_md *m, bool energy) {
+// Part one: f_slow = f_field (+ the charges' PME forces).
+static int charge_slow_forces(hymd_md *m, bool energy) {
     hipStream_t s = m->work;
     const int64_t n3 = 3 * m->n;
     HYMD_TRY(force_combine(m->f_slow, m->f_field, nullptr, n3, 3, s));
@@ -193,6 +198,14 @@ static int peptide_slow_forces(hymd_md *m, bool energy) {
                                            energy ? &m->extra_e[1] : nullptr));
         HYMD_TRY(force_combine(m->f_slow, (const float *)m->f_elec.ptr, nullptr, n3, 1, s));
     }
+    return HYMD_OK;
+}
+
+// Part two: + the dipole charges' PME forces redistributed onto the backbone
+// (reads the dipoles of the last inner step).
+static int dipole_slow_forces(hymd_md *m) {
+    hipStream_t s = m->work;
+    const int64_t n3 = 3 * m->n;
     if (m->protein) {
         const int64_t nd = 4 * m->m4;
         HYMD_TRY(convert_f64_f32((float *)m->dip32.ptr, (double *)m->dipoles.ptr, 3 * nd, true, s));
@@ -202,6 +215,26 @@ static int peptide_slow_forces(hymd_md *m, bool energy) {
         HYMD_TRY(hymd_dihedrals_redistribute(m->dih, (const double *)m->f_dip64.ptr, (const double *)m->tmat.ptr,
                                              (double *)m->f_recon.ptr, s));
         HYMD_TRY(force_combine(m->f_slow, nullptr, (const double *)m->f_recon.ptr, n3, 2, s));
+    }
+    return HYMD_OK;
+}
+
+static int peptide_slow_forces(hymd_md *m, bool energy) {
+    HYMD_TRY(charge_slow_forces(m, energy));
+    return dipole_slow_forces(m);
+}
+
+// Bonded forces of one inner step at the current x on stream s: bonds +
+// angles, then with dihedrals (dipoles on the last inner step, for a
+// protein) f_fast = f_ba + f_dih.  eb (host double[2]) with energies: the
+// bond / angle energies, the dihedral one into extra_e[0].
+static int bonded_forces(hymd_md *m, bool last_inner, double *eb, hipStream_t s) {
+    HYMD_TRY(hymd_topology_forces(m->topo, m->x, m->dih ? (float *)m->f_ba.ptr : m->f_fast, m->box, eb, nullptr, s));
+    if (m->dih) {
+        HYMD_TRY(hymd_dihedrals_forces(m->dih, m->x, (double *)m->f_dih.ptr, (double *)m->dipoles.ptr,
+                                       (double *)m->tmat.ptr, m->box, (last_inner && m->protein) ? 1 : 0,
+                                       eb ? &m->extra_e[0] : nullptr, s));
+        HYMD_TRY(force_combine(m->f_fast, (const float *)m->f_ba.ptr, (const double *)m->f_dih.ptr, 3 * m->n, 0, s));
     }
     return HYMD_OK;
 }
@@ -260,25 +293,28 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
     }
     const bool extra = m->charges || m->dih;
     float *f_slow = extra ? m->f_slow : m->f_field;
-    const int64_t n3 = 3 * m->n;
+    // the last inner step's bonded forces beside the field update and the
+    // charges' PME (all read the positions after the last drift; the
+    // dipoles' PME waits for them), except in steps with energies (their
+    // reductions synchronise) or stage timing
+    const bool overlap = !energies && !m->timed;
     for (int i = 0; i < m->respa_inner; ++i) {
         const bool first = (i == 0), last_inner = (i == m->respa_inner - 1);
         mark(m, 2);
         HYMD_TRY(hymd_respa_kick_drift(m->x, m->v, m->x32, m->n, m->f_fast, half_in, first ? f_slow : m->f_fast,
                                        first ? half_out : half_in, m->dt, m->box, m->work));
         mark(m, 2);
+        if (last_inner && overlap) {
+            HYMD_HIP(hipEventRecord(m->ev_fork, m->work));
+            HYMD_HIP(hipStreamWaitEvent(m->side, m->ev_fork, 0));
+            HYMD_TRY(bonded_forces(m, true, nullptr, m->side));
+            HYMD_HIP(hipEventRecord(m->ev_join, m->side));
+            break;
+        }
         const bool last_e = energies && last_inner;
         double eb[2];
         mark(m, 1);
-        HYMD_TRY(hymd_topology_forces(m->topo, m->x, m->dih ? (float *)m->f_ba.ptr : m->f_fast, m->box,
-                                      last_e ? eb : nullptr, nullptr, m->work));
-        if (m->dih) {  // dihedrals (dipoles on the last inner step), then f_fast = f_ba + f_dih
-            HYMD_TRY(hymd_dihedrals_forces(m->dih, m->x, (double *)m->f_dih.ptr, (double *)m->dipoles.ptr,
-                                           (double *)m->tmat.ptr, m->box, (last_inner && m->protein) ? 1 : 0,
-                                           last_e ? &m->extra_e[0] : nullptr, m->work));
-            HYMD_TRY(force_combine(m->f_fast, (const float *)m->f_ba.ptr, (const double *)m->f_dih.ptr, n3, 0,
-                                   m->work));
-        }
+        HYMD_TRY(bonded_forces(m, last_inner, last_e ? eb : nullptr, m->work));
         mark(m, 1);
         if (last_e) {
             energies[1] = eb[0];
@@ -294,7 +330,11 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
     mark(m, 4);
     if (extra) {
         mark(m, 5);
-        HYMD_TRY(peptide_slow_forces(m, energies != nullptr));
+        HYMD_TRY(charge_slow_forces(m, energies != nullptr));
+    }
+    if (overlap) HYMD_HIP(hipStreamWaitEvent(m->work, m->ev_join, 0));
+    if (extra) {
+        HYMD_TRY(dipole_slow_forces(m));
         mark(m, 5);
     }
     HYMD_TRY(hymd_respa_kick(m->v, m->n, m->f_fast, half_in, f_slow, half_out, m->work));
@@ -390,12 +430,14 @@ extern "C" int hymd_md_create(hymd_md **out, hymd_pm *pm, hymd_topology *topo, i
     for (int d = 0; d < 3; ++d) m->box[d] = box[d];
     m->pm_gen = pm_generation(pm);
     hipError_t e = hipStreamCreateWithFlags(&m->work, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_in, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_out, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&m->ev_join, hipEventDisableTiming);
     if (e != hipSuccess) {
         set_error("stream/event creation failed: %s", hipGetErrorString(e));
-        if (m->work) (void)hipStreamDestroy(m->work);
-        delete m;
+        (void)hymd_md_destroy(m);
         return HYMD_ERR_HIP;
     }
     // fuse the inner loop when the topology allows it; otherwise keep the
@@ -421,11 +463,15 @@ static void drop_graphs(hymd_md *m) {
 extern "C" int hymd_md_destroy(hymd_md *m) {
     if (!m) return HYMD_OK;
     if (m->work) (void)hipStreamSynchronize(m->work);
+    if (m->side) (void)hipStreamSynchronize(m->side);
     for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
     if (m->fused) (void)hymd_fused_destroy(m->fused);
     drop_graphs(m);
     if (m->ev_in) (void)hipEventDestroy(m->ev_in);
     if (m->ev_out) (void)hipEventDestroy(m->ev_out);
+    if (m->ev_fork) (void)hipEventDestroy(m->ev_fork);
+    if (m->ev_join) (void)hipEventDestroy(m->ev_join);
+    if (m->side) (void)hipStreamDestroy(m->side);
     if (m->work) (void)hipStreamDestroy(m->work);
     delete m;
     return HYMD_OK;
@@ -748,9 +794,11 @@ extern "C" int hymd_md_set_peptide(hymd_md *m, const float *charges, double elec
     HYMD_REQUIRE(f_slow || (!charges && !dih), "f_slow is required with charges or dihedrals");
     HYMD_REQUIRE(!dih || n_dihedrals >= 0, "bad dihedral count");
     HYMD_HIP(hipStreamSynchronize(m->work));
-    // a fused inner loop is rebuilt with the dihedral terms below (the
-    // per-kernel loop stays when they do not fit its blocks)
-    const bool fuse_again = dih && m->fused;
+    // With dihedrals the per-kernel inner loop is the default: its last
+    // sub-step's bonded and dihedral kernels run beside the field update and
+    // the charges' PME, which the fused loop cannot do (its FP64 dihedral
+    // terms hold a block at two waves per SIMD); hymd_md_set_fused(m, 1)
+    // builds the fused loop with the dihedral terms.
     if (dih) HYMD_TRY(hymd_md_set_fused(m, 0));
     // the periodic reorder permutes the handle's own per-particle arrays,
     // not the caller's charges or summed slow forces
@@ -777,10 +825,6 @@ extern "C" int hymd_md_set_peptide(hymd_md *m, const float *charges, double elec
         HYMD_TRY(m->dip32.reserve(sizeof(float) * 3 * nd));
         HYMD_TRY(m->f_dip64.reserve(sizeof(double) * 3 * nd));
         HYMD_TRY(m->f_recon.reserve(sizeof(double) * n3));
-    }
-    if (fuse_again) {
-        const int st = hymd_md_set_fused(m, 1);
-        if (st != HYMD_OK && st != HYMD_ERR_UNSUPPORTED) return st;
     }
     return HYMD_OK;
 }

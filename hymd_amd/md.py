@@ -20,10 +20,12 @@ main.py:508-588, 802-887, 1044-1095 do: the dihedral forces (FP64 kernel,
 dipoles reconstructed on the last inner step) to the fast forces, and the
 PME forces of the charges plus the dipole-charge PME forces redistributed
 onto the backbone (force.py:855-877) to the slow forces, both in ``step()``
-and in the library's captured outer step (``run()``).  The library's fused
-inner loop evaluates the dihedral terms with the bonds and angles every
-sub-step (FP64 dihedral arithmetic) and reconstructs the dipoles at the final
-positions after it; charges turn the periodic reorder off (it would leave
+and in the library's captured outer step (``run()``).  With dihedrals that
+step uses the per-kernel inner loop, whose last sub-step's bonded and
+dihedral kernels run on a second stream beside the field update; the fused
+inner loop with the dihedral terms (FP64 dihedral arithmetic every sub-step,
+dipoles reconstructed at the final positions after it) is there through
+``set_fused(True)``.  Charges turn the periodic reorder off (it would leave
 the charges behind).
 """
 import ctypes

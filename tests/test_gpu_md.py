@@ -162,13 +162,13 @@ def test_peptide_md_matches_oracle_loop(hip, oracle_built):
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_peptide_native_step_equals_python_step(hip, use_graph):
     """The library's outer step with PME, dihedrals and dipoles on the
-    per-kernel inner loop (eager and hipGraph replay) runs the kernels of the
-    Python step in its order: identical results."""
+    per-kernel inner loop (its default with dihedrals; eager and hipGraph
+    replay, the last sub-step's bonded kernels on a second stream beside the
+    field update) runs the kernels of the Python step: identical results."""
     from hymd_amd.md import RespaMD
     a = RespaMD(_small_peptide())
     b = RespaMD(_small_peptide())
-    assert b.fused           # dihedrals join the fused inner loop by default
-    b.set_fused(False)
+    assert not b.fused
     for _ in range(3):
         a.step()
     b.run(3, use_graph=use_graph)
@@ -193,7 +193,7 @@ def test_fused_peptide_loop_matches_per_kernel_path(hip, use_graph):
     from hymd_amd.md import RespaMD
     a = RespaMD(_small_peptide(), term_precision="fp64")
     b = RespaMD(_small_peptide(), term_precision="fp64")
-    a.set_fused(False)
+    b.set_fused(True)
     assert b.fused and not a.fused and b.protein
     a.run(1, use_graph=use_graph)
     b.run(1, use_graph=use_graph)
@@ -228,9 +228,9 @@ def test_fused_peptide_loop_matches_per_kernel_path(hip, use_graph):
 
 
 def test_fused_peptide_run_matches_oracle(hip, oracle_built):
-    """The library's default outer step for the peptide/solvent system (fused
-    inner loop with FP32 bond / angle terms and FP64 dihedral terms, PME of
-    the charges and of the dipole charges) against the FP64 oracle loop: two
+    """The library's outer step for the peptide/solvent system on the fused
+    inner loop (FP32 bond / angle terms, FP64 dihedral terms; PME of the
+    charges and of the dipole charges) against the FP64 oracle loop: two
     outer steps of two inner steps, then a third with energies."""
     from hymd_amd import synth
     from hymd_amd.md import RespaMD
@@ -240,6 +240,7 @@ def test_fused_peptide_run_matches_oracle(hip, oracle_built):
     A, b, c0 = synth.hamiltonian_form(s)
     ref = OM.State(s, F.Quadratic(A, b, c0))
     md = RespaMD(s)
+    md.set_fused(True)
     assert md.fused and md.protein
     io = md.in_input_order
     md.run(2, use_graph=True)
