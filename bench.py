@@ -69,9 +69,10 @@ def profiled(kernel, key):
     return None
 
 
-def cpu_baseline(system, steps):
-    """FP64 oracle outer steps on the host cores (bounded sample); also
-    returns the oracle's initial (field, bonded) forces."""
+def cpu_baseline(system, steps, warmup=3):
+    """FP64 oracle outer steps on the host cores (bounded sample): `warmup`
+    untimed steps, then `steps` steps timed one by one, value from the median
+    step; also returns the oracle's initial (field, bonded) forces."""
     from oracle import field as OF
     from oracle import md as OM
     cores = min(os.cpu_count() or 1, 16)
@@ -82,14 +83,20 @@ def cpu_baseline(system, steps):
     t_init = time.perf_counter() - t0
     initial = (st.f_field.copy(), st.f_fast.copy(),
                {"field": st.field_energy(), "bond": st.e_fast["bond"], "angle": st.e_fast["angle"]})
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    for _ in range(warmup):
         st.outer_step()
-    dt = time.perf_counter() - t0
-    return initial, {"value": system.n * steps / dt, "unit": "particle-steps/s", "cores": cores, "kind": "port",
-                "sample": f"{steps} outer steps of the same 1,048,576-bead workload after a {t_init:.1f} s "
-                          f"force initialisation; numpy/scipy.fft FP64 with {cores} FFT workers, "
-                          f"single-threaded paint/readout and bonded C loops"}
+    times = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        st.outer_step()
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return initial, {"value": system.n / med, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+                     "ms_per_step_median": round(1e3 * med, 1), "ms_per_step_min_max": [round(1e3 * min(times), 1),
+                                                                                     round(1e3 * max(times), 1)],
+                     "sample": f"{warmup} warm-up + {steps} timed outer steps (median) of the same 1,048,576-bead "
+                               f"workload after a {t_init:.1f} s force initialisation; numpy/scipy.fft FP64 with "
+                               f"{cores} FFT workers, single-threaded paint/readout and bonded C loops"}
 
 
 def batched_chi_gradients(n_replicas, iters):
@@ -495,7 +502,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--roofline-kernel", default="bonded", choices=["bonded", "kick_drift", "field", "readout"],
                     help="stage to time; 'bonded' is the fused inner loop when the topology allows it")
-    ap.add_argument("--cpu-steps", type=int, default=4, help="oracle outer steps for cpu_baseline (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=20,
+                    help="timed oracle outer steps for cpu_baseline, after 3 warm-up steps (0 = skip)")
     ap.add_argument("--batch-replicas", type=int, default=64, help="replicas of the batched chi-gradient metric (0 = skip)")
     ap.add_argument("--batch-iters", type=int, default=20)
     ap.add_argument("--peptide-reps", type=int, default=10,
