@@ -282,6 +282,11 @@ TopoTerms topology_terms(const hymd_topology *t);
 // compute_field_force at the particle set sorted by the last update_field on
 // this handle (same positions and types)
 int field_readout_last(hymd_pm *pm, int n_types, const float *force_mesh, float *force);
+// hymd_update_field_force_q with accumulate: elec_forces += the PME forces
+// (the float32 sum of a separate addition pass; uncharged rows untouched)
+int update_field_force_q_impl(hymd_pm *pm, const float *x, const float *charges, int64_t n, double sigma,
+                              double elec_factor, double self_energy, float *elec_forces, float *phi_q, float *psi,
+                              float *elec_field, double *energy_q, bool accumulate);
 
 // Changes whenever a buffer of the handle is reallocated or its box is set:
 // work captured against the handle (graphs hold its pointers and its mesh by

@@ -276,17 +276,22 @@ __global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict
                                                           const float *__restrict__ scale_per_particle,
                                                           int64_t n, int n_types, Mesh m,
                                                           float *__restrict__ out, const int *__restrict__ perm,
-                                                          const int *__restrict__ n_dev) {
+                                                          const int *__restrict__ n_dev, int accumulate) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t first = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    // with a device count of kept particles (a weighted binning, usually a
+    // few per cent of the grid's threads) the blocks keep their hardware
+    // order: the XCD-aware order would put every busy block on one XCD
+    const int64_t first = (int64_t)(n_dev ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
     if (n_dev) n = min(n, (int64_t)*n_dev);  // particles kept by the binning
     for (int64_t i = first; i < n; i += stride) {
         const float4 r = rec ? rec[i] : make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.f);
         const int t = __float_as_int(r.w);
         const int64_t o = perm ? (int64_t)perm[i] : i;
         if ((unsigned)t >= (unsigned)n_types) {
+            if (!accumulate) {
 #pragma unroll
-            for (int c = 0; c < NCOMP; ++c) out[NCOMP * o + c] = 0.f;
+                for (int c = 0; c < NCOMP; ++c) out[NCOMP * o + c] = 0.f;
+            }
             continue;
         }
         const CicAxis cx = cic_axis((double)r.x, m.inv_h[0], m.mx);
@@ -314,7 +319,11 @@ __global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict
         }
         const double s = scale_per_particle ? (double)scale_per_particle[i] : 1.0;
 #pragma unroll
-        for (int q = 0; q < NCOMP; ++q) out[NCOMP * o + q] = (float)(acc[q] * s);
+        for (int q = 0; q < NCOMP; ++q) {
+            const float f = (float)(acc[q] * s);
+            // accumulate: out += f, the float32 addition of a separate sum pass
+            out[NCOMP * o + q] = accumulate ? out[NCOMP * o + q] + f : f;
+        }
     }
 }
 
@@ -1335,8 +1344,11 @@ static int paint_sorted(hymd_pm *pm, int T, const double *mass, float *out, long
 }
 
 // Readout of ncomp grids per type at the last sorted particle set, written to
-// the particles' original rows (scaled by their weights if the sort had them).
-static int readout_sorted(hymd_pm *pm, int ncomp, int n_types, const float *grids, float *out, bool weighted) {
+// the particles' original rows (scaled by their weights if the sort had them);
+// accumulate: out += the readout (rows the binning left out keep their
+// values) instead of out = it.
+static int readout_sorted(hymd_pm *pm, int ncomp, int n_types, const float *grids, float *out, bool weighted,
+                          bool accumulate = false) {
     const int64_t n = pm->sorted_n;
     if (n > 0) {
         const float4 *rec = (const float4 *)pm->xs.ptr;
@@ -1346,16 +1358,16 @@ static int readout_sorted(hymd_pm *pm, int ncomp, int n_types, const float *grid
         // zero, the kept ones are counted in start[rows]
         const int *kept = nullptr;
         if (pm->sorted_weights) {
-            HYMD_TRY(zero_words(out, ncomp * n, pm->stream));
+            if (!accumulate) HYMD_TRY(zero_words(out, ncomp * n, pm->stream));
             kept = (const int *)pm->start.ptr + (int64_t)pm->mesh.mx * pm->mesh.my;
         }
         const int bs = pm->readout_block;
         if (ncomp == 3)
             readout_kernel<3><<<grid_for(n, bs), bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh,
-                                                                      out, pp, kept);
+                                                                      out, pp, kept, accumulate ? 1 : 0);
         else
             readout_kernel<1><<<grid_for(n, bs), bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh,
-                                                                      out, pp, kept);
+                                                                      out, pp, kept, accumulate ? 1 : 0);
     }
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
@@ -1423,7 +1435,7 @@ extern "C" int hymd_readout(hymd_pm *pm, const float *grid, const float *x, int6
     HYMD_REQUIRE(pm && grid && out && (x || n == 0) && n >= 0, "bad argument");
     if (n > 0)
         readout_kernel<1><<<grid_for(n), kBlock, 0, pm->stream>>>(grid, nullptr, x, nullptr, n, 1, pm->mesh, out,
-                                                                    nullptr, nullptr);
+                                                                    nullptr, nullptr, 0);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }
@@ -1823,10 +1835,23 @@ extern "C" int hymd_compute_field_force(hymd_pm *pm, const float *x, const int32
     return readout_sorted(pm, 3, n_types, force_mesh, force, false);
 }
 
+namespace hymd {
+int update_field_force_q_impl(hymd_pm *pm, const float *x, const float *charges, int64_t n, double sigma,
+                              double elec_factor, double self_energy, float *elec_forces, float *phi_q, float *psi,
+                              float *elec_field, double *energy_q, bool accumulate);
+}
+
 extern "C" int hymd_update_field_force_q(hymd_pm *pm, const float *x, const float *charges, int64_t n,
                                          double sigma, double elec_factor, double self_energy,
                                          float *elec_forces, float *phi_q, float *psi, float *elec_field,
                                          double *energy_q) {
+    return update_field_force_q_impl(pm, x, charges, n, sigma, elec_factor, self_energy, elec_forces, phi_q, psi,
+                                     elec_field, energy_q, false);
+}
+
+int hymd::update_field_force_q_impl(hymd_pm *pm, const float *x, const float *charges, int64_t n, double sigma,
+                                    double elec_factor, double self_energy, float *elec_forces, float *phi_q,
+                                    float *psi, float *elec_field, double *energy_q, bool accumulate) {
     HYMD_REQUIRE(pm && elec_forces && (n == 0 || (x && charges)) && n >= 0, "bad argument");
     const Mesh &m = pm->mesh;
     hipStream_t s = pm->stream;
@@ -1860,7 +1885,7 @@ extern "C" int hymd_update_field_force_q(hymd_pm *pm, const float *x, const floa
                                     s));
     }
     if (psi) HYMD_TRY(fft_c2r_yz(pm->fft, out + m.gc, psi, 1, 1.f, s));
-    HYMD_TRY(readout_sorted(pm, 3, 1, field, elec_forces, true));
+    HYMD_TRY(readout_sorted(pm, 3, 1, field, elec_forces, true, accumulate));
     if (energy_q) {
         double e = 0.0;
         HYMD_TRY(read_result(pm, pm->kblocks, 1, &e));

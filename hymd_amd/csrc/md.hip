@@ -67,7 +67,7 @@ struct hymd_md {
     bool protein = false;
     const float *q_dip = nullptr;
     float *f_slow = nullptr;  // caller's: summed slow forces
-    DevBuf f_elec, f_ba, f_dih, dipoles, tmat, f_dip, dip32, f_dip64, f_recon;
+    DevBuf f_ba, f_dih, dipoles, tmat, f_dip, dip32, f_dip64, f_recon;
     double extra_e[2] = {0.0, 0.0};  // dihedral, field_q of the last energies step
     // periodic reorder of the one-particle molecules (fused path): every
     // reorder_every outer steps the rows [single0, n) are put in the mesh-row
@@ -187,17 +187,20 @@ static float *slow_force(hymd_md *m) { return (m->charges || m->dih) ? m->f_slow
 // Slow forces beyond the field ones (main.py:1044-1095): f_slow = f_field
 // + PME forces of the charges + the dipole-charge PME forces redistributed
 // onto the backbone, in the order (and rounding) of the Python step.
-// Part one: f_slow = f_field (+ the charges' PME forces).
+// The field readout, then f_slow = f_field (a coalesced copy: a second
+// scattered store from the readout costs more).
+static int slow_readout(hymd_md *m) {
+    HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));
+    if (m->charges || m->dih) HYMD_TRY(force_combine(m->f_slow, m->f_field, nullptr, 3 * m->n, 3, m->work));
+    return HYMD_OK;
+}
+
+// Part one: f_slow += the charges' PME forces (read out into f_slow, each
+// a float32 addition to the field force as a separate sum pass would make).
 static int charge_slow_forces(hymd_md *m, bool energy) {
-    hipStream_t s = m->work;
-    const int64_t n3 = 3 * m->n;
-    HYMD_TRY(force_combine(m->f_slow, m->f_field, nullptr, n3, 3, s));
-    if (m->charges) {
-        HYMD_TRY(hymd_update_field_force_q(m->pm, m->x32, m->charges, m->n, m->sigma, m->elec, m->e_self,
-                                           (float *)m->f_elec.ptr, m->phi_q, m->psi, nullptr,
-                                           energy ? &m->extra_e[1] : nullptr));
-        HYMD_TRY(force_combine(m->f_slow, (const float *)m->f_elec.ptr, nullptr, n3, 1, s));
-    }
+    if (m->charges)
+        HYMD_TRY(update_field_force_q_impl(m->pm, m->x32, m->charges, m->n, m->sigma, m->elec, m->e_self, m->f_slow,
+                                           m->phi_q, m->psi, nullptr, energy ? &m->extra_e[1] : nullptr, true));
     return HYMD_OK;
 }
 
@@ -281,10 +284,10 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
                                     m->force_mesh, energies ? &energies[0] : nullptr, m->spectra));
         mark(m, 3);
         mark(m, 4);
-        HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));
+        HYMD_TRY(slow_readout(m));
         mark(m, 4);
         // PME of the charges (field.py:241-403) into f_slow = f_field + f_elec
-        if (m->charges) {
+        if (m->charges || m->dih) {
             mark(m, 5);
             HYMD_TRY(peptide_slow_forces(m, energies != nullptr));
             mark(m, 5);
@@ -326,7 +329,7 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
                                m->force_mesh, nullptr, m->spectra, nullptr, energies ? &energies[0] : nullptr));
     mark(m, 3);
     mark(m, 4);
-    HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));
+    HYMD_TRY(slow_readout(m));
     mark(m, 4);
     if (extra) {
         mark(m, 5);
@@ -387,7 +390,7 @@ static int enqueue_half(hymd_md *m, int part, double *virials, double *energies)
     }
     HYMD_TRY(hymd_update_field(m->pm, m->x32, m->types, m->n, m->T, m->A, m->b, m->c0, m->mass_t, m->sigma,
                                m->force_mesh, nullptr, m->spectra, nullptr, energies ? &energies[0] : nullptr));
-    HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));
+    HYMD_TRY(slow_readout(m));
     if (extra) HYMD_TRY(peptide_slow_forces(m, energies != nullptr));
     if (m->fused) {
         m->pending = 1;
@@ -813,7 +816,6 @@ extern "C" int hymd_md_set_peptide(hymd_md *m, const float *charges, double elec
     m->protein = dih && dipole_charges;
     m->f_slow = f_slow;
     const size_t n3 = 3 * (size_t)std::max<int64_t>(m->n, 1), nd = 4 * (size_t)std::max<int64_t>(m->m4, 1);
-    if (charges) HYMD_TRY(m->f_elec.reserve(sizeof(float) * n3));
     if (dih) {
         HYMD_TRY(m->f_ba.reserve(sizeof(float) * n3));
         HYMD_TRY(m->f_dih.reserve(sizeof(double) * n3));
