@@ -277,13 +277,24 @@ __global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict
                                                           int64_t n, int n_types, Mesh m,
                                                           float *__restrict__ out, const int *__restrict__ perm,
                                                           const int *__restrict__ n_dev, int accumulate) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    // with a device count of kept particles (a weighted binning, usually a
-    // few per cent of the grid's threads) the blocks keep their hardware
-    // order: the XCD-aware order would put every busy block on one XCD
-    const int64_t first = (int64_t)(n_dev ? blockIdx.x : xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
-    if (n_dev) n = min(n, (int64_t)*n_dev);  // particles kept by the binning
-    for (int64_t i = first; i < n; i += stride) {
+    int64_t first, stride, end = n;
+    if (n_dev && gridDim.x >= kXcds) {
+        // a device count of kept particles (a weighted binning): XCD x takes
+        // the x-th eighth of the kept range with its own blocks (the hardware
+        // deals blocks to the XCDs round robin), so each L2 sees neighbouring
+        // rows and every XCD has work whatever the count
+        const int64_t nk = min(n, (int64_t)*n_dev);
+        const unsigned x = blockIdx.x % kXcds, xb = blockIdx.x / kXcds, nbx = gridDim.x / kXcds;
+        if (xb >= nbx) return;  // the grid's remainder beyond a multiple of kXcds
+        first = nk * x / kXcds + (int64_t)xb * blockDim.x + threadIdx.x;
+        end = nk * (x + 1) / kXcds;
+        stride = (int64_t)nbx * blockDim.x;
+    } else {
+        if (n_dev) end = min(n, (int64_t)*n_dev);
+        first = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+        stride = (int64_t)gridDim.x * blockDim.x;
+    }
+    for (int64_t i = first; i < end; i += stride) {
         const float4 r = rec ? rec[i] : make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.f);
         const int t = __float_as_int(r.w);
         const int64_t o = perm ? (int64_t)perm[i] : i;
@@ -1362,12 +1373,15 @@ static int readout_sorted(hymd_pm *pm, int ncomp, int n_types, const float *grid
             kept = (const int *)pm->start.ptr + (int64_t)pm->mesh.mx * pm->mesh.my;
         }
         const int bs = pm->readout_block;
+        // a weighted binning keeps a count known on the device only: a grid
+        // of at most 4096 blocks loops over it
+        const int nb = kept ? grid_for(n, bs, 4096) : grid_for(n, bs);
         if (ncomp == 3)
-            readout_kernel<3><<<grid_for(n, bs), bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh,
-                                                                      out, pp, kept, accumulate ? 1 : 0);
+            readout_kernel<3><<<nb, bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh, out, pp, kept,
+                                                         accumulate ? 1 : 0);
         else
-            readout_kernel<1><<<grid_for(n, bs), bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh,
-                                                                      out, pp, kept, accumulate ? 1 : 0);
+            readout_kernel<1><<<nb, bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh, out, pp, kept,
+                                                         accumulate ? 1 : 0);
     }
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
