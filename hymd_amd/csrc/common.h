@@ -294,6 +294,8 @@ int update_field_force_q_impl(hymd_pm *pm, const float *x, const float *charges,
 uint64_t pm_generation(const hymd_pm *pm);
 // the handle's current box edges
 void pm_box(const hymd_pm *pm, double box[3]);
+// a new handle with src's mesh and box (own buffers and plans)
+int pm_clone(const hymd_pm *src, int32_t max_types, void *stream, hymd_pm **out);
 
 // CSVR thermostat (thermostat.hip): the groups of one call, in the
 // reference's order, and the tables fixed for a particle set.

@@ -985,6 +985,11 @@ uint64_t hymd::pm_generation(const hymd_pm *pm) {
     return g;
 }
 
+int hymd::pm_clone(const hymd_pm *src, int32_t max_types, void *stream, hymd_pm **out) {
+    const int32_t mesh[3] = {src->mesh.mx, src->mesh.my, src->mesh.mz};
+    return hymd_pm_create(out, mesh, src->mesh.box, max_types, stream);
+}
+
 void hymd::pm_box(const hymd_pm *pm, double box[3]) {
     for (int d = 0; d < 3; ++d) box[d] = pm->mesh.box[d];
 }

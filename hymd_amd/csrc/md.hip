@@ -66,6 +66,9 @@ struct hymd_md {
     int64_t m4 = 0;
     bool protein = false;
     const float *q_dip = nullptr;
+    // a second mesh handle (same mesh and box, own buffers) for the dipole
+    // charges' PME on the side stream, beside the field update on `pm`
+    hymd_pm *pm2 = nullptr;
     float *f_slow = nullptr;  // caller's: summed slow forces
     DevBuf f_ba, f_dih, dipoles, tmat, f_dip, dip32, f_dip64, f_recon;
     double extra_e[2] = {0.0, 0.0};  // dihedral, field_q of the last energies step
@@ -204,27 +207,61 @@ static int charge_slow_forces(hymd_md *m, bool energy) {
     return HYMD_OK;
 }
 
-// Part two: + the dipole charges' PME forces redistributed onto the backbone
-// (reads the dipoles of the last inner step).
+// Part two: the dipole charges' PME forces redistributed onto the backbone
+// (reads the dipoles of the last inner step) into f_recon, on stream s with
+// the mesh handle pmd (m->pm on the work stream, or m->pm2 on the side
+// stream), then f_slow += f_recon on the work stream.
+static int dipole_recon_forces(hymd_md *m, hymd_pm *pmd, hipStream_t s) {
+    if (!m->protein) return HYMD_OK;
+    const int64_t nd = 4 * m->m4;
+    HYMD_TRY(convert_f64_f32((float *)m->dip32.ptr, (double *)m->dipoles.ptr, 3 * nd, true, s));
+    HYMD_TRY(hymd_update_field_force_q(pmd, (const float *)m->dip32.ptr, m->q_dip, nd, m->sigma, m->elec, 0.0,
+                                       (float *)m->f_dip.ptr, nullptr, nullptr, nullptr, nullptr));
+    HYMD_TRY(convert_f64_f32((float *)m->f_dip.ptr, (double *)m->f_dip64.ptr, 3 * nd, false, s));
+    return hymd_dihedrals_redistribute(m->dih, (const double *)m->f_dip64.ptr, (const double *)m->tmat.ptr,
+                                       (double *)m->f_recon.ptr, s);
+}
+
+static int dipole_combine(hymd_md *m) {
+    if (!m->protein) return HYMD_OK;
+    return force_combine(m->f_slow, nullptr, (const double *)m->f_recon.ptr, 3 * m->n, 2, m->work);
+}
+
 static int dipole_slow_forces(hymd_md *m) {
-    hipStream_t s = m->work;
-    const int64_t n3 = 3 * m->n;
-    if (m->protein) {
-        const int64_t nd = 4 * m->m4;
-        HYMD_TRY(convert_f64_f32((float *)m->dip32.ptr, (double *)m->dipoles.ptr, 3 * nd, true, s));
-        HYMD_TRY(hymd_update_field_force_q(m->pm, (const float *)m->dip32.ptr, m->q_dip, nd, m->sigma, m->elec, 0.0,
-                                           (float *)m->f_dip.ptr, nullptr, nullptr, nullptr, nullptr));
-        HYMD_TRY(convert_f64_f32((float *)m->f_dip.ptr, (double *)m->f_dip64.ptr, 3 * nd, false, s));
-        HYMD_TRY(hymd_dihedrals_redistribute(m->dih, (const double *)m->f_dip64.ptr, (const double *)m->tmat.ptr,
-                                             (double *)m->f_recon.ptr, s));
-        HYMD_TRY(force_combine(m->f_slow, nullptr, (const double *)m->f_recon.ptr, n3, 2, s));
-    }
-    return HYMD_OK;
+    HYMD_TRY(dipole_recon_forces(m, m->pm, m->work));
+    return dipole_combine(m);
 }
 
 static int peptide_slow_forces(hymd_md *m, bool energy) {
     HYMD_TRY(charge_slow_forces(m, energy));
     return dipole_slow_forces(m);
+}
+
+// The same slow forces beside the field update: on the side stream with the
+// second mesh handle, f_recon = the dipoles' redistributed PME forces and,
+// with `charges` (the fused path, whose side stream has nothing else),
+// f_slow = the charges' PME forces (zero where uncharged).  After the field
+// readout into f_field, side_join_slow_forces forms on the work stream
+// f_slow = elec + field (the float32 sum field + elec: addition commutes),
+// or field + elec with the charges' PME on the work stream, then waits for
+// the side stream and adds f_recon.
+static int side_slow_forces(hymd_md *m, bool charges) {
+    if (charges && m->charges)
+        HYMD_TRY(update_field_force_q_impl(m->pm2, m->x32, m->charges, m->n, m->sigma, m->elec, m->e_self, m->f_slow,
+                                           m->phi_q, m->psi, nullptr, nullptr, false));
+    return dipole_recon_forces(m, m->pm2, m->side);
+}
+
+static int side_join_slow_forces(hymd_md *m, bool charges_on_side) {
+    if (charges_on_side) {
+        HYMD_HIP(hipStreamWaitEvent(m->work, m->ev_join, 0));
+        HYMD_TRY(force_combine(m->f_slow, m->f_field, nullptr, 3 * m->n, m->charges ? 1 : 3, m->work));
+    } else {
+        HYMD_TRY(force_combine(m->f_slow, m->f_field, nullptr, 3 * m->n, 3, m->work));
+        HYMD_TRY(charge_slow_forces(m, false));
+        HYMD_HIP(hipStreamWaitEvent(m->work, m->ev_join, 0));
+    }
+    return dipole_combine(m);
 }
 
 // Bonded forces of one inner step at the current x on stream s: bonds +
@@ -279,10 +316,25 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
         HYMD_TRY(field_key_target(m->pm, m->n, &key));
         HYMD_TRY(fused_inner(m, pending, energies, nullptr, &key));
         mark(m, 1);
+        // the PME of the charges and of the dipoles (second mesh handle) on
+        // the side stream, beside the field update, except in steps with
+        // energies (their reductions synchronise) or stage timing
+        const bool side = m->pm2 && !energies && !m->timed;
+        if (side) {
+            HYMD_HIP(hipEventRecord(m->ev_fork, m->work));
+            HYMD_HIP(hipStreamWaitEvent(m->side, m->ev_fork, 0));
+            HYMD_TRY(side_slow_forces(m, true));
+            HYMD_HIP(hipEventRecord(m->ev_join, m->side));
+        }
         mark(m, 3);
         HYMD_TRY(update_field_keyed(m->pm, m->x32, m->types, m->n, m->T, m->A, m->b, m->c0, m->mass_t, m->sigma,
                                     m->force_mesh, energies ? &energies[0] : nullptr, m->spectra));
         mark(m, 3);
+        if (side) {
+            HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));
+            HYMD_TRY(side_join_slow_forces(m, true));
+            return HYMD_OK;
+        }
         mark(m, 4);
         HYMD_TRY(slow_readout(m));
         mark(m, 4);
@@ -311,6 +363,10 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
             HYMD_HIP(hipEventRecord(m->ev_fork, m->work));
             HYMD_HIP(hipStreamWaitEvent(m->side, m->ev_fork, 0));
             HYMD_TRY(bonded_forces(m, true, nullptr, m->side));
+            // the dipoles' PME (second mesh handle) follows there; the charges'
+            // PME stays on the work stream after the field update (measured:
+            // the side stream already carries the dihedral kernel)
+            if (m->pm2) HYMD_TRY(side_slow_forces(m, false));
             HYMD_HIP(hipEventRecord(m->ev_join, m->side));
             break;
         }
@@ -328,17 +384,22 @@ static int enqueue_step(hymd_md *m, double *energies, int pending) {
     HYMD_TRY(hymd_update_field(m->pm, m->x32, m->types, m->n, m->T, m->A, m->b, m->c0, m->mass_t, m->sigma,
                                m->force_mesh, nullptr, m->spectra, nullptr, energies ? &energies[0] : nullptr));
     mark(m, 3);
-    mark(m, 4);
-    HYMD_TRY(slow_readout(m));
-    mark(m, 4);
-    if (extra) {
-        mark(m, 5);
-        HYMD_TRY(charge_slow_forces(m, energies != nullptr));
-    }
-    if (overlap) HYMD_HIP(hipStreamWaitEvent(m->work, m->ev_join, 0));
-    if (extra) {
-        HYMD_TRY(dipole_slow_forces(m));
-        mark(m, 5);
+    if (overlap && m->pm2) {
+        HYMD_TRY(field_readout_last(m->pm, m->T, m->force_mesh, m->f_field));
+        HYMD_TRY(side_join_slow_forces(m, false));
+    } else {
+        mark(m, 4);
+        HYMD_TRY(slow_readout(m));
+        mark(m, 4);
+        if (extra) {
+            mark(m, 5);
+            HYMD_TRY(charge_slow_forces(m, energies != nullptr));
+        }
+        if (overlap) HYMD_HIP(hipStreamWaitEvent(m->work, m->ev_join, 0));
+        if (extra) {
+            HYMD_TRY(dipole_slow_forces(m));
+            mark(m, 5);
+        }
     }
     HYMD_TRY(hymd_respa_kick(m->v, m->n, m->f_fast, half_in, f_slow, half_out, m->work));
     return HYMD_OK;
@@ -469,6 +530,7 @@ extern "C" int hymd_md_destroy(hymd_md *m) {
     if (m->side) (void)hipStreamSynchronize(m->side);
     for (hipEvent_t e : m->ev) (void)hipEventDestroy(e);
     if (m->fused) (void)hymd_fused_destroy(m->fused);
+    if (m->pm2) (void)hymd_pm_destroy(m->pm2);
     drop_graphs(m);
     if (m->ev_in) (void)hipEventDestroy(m->ev_in);
     if (m->ev_out) (void)hipEventDestroy(m->ev_out);
@@ -512,6 +574,7 @@ static void check_pm(hymd_md *m) {
     if (g == m->pm_gen) return;
     drop_graphs(m);
     pm_box(m->pm, m->box);
+    if (m->pm2) (void)hymd_pm_set_box(m->pm2, m->box);  // the dipoles' handle follows the box
     m->pm_gen = g;
 }
 
@@ -822,11 +885,30 @@ extern "C" int hymd_md_set_peptide(hymd_md *m, const float *charges, double elec
         HYMD_TRY(m->dipoles.reserve(sizeof(double) * 3 * nd));
         HYMD_TRY(m->tmat.reserve(sizeof(double) * 54 * (nd / 4)));
     }
+    if (m->pm2) (void)hymd_pm_destroy(m->pm2);
+    m->pm2 = nullptr;
     if (m->protein) {
         HYMD_TRY(m->f_dip.reserve(sizeof(float) * 3 * nd));
         HYMD_TRY(m->dip32.reserve(sizeof(float) * 3 * nd));
         HYMD_TRY(m->f_dip64.reserve(sizeof(double) * 3 * nd));
         HYMD_TRY(m->f_recon.reserve(sizeof(double) * n3));
+    }
+    if (charges || m->protein) {
+        // the side stream's mesh handle; eager PME calls (the charges at the
+        // current positions, the dipole charges at zero) size its buffers
+        // before any step is captured
+        HYMD_TRY(pm_clone(m->pm, 1, m->side, &m->pm2));
+        HYMD_TRY(m->f_recon.reserve(sizeof(double) * n3));  // (float) scratch of the warm-up
+        if (charges)
+            HYMD_TRY(hymd_update_field_force_q(m->pm2, m->x32, charges, m->n, m->sigma, m->elec, 0.0,
+                                               (float *)m->f_recon.ptr, nullptr, nullptr, nullptr, nullptr));
+        if (m->protein) {
+            HYMD_HIP(hipMemsetAsync(m->dip32.ptr, 0, sizeof(float) * 3 * nd, m->side));
+            HYMD_TRY(hymd_update_field_force_q(m->pm2, (const float *)m->dip32.ptr, m->q_dip, (int64_t)nd, m->sigma,
+                                               m->elec, 0.0, (float *)m->f_dip.ptr, nullptr, nullptr, nullptr,
+                                               nullptr));
+        }
+        HYMD_HIP(hipStreamSynchronize(m->side));
     }
     return HYMD_OK;
 }
