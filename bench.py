@@ -299,14 +299,16 @@ def peptide_solvent_stages(reps, check):
     return out
 
 
-def markov_melt_leg(steps, sweep_points):
+def markov_melt_leg(steps, sweep_points, tune=True):
     """The large single system: 16,777,216 beads, 100 nm box, 256^3 mesh, 6
     Markov-typed types in 16-bead chains, 5 % charges with PME on the same
     mesh, respa_inner 1.  `steps` outer steps of the library's captured step
     (fused inner loop, field + PME slow forces), then a chi sweep of
     `sweep_points` perturbations on the frozen final positions: one moments
     pass, then per point the energy derivatives, the forces from the held
-    spectra and the gradient of the force loss through the adjoint chain."""
+    spectra and the gradient of the force loss through the adjoint chain.
+    With `tune` the autotuner chooses the mesh kernels' launch shapes for
+    this workload first (the library's normal set-up)."""
     import torch
 
     from hymd_amd import synth
@@ -315,7 +317,7 @@ def markov_melt_leg(steps, sweep_points):
     t0 = time.perf_counter()
     s = synth.markov_melt()
     gen_s = time.perf_counter() - t0
-    md = RespaMD(s)
+    md = RespaMD(s, autotune=tune)
     md.run(3)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -337,7 +339,8 @@ def markov_melt_leg(steps, sweep_points):
            "generate_s": round(gen_s, 1), "fused_inner_loop": md.fused, "steps": steps,
            "ms_per_outer_step": round(step_ms, 4), "particle_steps_per_s": s.n / (step_ms * 1e-3),
            "stage_ms": stages, "energies_last_step": {k: e[k] for k in ("field", "field_q", "bond", "angle",
-                                                                          "kinetic")}}
+                                                                          "kinetic")},
+           "autotune": None if md.tuning is None else {k: md.tuning[k] for k in ("choice", "default_ms", "tuned_ms")}}
     if sweep_points > 0:
         rng = np.random.default_rng(2004)
         x32 = md.x32.clone()
@@ -685,7 +688,7 @@ def main():
     if args.large_steps > 0 and not args.small:
         torch.cuda.empty_cache()
         log("markov melt: 16,777,216 beads, 256^3 mesh, PME")
-        large = markov_melt_leg(args.large_steps, args.sweep_points)
+        large = markov_melt_leg(args.large_steps, args.sweep_points, tune=not args.no_autotune)
         log(f"markov melt: {large['ms_per_outer_step']:.3f} ms per outer step"
             + (f", chi sweep {large['chi_sweep']['ms_per_point']:.2f} ms per point" if "chi_sweep" in large else ""))
 
