@@ -119,15 +119,20 @@ def test_field_after_box_move_matches_oracle(hip, geom):
             assert rms_rel(fl[2][t][d].cpu().numpy(), ref["force_mesh"][t][d]) < 1e-5
 
 
+@pytest.mark.parametrize("btype", ["berendsen", "scr"])
 @pytest.mark.parametrize("kind", ["isotropic", "semiisotropic"])
-def test_npt_steps_match_the_oracle(hip, oracle_built, kind):
-    """RespaMD.simulate with a Berendsen barostat every outer step, called
-    between the inner loop and the field update as main.py:889-932, against
-    the same steps composed from the FP64 oracle: kicks and inner loop, the
-    pressure from the velocities after the inner loop, the last inner step's
-    bonded virials and the previous field update's spectra, the barostat's
-    box and position scaling, the field update in the new box, the second
-    kick.  Box to 1e-6 relative, positions to 1e-5 nm, velocities to 1e-4."""
+def test_npt_steps_match_the_oracle(hip, oracle_built, kind, btype):
+    """RespaMD.simulate with a barostat every outer step, called between the
+    inner loop and the field update as main.py:889-932, against the same
+    steps composed from the FP64 oracle: kicks and inner loop, the pressure
+    from the velocities after the inner loop, the last inner step's bonded
+    virials and the previous field update's spectra, the barostat's box and
+    position scaling, the field update in the new box, the second kick.  SCR
+    takes its normal draws from the same generator; its target temperature
+    switches the CSVR thermostat on, here with tau = 1e30 (c = 1: the
+    identity, up to rounding), whose draws follow the barostat's in the
+    reference's order.  Box to 1e-6 relative, positions to 1e-5 nm,
+    velocities to 1e-4."""
     from hymd_amd import synth
     from hymd_amd.config import Config, Target_pressure
     from hymd_amd.md import RespaMD
@@ -149,10 +154,15 @@ def test_npt_steps_match_the_oracle(hip, oracle_built, kind):
     P_t = float(np.mean(p0[-3:])) * OB.BAR * 0.7
     cfg = Config(n_steps=3, time_step=dt, mesh_size=list(s.mesh), sigma=s.sigma, kappa=s.kappa, mass=m,
                  respa_inner=n_in, n_types=T, n_particles=s.n, box_size=s.box.copy(), barostat=kind,
-                 barostat_type="berendsen", tau_p=1.0, n_b=1, target_pressure=Target_pressure(P_L=P_t, P_N=P_t))
+                 barostat_type=btype, tau_p=1.0, n_b=1, target_pressure=Target_pressure(P_L=P_t, P_N=P_t))
+    if btype == "scr":
+        cfg.target_temperature = 300.0
+        cfg.tau = 1e30
     md = RespaMD(s)
     md.simulate(3, cfg, np.random.default_rng(0))
     torch.cuda.synchronize()
+    from oracle import thermostat as OT
+    rng = np.random.default_rng(0)
     # the oracle's steps
     box = s.box.copy()
     last = out
@@ -167,15 +177,23 @@ def test_npt_steps_match_the_oracle(hip, oracle_built, kind):
             st.f_fast = fb + fa
             st.v = OM.integrate_velocity(st.v, st.f_fast / m, dt)
         p = OP.comp_pressure(last["phi"], last["phi_fourier"], ham, st.v, m, s.mesh, box, s.sigma, bp, ap)
-        if kind == "isotropic":
+        if btype == "berendsen" and kind == "isotropic":
             OB.berendsen_isotropic(p, st.x, box, n_in * dt, 1, 1.0, P_t)
-        else:
+        elif btype == "berendsen":
             OB.berendsen_semiisotropic(p, st.x, box, n_in * dt, 1, 1.0, P_t, P_t, rows_quirk=False)
+        elif kind == "isotropic":
+            OB.scr_isotropic(p, st.x, box, n_in * dt, 1, 1.0, P_t, cfg.gas_constant, 300.0, rng.normal())
+        else:
+            rxy, rz = rng.normal(), rng.normal()
+            OB.scr_semiisotropic(p, st.x, box, n_in * dt, 1, 1.0, P_t, P_t, cfg.gas_constant, 300.0, rxy, rz,
+                                 rows_quirk=False)
         last = F.update_field(st.x, s.types, T, s.mesh, box, s.sigma, ham)
         f_field = F.compute_field_force(st.x, s.types, last["force_mesh"], s.mesh, box)
         st.v = OM.integrate_velocity(st.v, f_field / m, n_in * dt)
         st.f_slow = [f_field]
-    assert np.max(np.abs(box / s.box - 1.0)) > 2e-3          # the barostat moved the box
+        if btype == "scr":
+            OT.draw_csvr_numbers(rng, [s.n])     # the thermostat's draws (tau = 1e30: no change)
+    assert np.max(np.abs(box / s.box - 1.0)) > (2e-3 if btype == "berendsen" else 1e-5)   # the box moved
     np.testing.assert_allclose(md.box, box, rtol=1e-6)
     np.testing.assert_allclose(cfg.box_size, box, rtol=1e-6)
     np.testing.assert_allclose(md.pm.BoxSize, box, rtol=1e-6)
