@@ -54,23 +54,15 @@ __device__ __forceinline__ int find_range(const int64_t *__restrict__ off, int R
     return lo;
 }
 
-__device__ __forceinline__ int find_slot_by_row(const RepGeom *__restrict__ geo, int S, int row) {
-    int lo = 0, hi = S;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (geo[mid].row_off <= row) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // Particles are consecutive per replica, so a wavefront's 64 particles
-// nearly always belong to one replica: it is looked up once per wavefront
-// (uniform, scalar loads), and only the lanes past its end (a wavefront that
+// nearly always belong to one replica: one uniform load of the chunk table
+// (chunk_in[c] = (slot, end of its replica) of particle 64c, built with the
+// handle) gives it, and only the lanes past its end (a wavefront that
 // straddles two replicas) search on their own.
 __global__ void __launch_bounds__(kBlock) batch_key_kernel(const float *__restrict__ x, int64_t n,
                                                             const int64_t *__restrict__ poff, int R,
                                                             const int *__restrict__ slot_of,
+                                                            const int2 *__restrict__ chunk_in,
                                                             const RepGeom *__restrict__ geo, int2 *__restrict__ keys,
                                                             int *__restrict__ counts) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -80,10 +72,9 @@ __global__ void __launch_bounds__(kBlock) batch_key_kernel(const float *__restri
         const int wbase = __builtin_amdgcn_readfirstlane((int)(i0 + (threadIdx.x & ~63u)));
         int key = -1;
         if (wbase < n) {
-            const int r0 = find_range(poff, R, wbase);
-            const int64_t end0 = poff[r0 + 1];
-            const RepGeom *G = geo + slot_of[r0];
-            if (i < n && i >= end0) G = geo + slot_of[find_range(poff, R, i)];
+            const int2 ck = chunk_in[wbase >> 6];
+            const RepGeom *G = geo + ck.x;
+            if (i < n && i >= ck.y) G = geo + slot_of[find_range(poff, R, i)];
             if (i < n) {
                 const CicAxis cx = cic_axis((double)x[3 * i + 0], G->inv_h[0], G->mx);
                 const CicAxis cy = cic_axis((double)x[3 * i + 1], G->inv_h[1], G->my);
@@ -104,7 +95,8 @@ constexpr int kBatchRows = kBlock / 64;
 
 __global__ void __launch_bounds__(kBlock) batch_paint_rows_kernel(const float4 *__restrict__ rec,
                                                                    const int *__restrict__ start,
-                                                                   const RepGeom *__restrict__ geo, int S, int rows,
+                                                                   const RepGeom *__restrict__ geo,
+                                                                   const int *__restrict__ row_slot, int rows,
                                                                    int max_mz, PaintArgs pa,
                                                                    float *__restrict__ out) {
 #pragma clang fp contract(off)
@@ -114,7 +106,7 @@ __global__ void __launch_bounds__(kBlock) batch_paint_rows_kernel(const float4 *
     const int grow = (int)xcd_block(blockIdx.x, gridDim.x) * kBatchRows + wave;
     const bool active = grow < rows;
     unsigned long long *acc = acc_all + (int64_t)wave * T * max_mz;
-    const RepGeom *G = geo + find_slot_by_row(geo, S, active ? grow : rows - 1);
+    const RepGeom *G = geo + row_slot[active ? grow : rows - 1];
     const int row = grow - G->row_off;
     const int ix = row / G->my, iy = row - ix * G->my;
     const int mz = G->mz;
@@ -220,37 +212,31 @@ __global__ void __launch_bounds__(kBlock) batch_kspace_kernel(const float2 *__re
     }
 }
 
-// replica slot of binned particle p: slots hold consecutive global rows, so
-// their particles are consecutive in the binned order, from start[row_off]
-__device__ __forceinline__ int find_slot_by_particle(const RepGeom *__restrict__ geo, const int *__restrict__ start,
-                                                     int S, int p) {
-    int lo = 0, hi = S;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (start[geo[mid].row_off] <= p) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
-
 // Forces of the binned particles: each reads its replica's 3 force grids of
 // its type (the single-mesh readout's arithmetic), written to row perm[p].
-// The slot is looked up once per wavefront (its particles are consecutive).
+// Every particle is binned and slots hold consecutive global rows, so slot s
+// holds binned particles [pstart[s], pstart[s + 1]) whatever the positions:
+// the chunk table (chunk_binned[c] = (slot, pstart[slot + 1]) of binned
+// particle 64c, built with the handle) gives a wavefront's slot in one
+// uniform load, and lanes past its end step to the next slots.
 __global__ void __launch_bounds__(kBlock) batch_readout_kernel(const float *__restrict__ fgrid,
                                                                 const float4 *__restrict__ rec,
                                                                 const int *__restrict__ perm,
-                                                                const int *__restrict__ start,
-                                                                const RepGeom *__restrict__ geo, int S, int T,
+                                                                const int2 *__restrict__ chunk_binned,
+                                                                const int *__restrict__ pstart,
+                                                                const RepGeom *__restrict__ geo, int T,
                                                                 int64_t n, float *__restrict__ out) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t p0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x; p0 < n; p0 += stride) {
         const int64_t p = p0 + threadIdx.x;
         const int wbase = __builtin_amdgcn_readfirstlane((int)(p0 + (threadIdx.x & ~63u)));
         if (wbase >= n) continue;
-        const int s0 = find_slot_by_particle(geo, start, S, wbase);
-        const int end0 = (s0 + 1 < S) ? start[geo[s0 + 1].row_off] : (int)n;
+        const int2 ck = chunk_binned[wbase >> 6];
         if (p >= n) continue;
-        const RepGeom &G = geo[p < end0 ? s0 : find_slot_by_particle(geo, start, S, (int)p)];
+        int sl = ck.x;
+        if (p >= ck.y)
+            while (p >= pstart[sl + 1]) ++sl;
+        const RepGeom &G = geo[sl];
         const float4 r = rec[p];
         const int t = __float_as_int(r.w);
         const int64_t o = perm[p];
@@ -319,6 +305,7 @@ struct hymd_batch {
     std::vector<int64_t> poff;          // caller particle offsets (R + 1)
     std::vector<std::unique_ptr<RepGroup>> groups;
     DevBuf d_geo, d_slot_of, d_poff, d_sigma2, d_nb, d_row0;
+    DevBuf d_chunk_in, d_chunk_binned, d_pstart, d_row_slot;  // lookup tables (see the kernels)
     DevBuf keys, perm, xs, counts, start, tiles;  // xs: binned records (x, y, z, type bits)
     DevBuf rho, spec, partials, result;
     DevBuf grad, fgrid, d_coupling;  // force stages: 3T spectra and grids per replica, T x T coupling per slot
@@ -348,6 +335,7 @@ static int batch_enqueue(hymd_batch *b, const float *x, const int32_t *types, fl
     if (b->n > 0)
         batch_key_kernel<<<grid_for(b->n), kBlock, 0, s>>>(x, b->n, (const int64_t *)b->d_poff.ptr, b->R,
                                                            (const int *)b->d_slot_of.ptr,
+                                                           (const int2 *)b->d_chunk_in.ptr,
                                                            (const RepGeom *)b->d_geo.ptr, (int2 *)b->keys.ptr,
                                                            (int *)b->counts.ptr);
     HYMD_HIP(hipGetLastError());
@@ -357,7 +345,8 @@ static int batch_enqueue(hymd_batch *b, const float *x, const int32_t *types, fl
                                 (int *)b->perm.ptr, (float4 *)b->xs.ptr, nullptr, s));
     const int pblocks = (b->rows + kBatchRows - 1) / kBatchRows;
     batch_paint_rows_kernel<<<pblocks, kBlock, sizeof(long long) * kBatchRows * b->T * b->max_mz, s>>>(
-        (const float4 *)b->xs.ptr, (const int *)b->start.ptr, (const RepGeom *)b->d_geo.ptr, S, b->rows, b->max_mz,
+        (const float4 *)b->xs.ptr, (const int *)b->start.ptr, (const RepGeom *)b->d_geo.ptr,
+        (const int *)b->d_row_slot.ptr, b->rows, b->max_mz,
         b->pa, (float *)b->rho.ptr);
     HYMD_HIP(hipGetLastError());
     for (const auto &gp : b->groups) {
@@ -400,8 +389,10 @@ static int batch_enqueue(hymd_batch *b, const float *x, const int32_t *types, fl
     }
     if (forces && b->n > 0) {
         batch_readout_kernel<<<grid_for(b->n), kBlock, 0, s>>>((const float *)b->fgrid.ptr, (const float4 *)b->xs.ptr,
-                                                              (const int *)b->perm.ptr, (const int *)b->start.ptr,
-                                                              (const RepGeom *)b->d_geo.ptr, S, b->T, b->n, forces);
+                                                              (const int *)b->perm.ptr,
+                                                              (const int2 *)b->d_chunk_binned.ptr,
+                                                              (const int *)b->d_pstart.ptr,
+                                                              (const RepGeom *)b->d_geo.ptr, b->T, b->n, forces);
         HYMD_HIP(hipGetLastError());
     }
     HYMD_TRY(reduce_partials_batched((const double *)b->partials.ptr, (const int *)b->d_nb.ptr,
@@ -516,6 +507,29 @@ extern "C" int hymd_batch_create(hymd_batch **out, int32_t n_replicas, const int
         nbs[sl] = b->geo[sl].nb;
         row0[sl] = b->geo[sl].part_off;
     }
+    // lookup tables: per 64-particle chunk of the input order, the slot of its
+    // first particle and the end of that particle's replica; per slot, its
+    // first binned particle; per 64-particle chunk of the binned order, the
+    // slot of its first particle and that slot's end; per global row, its slot
+    const int64_t nchunks = (b->n + 63) / 64;
+    std::vector<int2> chunk_in((size_t)std::max<int64_t>(nchunks, 1)), chunk_binned(chunk_in.size());
+    std::vector<int> pstart(R + 1, 0), row_slot((size_t)std::max(rows, 1));
+    for (int sl = 0; sl < R; ++sl) {
+        const int r = order[sl];
+        pstart[sl + 1] = pstart[sl] + (int)(b->poff[r + 1] - b->poff[r]);
+        for (int q = 0; q < b->geo[sl].mx * b->geo[sl].my; ++q) row_slot[b->geo[sl].row_off + q] = sl;
+    }
+    for (int64_t c = 0, r = 0, sl = 0; c < nchunks; ++c) {
+        const int64_t i = 64 * c;
+        while (i >= b->poff[r + 1]) ++r;  // the replica holding particle i (skips empty ones)
+        while (i >= pstart[sl + 1]) ++sl;
+        chunk_in[c] = make_int2(b->slot_of[r], (int)b->poff[r + 1]);
+        chunk_binned[c] = make_int2((int)sl, pstart[sl + 1]);
+    }
+    HYMD_TRY(b->d_chunk_in.reserve(sizeof(int2) * chunk_in.size()));
+    HYMD_TRY(b->d_chunk_binned.reserve(sizeof(int2) * chunk_binned.size()));
+    HYMD_TRY(b->d_pstart.reserve(sizeof(int) * (R + 1)));
+    HYMD_TRY(b->d_row_slot.reserve(sizeof(int) * row_slot.size()));
     HYMD_TRY(b->d_geo.reserve(sizeof(RepGeom) * R));
     HYMD_TRY(b->d_slot_of.reserve(sizeof(int) * R));
     HYMD_TRY(b->d_poff.reserve(sizeof(int64_t) * (R + 1)));
@@ -541,6 +555,13 @@ extern "C" int hymd_batch_create(hymd_batch **out, int32_t n_replicas, const int
     HYMD_HIP(hipMemcpyAsync(b->d_slot_of.ptr, b->slot_of.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
     HYMD_HIP(hipMemcpyAsync(b->d_poff.ptr, b->poff.data(), sizeof(int64_t) * (R + 1), hipMemcpyHostToDevice, s));
     HYMD_HIP(hipMemcpyAsync(b->d_nb.ptr, nbs.data(), sizeof(int) * R, hipMemcpyHostToDevice, s));
+    HYMD_HIP(hipMemcpyAsync(b->d_chunk_in.ptr, chunk_in.data(), sizeof(int2) * chunk_in.size(), hipMemcpyHostToDevice,
+                            s));
+    HYMD_HIP(hipMemcpyAsync(b->d_chunk_binned.ptr, chunk_binned.data(), sizeof(int2) * chunk_binned.size(),
+                            hipMemcpyHostToDevice, s));
+    HYMD_HIP(hipMemcpyAsync(b->d_pstart.ptr, pstart.data(), sizeof(int) * (R + 1), hipMemcpyHostToDevice, s));
+    HYMD_HIP(hipMemcpyAsync(b->d_row_slot.ptr, row_slot.data(), sizeof(int) * row_slot.size(), hipMemcpyHostToDevice,
+                            s));
     HYMD_HIP(hipMemcpyAsync(b->d_row0.ptr, row0.data(), sizeof(int64_t) * R, hipMemcpyHostToDevice, s));
     HYMD_TRY(zero_words(b->counts.ptr, rows + 2, s));
     HYMD_HIP(hipStreamSynchronize(s));

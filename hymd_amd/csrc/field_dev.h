@@ -29,15 +29,16 @@ struct PaintArgs {
 // off); the 8th corner (1, 1, 1) takes the remainder, so every particle
 // deposits exactly round(m 2^32).  acc_row: the row's int64 accumulators in
 // LDS (integer adds: order-independent).
-__device__ __forceinline__ void row_deposit(const float4 &xp, int a, int b, double mass, double scale,
-                                            const double inv_h[3], int mx, int my, int mz,
-                                            unsigned long long *acc_row) {
+template <bool FAST>
+__device__ __forceinline__ void row_deposit_t(const float4 &xp, int a, int b, double mass, double scale,
+                                              const double inv_h[3], int mx, int my, int mz,
+                                              unsigned long long *acc_row) {
 #pragma clang fp contract(off)
     const CicAxis cx = cic_axis((double)xp.x, inv_h[0], mx);
     const CicAxis cy = cic_axis((double)xp.y, inv_h[1], my);
     const CicAxis cz = cic_axis((double)xp.z, inv_h[2], mz);
     const double wxy = (a ? cx.w1 : cx.w0) * (b ? cy.w1 : cy.w0);
-    const long long q0 = __double2ll_rn(((wxy * cz.w0) * mass) * scale);
+    const long long q0 = fixed_round<FAST>(((wxy * cz.w0) * mass) * scale);
     long long q1;
     if (a & b) {
         long long s7 = 0;
@@ -45,14 +46,23 @@ __device__ __forceinline__ void row_deposit(const float4 &xp, int a, int b, doub
         for (int e = 0; e < 7; ++e) {
             const int ea = e >> 2, eb = (e >> 1) & 1, ed = e & 1;
             const double wgt = ((ea ? cx.w1 : cx.w0) * (eb ? cy.w1 : cy.w0)) * (ed ? cz.w1 : cz.w0);
-            s7 += __double2ll_rn((wgt * mass) * scale);
+            s7 += fixed_round<FAST>((wgt * mass) * scale);
         }
-        q1 = __double2ll_rn(mass * scale) - s7;
+        q1 = fixed_round<FAST>(mass * scale) - s7;
     } else {
-        q1 = __double2ll_rn(((wxy * cz.w1) * mass) * scale);
+        q1 = fixed_round<FAST>(((wxy * cz.w1) * mass) * scale);
     }
     atomicAdd(acc_row + cz.i0, (unsigned long long)q0);
     atomicAdd(acc_row + cz.i1, (unsigned long long)q1);
+}
+
+// Every deposit is |w m 2^32| <= |m 2^32| (CIC weights and their products lie
+// in [0, 1]), so one test of the particle's mass picks the conversion.
+__device__ __forceinline__ void row_deposit(const float4 &xp, int a, int b, double mass, double scale,
+                                            const double inv_h[3], int mx, int my, int mz,
+                                            unsigned long long *acc_row) {
+    if (fixed_fast(mass * scale)) row_deposit_t<true>(xp, a, b, mass, scale, inv_h, mx, my, mz, acc_row);
+    else row_deposit_t<false>(xp, a, b, mass, scale, inv_h, mx, my, mz, acc_row);
 }
 
 // ---------------------------------------------------------------------------
