@@ -328,6 +328,7 @@ __global__ void __launch_bounds__(kBlock) fft_y_grad_kernel(const float2 *__rest
 // maps, runtime stage plans) or a fixed (My, Mz) with everything known at
 // compile time.
 struct PlaneRt {
+    static constexpr int nt = kPlaneThreads;
     int my, mz, nh, zh, npairs;
     PlaneDivs dv;
     FftAxis ay, azh;
@@ -344,28 +345,31 @@ struct PlaneRt {
     }
 };
 
-template <int MY, int MZ>
+// NT threads per workgroup: kPlaneThreads, or 1024 for planes of which one
+// workgroup fills a CU's LDS (160 x 160: 104 KB), so that a CU still holds
+// 16 waves.
+template <int MY, int MZ, int NT = kPlaneThreads>
 struct PlaneCt {
-    static constexpr int my = MY, mz = MZ, nh = MZ / 2, zh = MZ / 2 + 1, npairs = MZ / 4 + 1;
+    static constexpr int my = MY, mz = MZ, nh = MZ / 2, zh = MZ / 2 + 1, npairs = MZ / 4 + 1, nt = NT;
     __device__ int div_nh(int i) const { return i / nh; }
     __device__ int div_zh(int i) const { return i / zh; }
     __device__ int div_pairs(int i) const { return i / npairs; }
     // rows: lines of nh complex, element stride 1, line stride zh
     template <int SGN>
     __device__ void fft_z(float2 *D, const float2 *tw) const {
-        fft_inplace_ct<nh, MY, 1, zh, kPlaneThreads, SGN>(D, tw);
+        fft_inplace_ct<nh, MY, 1, zh, NT, SGN>(D, tw);
     }
     // columns: lines of My, element stride zh, line stride 1
     template <int SGN>
     __device__ void fft_y(float2 *D, const float2 *tw) const {
-        fft_inplace_ct<MY, zh, zh, 1, kPlaneThreads, SGN>(D, tw);
+        fft_inplace_ct<MY, zh, zh, 1, NT, SGN>(D, tw);
     }
 };
 
 // forward: My rows of Mz reals -> packed complex FFT of Mz/2 per row + split
 // (as r2c_z_kernel) -> FFT along y of the Mz/2+1 columns -> out plane
 template <class G>
-__global__ void __launch_bounds__(kPlaneThreads) r2c_zy_plane_kernel(const float *__restrict__ in,
+__global__ void __launch_bounds__(G::nt) r2c_zy_plane_kernel(const float *__restrict__ in,
                                                                       float2 *__restrict__ out, int mx, G g,
                                                                       const float2 *__restrict__ twy_g,
                                                                       const float2 *__restrict__ twz_g,
@@ -384,16 +388,16 @@ __global__ void __launch_bounds__(kPlaneThreads) r2c_zy_plane_kernel(const float
     // rows of Mz reals as nh complex, row stride zh in D
     const float2 *src = reinterpret_cast<const float2 *>(in) + plane * my * nh;
     const int nel = my * nh;
-    for (int i0 = threadIdx.x; i0 < nel; i0 += kPlaneThreads * kPlaneLoadBatch) {
+    for (int i0 = threadIdx.x; i0 < nel; i0 += G::nt * kPlaneLoadBatch) {
         float2 v[kPlaneLoadBatch];
 #pragma unroll
         for (int j = 0; j < kPlaneLoadBatch; ++j) {
-            const int idx = i0 + j * kPlaneThreads;
+            const int idx = i0 + j * G::nt;
             v[j] = idx < nel ? src[idx] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int j = 0; j < kPlaneLoadBatch; ++j) {
-            const int idx = i0 + j * kPlaneThreads;
+            const int idx = i0 + j * G::nt;
             if (idx < nel) {
                 const int y = g.div_nh(idx);
                 D[y * zh + (idx - y * nh)] = v[j];
@@ -443,7 +447,7 @@ struct GradK {
 __device__ __forceinline__ float2 mul_minus_i(float2 v, float kd) { return make_float2(kd * v.y, -kd * v.x); }
 
 template <class G, bool GRAD>
-__global__ void __launch_bounds__(kPlaneThreads) c2r_yz_plane_kernel(const float2 *__restrict__ in,
+__global__ void __launch_bounds__(G::nt) c2r_yz_plane_kernel(const float2 *__restrict__ in,
                                                                       float *__restrict__ out, int mx, G g,
                                                                       const float2 *__restrict__ twy_g,
                                                                       const float2 *__restrict__ twz_g,
@@ -471,16 +475,16 @@ __global__ void __launch_bounds__(kPlaneThreads) c2r_yz_plane_kernel(const float
     }
     const float2 *src = in + in_plane * my * zh;
     const int nel = my * zh;
-    for (int i0 = threadIdx.x; i0 < nel; i0 += kPlaneThreads * kPlaneLoadBatch) {
+    for (int i0 = threadIdx.x; i0 < nel; i0 += G::nt * kPlaneLoadBatch) {
         float2 v[kPlaneLoadBatch];
 #pragma unroll
         for (int j = 0; j < kPlaneLoadBatch; ++j) {
-            const int idx = i0 + j * kPlaneThreads;
+            const int idx = i0 + j * G::nt;
             v[j] = idx < nel ? src[idx] : make_float2(0.f, 0.f);
         }
 #pragma unroll
         for (int j = 0; j < kPlaneLoadBatch; ++j) {
-            const int idx = i0 + j * kPlaneThreads;
+            const int idx = i0 + j * G::nt;
             if (idx < nel) {
                 float2 w = v[j];
                 if (GRAD && d == 1) {  // row y: k_y, zero on the Nyquist row
@@ -586,8 +590,9 @@ static int with_plane_geom(const FftPlan &plan, F &&f) {
     HYMD_PLANE_CT(64, 64)
     HYMD_PLANE_CT(96, 96)
     HYMD_PLANE_CT(128, 128)
-    HYMD_PLANE_CT(160, 160)
 #undef HYMD_PLANE_CT
+    if (m.my == 160 && m.mz == 160 && same_plan(plan.ay, CtPlan<160>::v) && same_plan(plan.azh, CtPlan<80>::v))
+        return f(PlaneCt<160, 160, 1024>{});
     return f(PlaneRt{m.my, m.mz, m.mz / 2, m.zh, m.mz / 4 + 1, plan.pdiv, plan.ay, plan.azh});
 }
 
@@ -606,7 +611,7 @@ int fft_r2c_zy(const FftPlan &plan, const float *in, float2 *out, int batch, hip
             using G = decltype(g);
             HYMD_TRY(plane_lds_opt_in<G>());
             dim3 grid((unsigned)m.mx, (unsigned)batch);
-            r2c_zy_plane_kernel<G><<<grid, kPlaneThreads, plan.lds_plane, s>>>(
+            r2c_zy_plane_kernel<G><<<grid, G::nt, plan.lds_plane, s>>>(
                 in, out, m.mx, g, (const float2 *)plan.tw_y.ptr, (const float2 *)plan.tw_zh.ptr,
                 (const float2 *)plan.post.ptr);
             HYMD_HIP(hipGetLastError());
@@ -634,7 +639,7 @@ int fft_c2r_yz(const FftPlan &plan, float2 *in, float *out, int batch, float sca
             using G = decltype(g);
             HYMD_TRY(plane_lds_opt_in<G>());
             dim3 grid((unsigned)m.mx, (unsigned)batch);
-            c2r_yz_plane_kernel<G, false><<<grid, kPlaneThreads, plan.lds_plane, s>>>(
+            c2r_yz_plane_kernel<G, false><<<grid, G::nt, plan.lds_plane, s>>>(
                 in, out, m.mx, g, (const float2 *)plan.tw_y.ptr, (const float2 *)plan.tw_zh.ptr,
                 (const float2 *)plan.post.ptr, scale, GradK{});
             HYMD_HIP(hipGetLastError());
@@ -662,7 +667,7 @@ int fft_c2r_yz_grad(const FftPlan &plan, const float2 *in, float *out, int T, do
         using G = decltype(g);
         HYMD_TRY(plane_lds_opt_in<G>());
         dim3 grid((unsigned)m.mx, (unsigned)(3 * T));
-        c2r_yz_plane_kernel<G, true><<<grid, kPlaneThreads, plan.lds_plane, s>>>(
+        c2r_yz_plane_kernel<G, true><<<grid, G::nt, plan.lds_plane, s>>>(
             in, out, m.mx, g, (const float2 *)plan.tw_y.ptr, (const float2 *)plan.tw_zh.ptr,
             (const float2 *)plan.post.ptr, scale, GradK{T, kfy, kfz});
         HYMD_HIP(hipGetLastError());
