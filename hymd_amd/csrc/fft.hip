@@ -328,7 +328,7 @@ __global__ void __launch_bounds__(kBlock) fft_y_grad_kernel(const float2 *__rest
 // maps, runtime stage plans) or a fixed (My, Mz) with everything known at
 // compile time.
 struct PlaneRt {
-    static constexpr int nt = kPlaneThreads;
+    static constexpr int nt = kPlaneThreads, wpe = 1;
     int my, mz, nh, zh, npairs;
     PlaneDivs dv;
     FftAxis ay, azh;
@@ -345,12 +345,15 @@ struct PlaneRt {
     }
 };
 
-// NT threads per workgroup: kPlaneThreads, or 1024 for planes of which one
-// workgroup fills a CU's LDS (160 x 160: 104 KB), so that a CU still holds
-// 16 waves.
-template <int MY, int MZ, int NT = kPlaneThreads>
+// NT threads per workgroup and at least WPE waves per SIMD (the register
+// budget): kPlaneThreads, or 1024 threads where that keeps more waves on a
+// CU.  160 x 160 (a 104 KB plane: one workgroup per CU) runs 16 waves per CU
+// instead of 8; 128 x 128 (66.5 KB: two per CU) runs 32 instead of 16 when
+// held to 64 VGPRs (WPE 8; the c2r pass then keeps 12 B/lane in scratch and
+// is still faster: 0.3637 -> 0.3616 ms per lipid/water outer step).
+template <int MY, int MZ, int NT = kPlaneThreads, int WPE = 1>
 struct PlaneCt {
-    static constexpr int my = MY, mz = MZ, nh = MZ / 2, zh = MZ / 2 + 1, npairs = MZ / 4 + 1, nt = NT;
+    static constexpr int my = MY, mz = MZ, nh = MZ / 2, zh = MZ / 2 + 1, npairs = MZ / 4 + 1, nt = NT, wpe = WPE;
     __device__ int div_nh(int i) const { return i / nh; }
     __device__ int div_zh(int i) const { return i / zh; }
     __device__ int div_pairs(int i) const { return i / npairs; }
@@ -369,7 +372,7 @@ struct PlaneCt {
 // forward: My rows of Mz reals -> packed complex FFT of Mz/2 per row + split
 // (as r2c_z_kernel) -> FFT along y of the Mz/2+1 columns -> out plane
 template <class G>
-__global__ void __launch_bounds__(G::nt) r2c_zy_plane_kernel(const float *__restrict__ in,
+__global__ void __launch_bounds__(G::nt, G::wpe) r2c_zy_plane_kernel(const float *__restrict__ in,
                                                                       float2 *__restrict__ out, int mx, G g,
                                                                       const float2 *__restrict__ twy_g,
                                                                       const float2 *__restrict__ twz_g,
@@ -447,7 +450,7 @@ struct GradK {
 __device__ __forceinline__ float2 mul_minus_i(float2 v, float kd) { return make_float2(kd * v.y, -kd * v.x); }
 
 template <class G, bool GRAD>
-__global__ void __launch_bounds__(G::nt) c2r_yz_plane_kernel(const float2 *__restrict__ in,
+__global__ void __launch_bounds__(G::nt, G::wpe) c2r_yz_plane_kernel(const float2 *__restrict__ in,
                                                                       float *__restrict__ out, int mx, G g,
                                                                       const float2 *__restrict__ twy_g,
                                                                       const float2 *__restrict__ twz_g,
@@ -589,8 +592,9 @@ static int with_plane_geom(const FftPlan &plan, F &&f) {
     HYMD_PLANE_CT(48, 48)
     HYMD_PLANE_CT(64, 64)
     HYMD_PLANE_CT(96, 96)
-    HYMD_PLANE_CT(128, 128)
 #undef HYMD_PLANE_CT
+    if (m.my == 128 && m.mz == 128 && same_plan(plan.ay, CtPlan<128>::v) && same_plan(plan.azh, CtPlan<64>::v))
+        return f(PlaneCt<128, 128, 1024, 8>{});
     if (m.my == 160 && m.mz == 160 && same_plan(plan.ay, CtPlan<160>::v) && same_plan(plan.azh, CtPlan<80>::v))
         return f(PlaneCt<160, 160, 1024>{});
     return f(PlaneRt{m.my, m.mz, m.mz / 2, m.zh, m.mz / 4 + 1, plan.pdiv, plan.ay, plan.azh});
