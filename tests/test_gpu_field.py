@@ -264,12 +264,12 @@ def test_fused_xline_pass_equals_unfused(hip, mesh, T):
 
 
 @pytest.mark.parametrize("mesh,T", [((20, 24, 18), 3), ((16, 16, 16), 2), ((12, 10, 8), 5), ((24, 20, 30), 8),
-                                    ((6, 6, 144), 8)])
+                                    ((6, 6, 144), 8), ((12, 256, 256), 6), ((8, 200, 256), 4)])
 def test_readout_matches_oracle(hip, mesh, T):
     """compute_field_force on arbitrary grids against the oracle's CIC readout
-    (field.py:317-330), with types out of range giving zero: the row-staged
-    readout (4-aligned Mz with vector loads, other Mz with scalar loads) and
-    the per-particle one (T * Mz past the row kernel's LDS budget) alike."""
+    (field.py:317-330), with types out of range giving zero, on meshes up to
+    256 x 256 y-z planes (where two x-planes of the 3T force grids exceed an
+    XCD's 4 MB L2)."""
     rng = np.random.default_rng(sum(mesh) + T)
     box = (3.0, 4.5, 2.5)
     n = 5000
