@@ -7,8 +7,9 @@ behind the C-ABI in include/hymd_amd.h):
                                compute_field_force, update_field_force_q,
                                compute_self_energy_q, compute_field_and_kinetic_energy
 * ``hymd_amd.hamiltonian``    hamiltonian.py: DefaultNoChi, DefaultWithChi, SquaredPhi
-* ``hymd_amd.force``          force.py compiled kernels: compute_*_forces__fortran(+__double),
-                               dipole_forces_redistribution; ``force_kernels`` = cbf/caf/cdf(+_d)
+* ``hymd_amd.force_kernels``  the f2py module force.py binds: cbf/caf/cdf(+_d) on the HIP kernels
+* ``hymd_amd.force``          force.py: compute_*_forces__fortran(+__double) (bound from
+                               force_kernels, as force.py:8-23), dipole_forces_redistribution
 * ``hymd_amd.md``             the rRESPA loop of main.py on the device (+ thermostat / COM coupling)
 * ``hymd_amd.topology``       force.py: prepare_bonds (connectivity table or TOML topology)
 * ``hymd_amd.thermostat``     thermostat.py: csvr_thermostat, cancel_com_momentum
