@@ -109,9 +109,10 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb) {
 }
 
 // np.mod(x, L) for L > 0 as numpy computes it: fmod, negatives shifted up by
-// L (so a tiny negative x gives L itself), zero returned as +0.  After a
-// drift x lies in [-L, 2L), where x - L (exact by Sterbenz) and x + L are
-// those values; anything else takes the general route.
+// L (so a tiny negative x gives L itself), zero returned as +0 (r + 0.0 turns
+// -0 into +0 and leaves every other value as it is).  After a drift x lies
+// in [-L, 2L), where x - L (exact by Sterbenz) and x + L are those values;
+// anything else takes the general route.
 __device__ __forceinline__ double np_mod(double x, double L) {
     double r;
     if (x >= 0.0 && x < L) {
@@ -124,7 +125,7 @@ __device__ __forceinline__ double np_mod(double x, double L) {
         r = fmod(x, L);
         if (r != 0.0 && r < 0.0) r += L;
     }
-    return r == 0.0 ? 0.0 : r;
+    return r + 0.0;
 }
 
 // Rank of this lane's item among the items of the same bin, counted in

@@ -171,9 +171,7 @@ fused_inner_kernel(FusedArgs A) {
             const int j = threadIdx.x + k * kBlock;
             if (j >= np) continue;
             for (int d = 0; d < 3; ++d) {
-                double vi = v[k][d];
-                if (it > 0) vi = vi + A.h_in * (double)ff[k][d];
-                vi = vi + A.h_in * (double)ff[k][d];
+                const double vi = v[k][d] + A.h_in * (double)ff[k][d];
                 v[k][d] = vi;
                 xs[3 * j + d] = np_mod(xs[3 * j + d] + A.dt * vi, A.box[d]);
             }
@@ -285,11 +283,16 @@ fused_inner_kernel(FusedArgs A) {
                     for (int d = 0; d < 3; ++d) acc[d] += sd[12 * t + 3 * role + d];
                 }
             }
-            for (int d = 0; d < 3; ++d) ff[k][d] = (float)acc[d];
+            // the sub-step's second half kick (the next sub-step's first
+            // adds the same term again: two FP64 additions, as before)
+            for (int d = 0; d < 3; ++d) {
+                ff[k][d] = (float)acc[d];
+                v[k][d] = v[k][d] + A.h_in * (double)ff[k][d];
+            }
         }
         // slots are rewritten only after the next sub-step's barrier
     }
-    // second half kick of the last sub-step; write back
+    // write back (the last sub-step's second half kick is in v)
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         const int j = threadIdx.x + k * kBlock;
@@ -297,7 +300,7 @@ fused_inner_kernel(FusedArgs A) {
         const int64_t g = 3 * (int64_t)(p0 + j);
         for (int d = 0; d < 3; ++d) {
             const double xi = xs[3 * j + d];
-            A.v[g + d] = v[k][d] + A.h_in * (double)ff[k][d];
+            A.v[g + d] = v[k][d];
             A.x[g + d] = xi;
             A.x32[g + d] = (float)xi;
             A.f_fast[g + d] = ff[k][d];
