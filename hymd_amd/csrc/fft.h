@@ -373,6 +373,32 @@ __device__ __forceinline__ void stockham_inplace_ct(float2 *buf, const float2 *_
     __syncthreads();
 }
 
+// Stage 0 of fft_inplace_ct (P = 1, no twiddles) with its inputs from
+// ld(line, element) (global memory) instead of buf: the load pass of the
+// plane kernels and the first LDS round trip become one.  buf is only
+// written, then a barrier; fft_inplace_ct<..., 1> runs the other stages.
+template <int N, int NLINES, int SE, int SL, int NT, int SGN, class Load>
+__device__ __forceinline__ void fft_first_ct(float2 *buf, Load ld) {
+    constexpr StagePlan sp = CtPlan<N>::v;
+    static_assert(sp.ns >= 1, "no stage plan");
+    constexpr int R = sp.r[0], NR = N / R, TOTAL = NLINES * NR, KB = (TOTAL + NT - 1) / NT;
+    constexpr float sgn = (float)SGN;
+    float2 u[KB][8];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+        const int b = threadIdx.x + kb * NT;
+        if (TOTAL % NT != 0 && b >= TOTAL) continue;
+        const int i = b / NLINES, l = b - i * NLINES;
+#pragma unroll
+        for (int j = 0; j < R; ++j) u[kb][j] = ld(l, i + j * NR);
+        small_dft<R>(u[kb], sgn);
+        float2 *dst = buf + l * SL + i * R * SE;
+#pragma unroll
+        for (int j = 0; j < R; ++j) dst[j * SE] = u[kb][j];
+    }
+    __syncthreads();
+}
+
 template <int N, int NLINES, int SE, int SL, int NT, int SGN, int S = 0>
 __device__ __forceinline__ void fft_inplace_ct(float2 *buf, const float2 *__restrict__ tw) {
     constexpr StagePlan sp = CtPlan<N>::v;

@@ -329,6 +329,7 @@ __global__ void __launch_bounds__(kBlock) fft_y_grad_kernel(const float2 *__rest
 // compile time.
 struct PlaneRt {
     static constexpr int nt = kPlaneThreads, wpe = 1;
+    static constexpr bool kCt = false;
     int my, mz, nh, zh, npairs;
     PlaneDivs dv;
     FftAxis ay, azh;
@@ -367,6 +368,13 @@ struct PlaneCt {
     __device__ void fft_y(float2 *D, const float2 *tw) const {
         fft_inplace_ct<MY, zh, zh, 1, NT, SGN>(D, tw);
     }
+    // the same with the first stage's inputs from ld(kz, y)
+    template <int SGN, class Load>
+    __device__ void fft_y_from(float2 *D, const float2 *tw, Load ld) const {
+        fft_first_ct<MY, zh, zh, 1, NT, SGN>(D, ld);
+        fft_inplace_ct<MY, zh, zh, 1, NT, SGN, 1>(D, tw);
+    }
+    static constexpr bool kCt = true;
 };
 
 // forward: My rows of Mz reals -> packed complex FFT of Mz/2 per row + split
@@ -477,30 +485,42 @@ __global__ void __launch_bounds__(G::nt, G::wpe) c2r_yz_plane_kernel(const float
         in_plane = out_plane = (int64_t)(flat / gridDim.x) * mx + (flat % gridDim.x);
     }
     const float2 *src = in + in_plane * my * zh;
-    const int nel = my * zh;
-    for (int i0 = threadIdx.x; i0 < nel; i0 += G::nt * kPlaneLoadBatch) {
-        float2 v[kPlaneLoadBatch];
+    if constexpr (G::kCt) {
+        // the y pass's first stage reads the plane from global memory
+        g.template fft_y_from<1>(D, twy, [&](int kz, int y) -> float2 {
+            float2 w = src[y * zh + kz];
+            if (GRAD && d == 1) {  // row y: k_y, zero on the Nyquist row
+                const int ny = 2 * y < my ? y : y - my;
+                w = mul_minus_i(w, 2 * y == my ? 0.f : (float)(gk.kfy * ny));
+            }
+            return w;
+        });
+    } else {
+        const int nel = my * zh;
+        for (int i0 = threadIdx.x; i0 < nel; i0 += G::nt * kPlaneLoadBatch) {
+            float2 v[kPlaneLoadBatch];
 #pragma unroll
-        for (int j = 0; j < kPlaneLoadBatch; ++j) {
-            const int idx = i0 + j * G::nt;
-            v[j] = idx < nel ? src[idx] : make_float2(0.f, 0.f);
-        }
+            for (int j = 0; j < kPlaneLoadBatch; ++j) {
+                const int idx = i0 + j * G::nt;
+                v[j] = idx < nel ? src[idx] : make_float2(0.f, 0.f);
+            }
 #pragma unroll
-        for (int j = 0; j < kPlaneLoadBatch; ++j) {
-            const int idx = i0 + j * G::nt;
-            if (idx < nel) {
-                float2 w = v[j];
-                if (GRAD && d == 1) {  // row y: k_y, zero on the Nyquist row
-                    const int y = g.div_zh(idx);
-                    const int ny = 2 * y < my ? y : y - my;
-                    w = mul_minus_i(w, 2 * y == my ? 0.f : (float)(gk.kfy * ny));
+            for (int j = 0; j < kPlaneLoadBatch; ++j) {
+                const int idx = i0 + j * G::nt;
+                if (idx < nel) {
+                    float2 w = v[j];
+                    if (GRAD && d == 1) {  // row y: k_y, zero on the Nyquist row
+                        const int y = g.div_zh(idx);
+                        const int ny = 2 * y < my ? y : y - my;
+                        w = mul_minus_i(w, 2 * y == my ? 0.f : (float)(gk.kfy * ny));
+                    }
+                    D[idx] = w;
                 }
-                D[idx] = w;
             }
         }
+        __syncthreads();
+        g.template fft_y<1>(D, twy);
     }
-    __syncthreads();
-    g.template fft_y<1>(D, twy);
     // pre-process in place, pairs (k, nh - k) per thread; k = 0 reads X[nh]
     const int npairs = g.npairs;
     for (int idx = threadIdx.x; idx < my * npairs; idx += blockDim.x) {
