@@ -377,7 +377,10 @@ __device__ __forceinline__ void stockham_inplace_ct(float2 *buf, const float2 *_
 // ld(line, element) (global memory) instead of buf: the load pass of the
 // plane kernels and the first LDS round trip become one.  buf is only
 // written, then a barrier; fft_inplace_ct<..., 1> runs the other stages.
-template <int N, int NLINES, int SE, int SL, int NT, int SGN, class Load>
+// I_FAST: consecutive threads take consecutive butterflies of one line
+// (the global reads of a line stored contiguously coalesce), else
+// consecutive lines (lines stored side by side).
+template <int N, int NLINES, int SE, int SL, int NT, int SGN, bool I_FAST = false, class Load>
 __device__ __forceinline__ void fft_first_ct(float2 *buf, Load ld) {
     constexpr StagePlan sp = CtPlan<N>::v;
     static_assert(sp.ns >= 1, "no stage plan");
@@ -388,7 +391,7 @@ __device__ __forceinline__ void fft_first_ct(float2 *buf, Load ld) {
     for (int kb = 0; kb < KB; ++kb) {
         const int b = threadIdx.x + kb * NT;
         if (TOTAL % NT != 0 && b >= TOTAL) continue;
-        const int i = b / NLINES, l = b - i * NLINES;
+        const int i = I_FAST ? b % NR : b / NLINES, l = I_FAST ? b / NR : b - i * NLINES;
 #pragma unroll
         for (int j = 0; j < R; ++j) u[kb][j] = ld(l, i + j * NR);
         small_dft<R>(u[kb], sgn);
@@ -399,12 +402,47 @@ __device__ __forceinline__ void fft_first_ct(float2 *buf, Load ld) {
     __syncthreads();
 }
 
-template <int N, int NLINES, int SE, int SL, int NT, int SGN, int S = 0>
+template <int N, int NLINES, int SE, int SL, int NT, int SGN, int S = 0, int SEND = CtPlan<N>::v.ns>
 __device__ __forceinline__ void fft_inplace_ct(float2 *buf, const float2 *__restrict__ tw) {
     constexpr StagePlan sp = CtPlan<N>::v;
-    if constexpr (S < sp.ns) {
+    if constexpr (S < SEND) {
         stockham_inplace_ct<sp.r[S], sp.p[S], N, NLINES, SE, SL, NT, SGN>(buf, tw + sp.twoff[S]);
-        fft_inplace_ct<N, NLINES, SE, SL, NT, SGN, S + 1>(buf, tw);
+        fft_inplace_ct<N, NLINES, SE, SL, NT, SGN, S + 1, SEND>(buf, tw);
+    }
+}
+
+// The last stage of fft_inplace_ct (q = 0: output element k + j P) with its
+// outputs to st(line, element, value) (global memory) instead of buf: the
+// store pass and the last LDS round trip become one.  buf is only read.
+template <int N, int NLINES, int SE, int SL, int NT, int SGN, bool I_FAST, class Store>
+__device__ __forceinline__ void fft_last_ct(const float2 *buf, const float2 *__restrict__ tw, Store st) {
+    constexpr StagePlan sp = CtPlan<N>::v;
+    static_assert(sp.ns >= 1, "no stage plan");
+    constexpr int S = sp.ns - 1, R = sp.r[S], P = sp.p[S], NR = N / R, TOTAL = NLINES * NR,
+                  KB = (TOTAL + NT - 1) / NT;
+    static_assert(P == NR, "the last stage spans the line");
+    constexpr float sgn = (float)SGN;
+    const float2 *tws = tw + sp.twoff[S];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+        const int b = threadIdx.x + kb * NT;
+        if (TOTAL % NT != 0 && b >= TOTAL) continue;
+        const int i = I_FAST ? b % NR : b / NLINES, l = I_FAST ? b / NR : b - i * NLINES;
+        const float2 *src = buf + l * SL + i * SE;
+        float2 u[8];
+#pragma unroll
+        for (int j = 0; j < R; ++j) u[j] = src[j * NR * SE];
+        if (P > 1) {
+#pragma unroll
+            for (int j = 1; j < R; ++j) {
+                float2 w = tws[i * (R - 1) + j - 1];
+                if (SGN > 0) w.y = -w.y;
+                u[j] = cmul(u[j], w);
+            }
+        }
+        small_dft<R>(u, sgn);
+#pragma unroll
+        for (int j = 0; j < R; ++j) st(l, i + j * P, u[j]);
     }
 }
 

@@ -363,6 +363,19 @@ struct PlaneCt {
     __device__ void fft_z(float2 *D, const float2 *tw) const {
         fft_inplace_ct<nh, MY, 1, zh, NT, SGN>(D, tw);
     }
+    // the same with the first stage's inputs from ld(y, k) (rows contiguous
+    // in global memory: consecutive threads take consecutive k)
+    template <int SGN, class Load>
+    __device__ void fft_z_from(float2 *D, const float2 *tw, Load ld) const {
+        fft_first_ct<nh, MY, 1, zh, NT, SGN, true>(D, ld);
+        fft_inplace_ct<nh, MY, 1, zh, NT, SGN, 1>(D, tw);
+    }
+    // the same with the last stage's outputs to st(y, k, value)
+    template <int SGN, class Store>
+    __device__ void fft_z_to(float2 *D, const float2 *tw, Store st) const {
+        fft_inplace_ct<nh, MY, 1, zh, NT, SGN, 0, CtPlan<nh>::v.ns - 1>(D, tw);
+        fft_last_ct<nh, MY, 1, zh, NT, SGN, true>(D, tw, st);
+    }
     // columns: lines of My, element stride zh, line stride 1
     template <int SGN>
     __device__ void fft_y(float2 *D, const float2 *tw) const {
@@ -373,6 +386,12 @@ struct PlaneCt {
     __device__ void fft_y_from(float2 *D, const float2 *tw, Load ld) const {
         fft_first_ct<MY, zh, zh, 1, NT, SGN>(D, ld);
         fft_inplace_ct<MY, zh, zh, 1, NT, SGN, 1>(D, tw);
+    }
+    // the same with the last stage's outputs to st(kz, y, value)
+    template <int SGN, class Store>
+    __device__ void fft_y_to(float2 *D, const float2 *tw, Store st) const {
+        fft_inplace_ct<MY, zh, zh, 1, NT, SGN, 0, CtPlan<MY>::v.ns - 1>(D, tw);
+        fft_last_ct<MY, zh, zh, 1, NT, SGN, false>(D, tw, st);
     }
     static constexpr bool kCt = true;
 };
@@ -398,25 +417,30 @@ __global__ void __launch_bounds__(G::nt, G::wpe) r2c_zy_plane_kernel(const float
     const int64_t plane = (int64_t)(flat / gridDim.x) * mx + (flat % gridDim.x);  // grid * mx + x
     // rows of Mz reals as nh complex, row stride zh in D
     const float2 *src = reinterpret_cast<const float2 *>(in) + plane * my * nh;
-    const int nel = my * nh;
-    for (int i0 = threadIdx.x; i0 < nel; i0 += G::nt * kPlaneLoadBatch) {
-        float2 v[kPlaneLoadBatch];
+    if constexpr (G::kCt) {
+        // the z pass's first stage reads the rows from global memory
+        g.template fft_z_from<-1>(D, twz, [&](int y, int k) -> float2 { return src[y * nh + k]; });
+    } else {
+        const int nel = my * nh;
+        for (int i0 = threadIdx.x; i0 < nel; i0 += G::nt * kPlaneLoadBatch) {
+            float2 v[kPlaneLoadBatch];
 #pragma unroll
-        for (int j = 0; j < kPlaneLoadBatch; ++j) {
-            const int idx = i0 + j * G::nt;
-            v[j] = idx < nel ? src[idx] : make_float2(0.f, 0.f);
-        }
+            for (int j = 0; j < kPlaneLoadBatch; ++j) {
+                const int idx = i0 + j * G::nt;
+                v[j] = idx < nel ? src[idx] : make_float2(0.f, 0.f);
+            }
 #pragma unroll
-        for (int j = 0; j < kPlaneLoadBatch; ++j) {
-            const int idx = i0 + j * G::nt;
-            if (idx < nel) {
-                const int y = g.div_nh(idx);
-                D[y * zh + (idx - y * nh)] = v[j];
+            for (int j = 0; j < kPlaneLoadBatch; ++j) {
+                const int idx = i0 + j * G::nt;
+                if (idx < nel) {
+                    const int y = g.div_nh(idx);
+                    D[y * zh + (idx - y * nh)] = v[j];
+                }
             }
         }
+        __syncthreads();
+        g.template fft_z<-1>(D, twz);
     }
-    __syncthreads();
-    g.template fft_z<-1>(D, twz);
     // split in place: the thread of pair (k, nh - k) writes X[k] and X[nh - k]
     // (k = 0 writes X[0] and X[nh]; k = nh/2 is its own pair)
     const int npairs = g.npairs;
@@ -437,9 +461,14 @@ __global__ void __launch_bounds__(G::nt, G::wpe) r2c_zy_plane_kernel(const float
         if (km != k) row[km] = xm;
     }
     __syncthreads();
-    g.template fft_y<-1>(D, twy);
     float2 *dst = out + plane * my * zh;
-    for (int idx = threadIdx.x; idx < my * zh; idx += blockDim.x) dst[idx] = D[idx];
+    if constexpr (G::kCt) {
+        // the y pass's last stage writes the plane to global memory
+        g.template fft_y_to<-1>(D, twy, [&](int kz, int y, float2 v) { dst[y * zh + kz] = v; });
+    } else {
+        g.template fft_y<-1>(D, twy);
+        for (int idx = threadIdx.x; idx < my * zh; idx += blockDim.x) dst[idx] = D[idx];
+    }
 }
 
 // inverse: FFT along y of the Mz/2+1 columns -> per row Z'[k] = (X[k] + X*[nh-k])
@@ -543,12 +572,19 @@ __global__ void __launch_bounds__(G::nt, G::wpe) c2r_yz_plane_kernel(const float
         row[k] = zk;
     }
     __syncthreads();
-    g.template fft_z<1>(D, twz);
     float2 *dst = reinterpret_cast<float2 *>(out) + out_plane * my * nh;
-    for (int idx = threadIdx.x; idx < my * nh; idx += blockDim.x) {
-        const int y = g.div_nh(idx);
-        const float2 v = D[y * zh + (idx - y * nh)];
-        dst[idx] = make_float2(v.x * scale, v.y * scale);
+    if constexpr (G::kCt) {
+        // the z pass's last stage writes the rows to global memory
+        g.template fft_z_to<1>(D, twz, [&](int y, int k, float2 v) {
+            dst[y * nh + k] = make_float2(v.x * scale, v.y * scale);
+        });
+    } else {
+        g.template fft_z<1>(D, twz);
+        for (int idx = threadIdx.x; idx < my * nh; idx += blockDim.x) {
+            const int y = g.div_nh(idx);
+            const float2 v = D[y * zh + (idx - y * nh)];
+            dst[idx] = make_float2(v.x * scale, v.y * scale);
+        }
     }
 }
 
