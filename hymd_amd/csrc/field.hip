@@ -534,6 +534,7 @@ __global__ void __launch_bounds__(kBlock) field_xline_kernel(const float2 *__res
 // multiply-shift and the Stockham stages of fft_inplace_ct.
 template <int ELEMS>
 struct XlineRt {
+    static constexpr bool kCt = false;
     int n, W, wshift, T, ld;
     FastDiv ndiv, flines;
     FftAxis ax;
@@ -548,10 +549,23 @@ template <int T_, int N, int W_>
 struct XlineCt {
     static constexpr int n = N, W = W_, T = T_, ld = N + 1;
     static constexpr int wshift = W_ == 1 ? 0 : W_ == 2 ? 1 : W_ == 4 ? 2 : W_ == 8 ? 3 : 4;
+    static constexpr bool kCt = true;
     __device__ int div_n(int i) const { return i / N; }
     template <int SGN, int NT>
     __device__ void fft(float2 *D, const float2 *tw) const {
         fft_inplace_ct<N, T_ * W_, 1, N + 1, NT, SGN>(D, tw);
+    }
+    // the same with the first stage's inputs from ld(line, e) / the last
+    // stage's outputs to st(line, e, value), line = t W + l
+    template <int SGN, int NT, class Load>
+    __device__ void fft_from(float2 *D, const float2 *tw, Load ld) const {
+        fft_first_ct<N, T_ * W_, 1, N + 1, NT, SGN>(D, ld);
+        fft_inplace_ct<N, T_ * W_, 1, N + 1, NT, SGN, 1>(D, tw);
+    }
+    template <int SGN, int NT, class Store>
+    __device__ void fft_to(float2 *D, const float2 *tw, Store st) const {
+        fft_inplace_ct<N, T_ * W_, 1, N + 1, NT, SGN, 0, CtPlan<N>::v.ns - 1>(D, tw);
+        fft_last_ct<N, T_ * W_, 1, N + 1, NT, SGN, false>(D, tw, st);
     }
 };
 
@@ -573,6 +587,13 @@ __global__ void __launch_bounds__(NT) field_xline_grad_kernel(const float2 *__re
     const int wv = min(W, plane - line0);
     for (int i = threadIdx.x; i < n - 1; i += NT) tw[i] = tw_glob[i];
     const int nel = T * W * n;
+    if constexpr (G::kCt) {
+        // the forward transform's first stage reads the lines from global memory
+        geo.template fft_from<-1, NT>(D, tw, [&](int line, int e) -> float2 {
+            const int t = line >> wshift, l = line & (W - 1);
+            return l < wv ? spec[(int64_t)t * g.gc + (int64_t)e * plane + line0 + l] : make_float2(0.f, 0.f);
+        });
+    } else {
     for (int i0 = threadIdx.x; i0 < nel; i0 += NT * kLoadBatch) {
         float2 v[kLoadBatch];
 #pragma unroll
@@ -595,6 +616,7 @@ __global__ void __launch_bounds__(NT) field_xline_grad_kernel(const float2 *__re
     }
     __syncthreads();
     geo.template fft<-1, NT>(D, tw);
+    }
     // k-space pass (the arithmetic of field_kspace_kernel): V_t kept in
     // registers, -i k_x V_t written back in place
     double e_acc[1] = {0.0};
@@ -656,10 +678,18 @@ __global__ void __launch_bounds__(NT) field_xline_grad_kernel(const float2 *__re
         }
     }
     __syncthreads();
-    geo.template fft<1, NT>(D, tw);
-    for (int idx = threadIdx.x; idx < nel; idx += NT) {
-        const int l = idx & (W - 1), te = idx >> wshift, t = geo.div_n(te), e = te - t * n;
-        if (l < wv) out[(int64_t)t * g.gc + (int64_t)e * plane + line0 + l] = D[(t * W + l) * ld + e];
+    if constexpr (G::kCt) {
+        // the inverse transforms' last stage writes the lines to global memory
+        geo.template fft_to<1, NT>(D, tw, [&](int line, int e, float2 v) {
+            const int t = line >> wshift, l = line & (W - 1);
+            if (l < wv) out[(int64_t)t * g.gc + (int64_t)e * plane + line0 + l] = v;
+        });
+    } else {
+        geo.template fft<1, NT>(D, tw);
+        for (int idx = threadIdx.x; idx < nel; idx += NT) {
+            const int l = idx & (W - 1), te = idx >> wshift, t = geo.div_n(te), e = te - t * n;
+            if (l < wv) out[(int64_t)t * g.gc + (int64_t)e * plane + line0 + l] = D[(t * W + l) * ld + e];
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -672,10 +702,17 @@ __global__ void __launch_bounds__(NT) field_xline_grad_kernel(const float2 *__re
             if (t < T) D[(t * W + l) * ld + e] = hold[j][t];
     }
     __syncthreads();
-    geo.template fft<1, NT>(D, tw);
-    for (int idx = threadIdx.x; idx < nel; idx += NT) {
-        const int l = idx & (W - 1), te = idx >> wshift, t = geo.div_n(te), e = te - t * n;
-        if (l < wv) out[(int64_t)(T + t) * g.gc + (int64_t)e * plane + line0 + l] = D[(t * W + l) * ld + e];
+    if constexpr (G::kCt) {
+        geo.template fft_to<1, NT>(D, tw, [&](int line, int e, float2 v) {
+            const int t = line >> wshift, l = line & (W - 1);
+            if (l < wv) out[(int64_t)(T + t) * g.gc + (int64_t)e * plane + line0 + l] = v;
+        });
+    } else {
+        geo.template fft<1, NT>(D, tw);
+        for (int idx = threadIdx.x; idx < nel; idx += NT) {
+            const int l = idx & (W - 1), te = idx >> wshift, t = geo.div_n(te), e = te - t * n;
+            if (l < wv) out[(int64_t)(T + t) * g.gc + (int64_t)e * plane + line0 + l] = D[(t * W + l) * ld + e];
+        }
     }
     if (partials) block_reduce_store<1, NT>(e_acc, partials + blockIdx.x);
 }
