@@ -1150,7 +1150,8 @@ static bool xline_ct_plan(const FftPlan &f) {
 }
 
 // The x-line pass for T types in the shape sh: a compile-time geometry for
-// Mx = 128 or 160 at the 256-thread W that xline_grad_shape picks there, the
+// Mx = 128 or 160 at the 256-thread W that xline_grad_shape picks there and
+// for Mx = 256 with six types at 8 lines of 1024 threads or 4 of 512, the
 // runtime one otherwise.
 static int launch_xline_grad(hymd_pm *pm, const float2 *spec, float2 *out, const KGeom &g, const FieldK &p,
                              const XlineShape &sh, double *partials, int *nblocks) {
@@ -1169,6 +1170,16 @@ static int launch_xline_grad(hymd_pm *pm, const float2 *spec, float2 *out, const
     HYMD_XLINE_CT(160, 3, 4, 4)
     HYMD_XLINE_CT(160, 4, 4, 4)
 #undef HYMD_XLINE_CT
+    // 256-point lines with six types (the 256^3 workloads): 8 lines of 1024
+    // threads or 4 lines of 512
+    if (f.ax.n == 256 && T == 6 && xline_ct_plan<256>(f)) {
+        if (sh.NT == 4 * kXlineGradThreads && W == 8)
+            return launch_xline_grad_g<8, 4 * kXlineGradThreads>(pm, spec, out, g, p, W, XlineCt<6, 256, 8>{},
+                                                                 partials, nblocks);
+        if (sh.NT == 2 * kXlineGradThreads && W == 4)
+            return launch_xline_grad_g<8, 2 * kXlineGradThreads>(pm, spec, out, g, p, W, XlineCt<6, 256, 4>{},
+                                                                 partials, nblocks);
+    }
     int wshift = 0;
     while ((1 << wshift) < W) ++wshift;
     const XlineRt<kXlineGradElems> geo{f.ax.n, W, wshift, T, f.ax.ld, f.ax.ndiv, make_fastdiv((uint32_t)(T * W)), f.ax};

@@ -291,7 +291,8 @@ def test_readout_matches_oracle(hip, mesh, T):
 
 
 @pytest.mark.parametrize("mesh,T", [((32, 32, 32), 2), ((24, 20, 30), 5), ((128, 128, 128), 4),
-                                    ((128, 24, 24), 6)])   # x-line widths on 256, 512 and 1024 threads
+                                    ((128, 24, 24), 6),    # x-line widths on 256, 512 and 1024 threads
+                                    ((256, 16, 16), 6)])   # compile-time 256-point lines (8 of 1024, 4 of 512 threads)
 def test_tuning_knobs_keep_the_bits(hip, mesh, T, tmp_path):
     """The paint and readout launch shapes the autotuner may choose give the
     same density, force grids, forces and energy bit for bit (integer paint
