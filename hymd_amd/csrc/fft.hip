@@ -318,6 +318,48 @@ __global__ void __launch_bounds__(kBlock) fft_y_grad_kernel(const float2 *__rest
             dst[mybase + (int64_t)(idx >> wshift) * es] = r[myl * ax.ld + (idx >> wshift)];
 }
 
+// fft_y_grad_kernel for a compile-time line length N and W lines per
+// workgroup (the plan's own stage plan and twiddles): the first stage reads
+// the lines from global memory (with -i k_y for d = 1) and the last stage
+// writes them, so the lines cross LDS in place once per middle stage only.
+template <int N, int W>
+__global__ void __launch_bounds__(kBlock) fft_y_grad_ct_kernel(const float2 *__restrict__ in,
+                                                                float2 *__restrict__ out, int64_t gc, int nlines,
+                                                                int inner, int64_t outer, int64_t es, int T,
+                                                                const float2 *__restrict__ tw_glob, float kfy) {
+    constexpr int NS = CtPlan<N>::v.ns;
+    extern __shared__ float2 smem[];
+    float2 *tw = smem;
+    float2 *A = tw + N;  // W lines at stride N + 1
+    const unsigned flat = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const unsigned bx = flat % gridDim.x, by = flat / gridDim.x;
+    const int t = (int)by / 3, d = (int)by - 3 * t;
+    const float2 *src = in + (int64_t)(d == 0 ? t : T + t) * gc;
+    float2 *dst = out + (int64_t)by * gc;
+    const int line0 = (int)bx * W;
+    for (int i = threadIdx.x; i < N - 1; i += kBlock) tw[i] = tw_glob[i];
+    auto base = [&](int l) -> int64_t {
+        const int line = line0 + l;
+        return line < nlines ? (int64_t)(line / inner) * outer + (line % inner) : -1;
+    };
+    fft_first_ct<N, W, 1, N + 1, kBlock, 1>(A, [&](int l, int e) -> float2 {
+        const int64_t b = base(l);
+        if (b < 0) return make_float2(0.f, 0.f);
+        float2 v = src[b + (int64_t)e * es];
+        if (d == 1) {  // -i k_y v = (k_y v.y, -k_y v.x)
+            const int ny = (2 * e < N) ? e : e - N;
+            const float k = (2 * e == N) ? 0.f : kfy * (float)ny;
+            v = make_float2(k * v.y, -k * v.x);
+        }
+        return v;
+    });
+    fft_inplace_ct<N, W, 1, N + 1, kBlock, 1, 1, NS - 1>(A, tw);
+    fft_last_ct<N, W, 1, N + 1, kBlock, 1, false>(A, tw, [&](int l, int e, float2 v) {
+        const int64_t b = base(l);
+        if (b >= 0) dst[b + (int64_t)e * es] = v;
+    });
+}
+
 // ---------------------------------------------------------------------------
 // y-z plane kernels: one workgroup per (x-plane, grid) holds the plane's My x
 // (Mz/2+1) half spectrum in LDS and runs the z and y passes in place, so a
@@ -741,9 +783,13 @@ int fft_c2r_lines_grad(const FftPlan &plan, const float2 *in, float2 *tmp, float
     HYMD_REQUIRE(kfy != 0.0 && kfz != 0.0, "the gradient passes need non-zero k factors");
     const int nlines = m.mx * m.zh;
     dim3 gy((unsigned)((nlines + plan.wy - 1) / plan.wy), (unsigned)(3 * T));
-    fft_y_grad_kernel<<<gy, kBlock, plan.lds_y, s>>>(in, tmp, m.gc, nlines, m.zh, (int64_t)m.my * m.zh, m.zh, T,
-                                                     plan.ay, (const float2 *)plan.tw_y.ptr, (float)kfy,
-                                                     log2_exact(plan.wy));
+    if (m.my == 256 && plan.wy == 8 && same_plan(plan.ay, CtPlan<256>::v))
+        fft_y_grad_ct_kernel<256, 8><<<gy, kBlock, sizeof(float2) * (256 + 8 * 257), s>>>(
+            in, tmp, m.gc, nlines, m.zh, (int64_t)m.my * m.zh, m.zh, T, (const float2 *)plan.tw_y.ptr, (float)kfy);
+    else
+        fft_y_grad_kernel<<<gy, kBlock, plan.lds_y, s>>>(in, tmp, m.gc, nlines, m.zh, (int64_t)m.my * m.zh, m.zh, T,
+                                                         plan.ay, (const float2 *)plan.tw_y.ptr, (float)kfy,
+                                                         log2_exact(plan.wy));
     HYMD_HIP(hipGetLastError());
     const int64_t rows = (int64_t)m.mx * m.my;
     dim3 gz((unsigned)((rows + plan.wz - 1) / plan.wz), (unsigned)(3 * T));

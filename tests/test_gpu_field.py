@@ -346,14 +346,16 @@ def test_tuning_knobs_keep_the_bits(hip, mesh, T, tmp_path):
 
 
 @pytest.mark.parametrize("T", [1, 3, 6])
-def test_line_gradient_path_matches_oracle(hip, T):
-    """Meshes whose y-z plane does not fit in LDS (16 x 200 x 200 here, like
-    256^3) take the line-pass gradient path: the fused x-line pass, the
-    out-of-place y pass with -i k_y on load and the z pass with -i k_z on load.
-    Field forces, force grids and energy against the FP64 oracle; PME forces
-    and energy against the oracle on the same mesh."""
+@pytest.mark.parametrize("mesh,box", [((16, 200, 200), (3.2, 40.0, 40.0)), ((8, 256, 256), (1.6, 51.2, 51.2))])
+def test_line_gradient_path_matches_oracle(hip, T, mesh, box):
+    """Meshes whose y-z plane does not fit in LDS (16 x 200 x 200 and
+    8 x 256 x 256 here, like 256^3) take the line-pass gradient path: the
+    fused x-line pass, the out-of-place y pass with -i k_y on load (for 256-
+    point y lines the compile-time kernel whose first and last stages read
+    and write global memory) and the z pass with -i k_z on load.  Field
+    forces, force grids and energy against the FP64 oracle; PME forces and
+    energy against the oracle on the same mesh."""
     from hymd_amd.md import COULOMB_CONSTANT
-    mesh, box = (16, 200, 200), (3.2, 40.0, 40.0)
     rng = np.random.default_rng(40 + T)
     n = 60_000
     x = (rng.uniform(0, 1, (n, 3)) * np.asarray(box)).astype(np.float32)
