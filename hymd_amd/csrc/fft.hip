@@ -218,6 +218,69 @@ __global__ void __launch_bounds__(kBlock) c2r_z_kernel(const float2 *__restrict_
     }
 }
 
+// c2r_z_kernel for a compile-time half length NH (rows of NH + 1 spectra)
+// and W rows per workgroup (the plan's own stage plan and twiddles): the
+// transform runs in place and its last stage writes the rows, scaled, to
+// global memory.
+template <int NH, int W>
+__global__ void __launch_bounds__(kBlock) c2r_z_ct_kernel(const float2 *__restrict__ in, float *__restrict__ out,
+                                                           int64_t rows, const float2 *__restrict__ tw_glob,
+                                                           const float2 *__restrict__ post_glob, float scale,
+                                                           float kfz) {
+    constexpr int ZH = NH + 1, MZ = 2 * NH, NS = CtPlan<NH>::v.ns;
+    extern __shared__ float2 smem[];
+    float2 *tw = smem;
+    float2 *post = tw + NH;
+    float2 *A = post + ZH;
+    float2 *B = A + W * ZH;
+    for (int i = threadIdx.x; i < NH - 1; i += blockDim.x) tw[i] = tw_glob[i];
+    for (int i = threadIdx.x; i < ZH; i += blockDim.x) post[i] = post_glob[i];
+    const unsigned flat = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const unsigned bx = flat % gridDim.x, by = flat / gridDim.x;
+    const int64_t row0 = (int64_t)bx * W;
+    const int64_t bofs = (int64_t)by * rows;
+    const int wv = (int)(rows - row0 < W ? rows - row0 : W);
+    const float2 *src = in + (bofs + row0) * ZH;  // W contiguous half-spectrum rows
+    const int nvalid = wv * ZH, total = W * ZH;
+    const bool dz = kfz != 0.f && by % 3 == 2;
+    for (int i0 = threadIdx.x; i0 < total; i0 += kBlock * kLoadBatch) {
+        float2 v[kLoadBatch];
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kBlock;
+            v[j] = idx < nvalid ? src[idx] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int j = 0; j < kLoadBatch; ++j) {
+            const int idx = i0 + j * kBlock;
+            if (idx < total) {
+                if (dz) {  // -i k_z v = (k_z v.y, -k_z v.x)
+                    const int kz = idx % ZH;
+                    const float k = (2 * kz == MZ) ? 0.f : kfz * (float)kz;
+                    v[j] = make_float2(k * v[j].y, -k * v[j].x);
+                }
+                A[idx] = v[j];
+            }
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < W * NH; idx += blockDim.x) {
+        const int l = idx / NH, k = idx - l * NH;
+        const float2 *xr = A + l * ZH;
+        float2 xk = xr[k];
+        float2 xc = cconj(xr[NH - k]);
+        float2 sm = cadd(xk, xc);
+        float2 df = cmul(cconj(post[k]), csub(xk, xc));
+        B[l * ZH + k] = cadd(sm, cmul_i(df, 1.f));
+    }
+    __syncthreads();
+    float2 *dst = reinterpret_cast<float2 *>(out) + (bofs + row0) * NH;
+    fft_inplace_ct<NH, W, 1, ZH, kBlock, 1, 0, NS - 1>(B, tw);
+    fft_last_ct<NH, W, 1, ZH, kBlock, 1, true>(B, tw, [&](int l, int e, float2 v) {
+        if (l < wv) dst[l * NH + e] = make_float2(v.x * scale, v.y * scale);
+    });
+}
+
 // In-place complex pass along a strided axis.  Line l -> base (l / inner) *
 // outer + (l % inner), element e at base + e * es.  2^wshift consecutive
 // lines per workgroup (consecutive in memory except at `inner` boundaries).
@@ -793,8 +856,13 @@ int fft_c2r_lines_grad(const FftPlan &plan, const float2 *in, float2 *tmp, float
     HYMD_HIP(hipGetLastError());
     const int64_t rows = (int64_t)m.mx * m.my;
     dim3 gz((unsigned)((rows + plan.wz - 1) / plan.wz), (unsigned)(3 * T));
-    c2r_z_kernel<<<gz, kBlock, plan.lds_z, s>>>(tmp, out, rows, m.mz, m.zh, plan.azh, (const float2 *)plan.tw_zh.ptr,
-                                                (const float2 *)plan.post.ptr, scale, plan.wz, (float)kfz);
+    if (m.mz == 256 && plan.wz == 16 && plan.azh.ld == 129 && same_plan(plan.azh, CtPlan<128>::v))
+        c2r_z_ct_kernel<128, 16><<<gz, kBlock, plan.lds_z, s>>>(tmp, out, rows, (const float2 *)plan.tw_zh.ptr,
+                                                                (const float2 *)plan.post.ptr, scale, (float)kfz);
+    else
+        c2r_z_kernel<<<gz, kBlock, plan.lds_z, s>>>(tmp, out, rows, m.mz, m.zh, plan.azh,
+                                                    (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr,
+                                                    scale, plan.wz, (float)kfz);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }
