@@ -152,6 +152,42 @@ __global__ void __launch_bounds__(kBlock) r2c_z_kernel(const float *__restrict__
     }
 }
 
+// r2c_z_kernel for a compile-time half length NH and W rows per workgroup
+// (the plan's own stage plan and twiddles): the first stage reads the rows
+// from global memory, the transform runs in place, then the split.
+template <int NH, int W>
+__global__ void __launch_bounds__(kBlock) r2c_z_ct_kernel(const float *__restrict__ in_, float2 *__restrict__ out,
+                                                           int64_t rows, const float2 *__restrict__ tw_glob,
+                                                           const float2 *__restrict__ post_glob) {
+    constexpr int ZH = NH + 1;
+    extern __shared__ float2 smem[];
+    float2 *tw = smem;
+    float2 *post = tw + NH;
+    float2 *A = post + ZH;  // W rows at stride ZH
+    for (int i = threadIdx.x; i < NH - 1; i += blockDim.x) tw[i] = tw_glob[i];
+    for (int i = threadIdx.x; i < ZH; i += blockDim.x) post[i] = post_glob[i];
+    const unsigned flat = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const unsigned bx = flat % gridDim.x, by = flat / gridDim.x;
+    const int64_t row0 = (int64_t)bx * W;
+    const int64_t bofs = (int64_t)by * rows;
+    const int wv = (int)(rows - row0 < W ? rows - row0 : W);
+    const float2 *src = reinterpret_cast<const float2 *>(in_) + (bofs + row0) * NH;  // rows of NH complex
+    fft_first_ct<NH, W, 1, ZH, kBlock, -1, true>(A, [&](int l, int e) -> float2 {
+        return l < wv ? src[l * NH + e] : make_float2(0.f, 0.f);
+    });
+    fft_inplace_ct<NH, W, 1, ZH, kBlock, -1, 1>(A, tw);
+    float2 *o = out + (bofs + row0) * ZH;
+    for (int idx = threadIdx.x; idx < wv * ZH; idx += blockDim.x) {
+        const int l = idx / ZH, k = idx - l * ZH;
+        const float2 *zr = A + l * ZH;
+        float2 zk = zr[k == NH ? 0 : k];
+        float2 zc = cconj(zr[k == 0 ? 0 : NH - k]);
+        float2 a = cscale(cadd(zk, zc), 0.5f);
+        float2 b = cmul_i(csub(zk, zc), -0.5f);
+        o[idx] = cadd(a, cmul(post[k], b));
+    }
+}
+
 // z pass, half spectrum -> real:  Z'[k] = (X[k] + X*[Nh-k]) + i e^{2 pi i k/Mz} (X[k] - X*[Nh-k]),
 // inverse FFT of length Nh gives x[2n] + i x[2n+1] (unnormalised).
 // With kfz != 0 the grids by = 3t + 2 are multiplied by -i k_z (k_z = kfz kz,
@@ -785,9 +821,13 @@ int fft_r2c_zy(const FftPlan &plan, const float *in, float2 *out, int batch, hip
     }
     const int64_t rows = (int64_t)m.mx * m.my;
     dim3 grid((unsigned)((rows + plan.wz - 1) / plan.wz), (unsigned)batch);
-    r2c_z_kernel<<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, m.mz, m.zh, plan.azh,
-                                                   (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr,
-                                                   plan.wz);
+    if (m.mz == 256 && plan.wz == 16 && plan.azh.ld == 129 && same_plan(plan.azh, CtPlan<128>::v))
+        r2c_z_ct_kernel<128, 16><<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, (const float2 *)plan.tw_zh.ptr,
+                                                                  (const float2 *)plan.post.ptr);
+    else
+        r2c_z_kernel<<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, m.mz, m.zh, plan.azh,
+                                                       (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr,
+                                                       plan.wz);
     HYMD_HIP(hipGetLastError());
     return launch_strided(plan, out, batch, 1, -1.f, s);
 }
