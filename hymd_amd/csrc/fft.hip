@@ -361,6 +361,38 @@ __global__ void __launch_bounds__(kBlock) fft_strided_kernel(float2 *__restrict_
         for (int idx = threadIdx.x; idx < nel; idx += blockDim.x) d[mybase + (int64_t)(idx >> wshift) * es] = r[myl * ax.ld + (idx >> wshift)];
 }
 
+// fft_strided_kernel for a compile-time line length N, W lines per
+// workgroup and sign SGN (the plan's own stage plan and twiddles): the first
+// stage reads the lines from global memory, the middle stages run in place
+// in LDS and the last stage writes the lines back.
+template <int N, int W, int SGN>
+__global__ void __launch_bounds__(kBlock) fft_strided_ct_kernel(float2 *__restrict__ data, int64_t batch_stride,
+                                                                 int nlines, int inner, int64_t outer, int64_t es,
+                                                                 const float2 *__restrict__ tw_glob) {
+    constexpr int NS = CtPlan<N>::v.ns;
+    extern __shared__ float2 smem[];
+    float2 *tw = smem;
+    float2 *A = tw + N;  // W lines at stride N + 1
+    const unsigned flat = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const unsigned bx = flat % gridDim.x, by = flat / gridDim.x;
+    float2 *d = data + (int64_t)by * batch_stride;
+    const int line0 = (int)bx * W;
+    for (int i = threadIdx.x; i < N - 1; i += kBlock) tw[i] = tw_glob[i];
+    auto base = [&](int l) -> int64_t {
+        const int line = line0 + l;
+        return line < nlines ? (int64_t)(line / inner) * outer + (line % inner) : -1;
+    };
+    fft_first_ct<N, W, 1, N + 1, kBlock, SGN>(A, [&](int l, int e) -> float2 {
+        const int64_t b = base(l);
+        return b >= 0 ? d[b + (int64_t)e * es] : make_float2(0.f, 0.f);
+    });
+    fft_inplace_ct<N, W, 1, N + 1, kBlock, SGN, 1, NS - 1>(A, tw);
+    fft_last_ct<N, W, 1, N + 1, kBlock, SGN, false>(A, tw, [&](int l, int e, float2 v) {
+        const int64_t b = base(l);
+        if (b >= 0) d[b + (int64_t)e * es] = v;
+    });
+}
+
 // Out-of-place inverse y pass of the gradient spectra: grid by = 3t + d of
 // out[] from in[t] (d = 0: in[t] = IFFT_x(-i k_x V_t)) or in[T + t]
 // (IFFT_x(V_t)), times -i k_y on load for d = 1 (k_y = kfy ny, zero on the
@@ -735,8 +767,35 @@ static int log2_exact(int w) {
     return s;
 }
 
+static bool same_plan(const FftAxis &ax, const StagePlan &sp) {
+    if (ax.nstages != sp.ns) return false;
+    for (int i = 0; i < sp.ns; ++i)
+        if (ax.radix[i] != sp.r[i] || ax.twoff[i] != sp.twoff[i]) return false;
+    return true;
+}
+
 static int launch_strided(const FftPlan &plan, float2 *data, int batch, int axis, float sgn, hipStream_t s) {
     const Mesh &m = plan.mesh;
+    {
+        // 256-point lines, 8 per workgroup: the compile-time kernel
+        const FftAxis &ax = axis == 1 ? plan.ay : plan.ax;
+        const int w = axis == 1 ? plan.wy : plan.wx;
+        if (ax.n == 256 && w == 8 && same_plan(ax, CtPlan<256>::v)) {
+            const int nlines = axis == 1 ? m.mx * m.zh : m.my * m.zh;
+            const int inner = axis == 1 ? m.zh : nlines;
+            const int64_t outer = axis == 1 ? (int64_t)m.my * m.zh : 0;
+            const int64_t es = axis == 1 ? m.zh : nlines;
+            dim3 grid((unsigned)((nlines + 7) / 8), (unsigned)batch);
+            const size_t lds = sizeof(float2) * (256 + 8 * 257);
+            const float2 *tw = (const float2 *)(axis == 1 ? plan.tw_y.ptr : plan.tw_x.ptr);
+            if (sgn < 0.f)
+                fft_strided_ct_kernel<256, 8, -1><<<grid, kBlock, lds, s>>>(data, m.gc, nlines, inner, outer, es, tw);
+            else
+                fft_strided_ct_kernel<256, 8, 1><<<grid, kBlock, lds, s>>>(data, m.gc, nlines, inner, outer, es, tw);
+            HYMD_HIP(hipGetLastError());
+            return HYMD_OK;
+        }
+    }
     if (axis == 1) {  // y: lines (x, kz), stride zh
         const int nlines = m.mx * m.zh;
         dim3 grid((unsigned)((nlines + plan.wy - 1) / plan.wy), (unsigned)batch);
@@ -769,13 +828,6 @@ static int plane_lds_opt_in() {
         done = true;
     }
     return HYMD_OK;
-}
-
-static bool same_plan(const FftAxis &ax, const StagePlan &sp) {
-    if (ax.nstages != sp.ns) return false;
-    for (int i = 0; i < sp.ns; ++i)
-        if (ax.radix[i] != sp.r[i] || ax.twoff[i] != sp.twoff[i]) return false;
-    return true;
 }
 
 // Calls f(geometry) with the compile-time geometry of the plan's (My, Mz) when
