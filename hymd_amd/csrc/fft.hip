@@ -774,28 +774,43 @@ static bool same_plan(const FftAxis &ax, const StagePlan &sp) {
     return true;
 }
 
+// The compile-time strided pass when the axis has N points, W lines per
+// workgroup and the stage plan of CtPlan<N>; false if not.
+template <int N, int W>
+static bool strided_ct(const FftPlan &plan, float2 *data, int batch, int axis, float sgn, hipStream_t s, int *err) {
+    const Mesh &m = plan.mesh;
+    const FftAxis &ax = axis == 1 ? plan.ay : plan.ax;
+    const int w = axis == 1 ? plan.wy : plan.wx;
+    if (ax.n != N || w != W || !same_plan(ax, CtPlan<N>::v)) return false;
+    const int nlines = axis == 1 ? m.mx * m.zh : m.my * m.zh;
+    const int inner = axis == 1 ? m.zh : nlines;
+    const int64_t outer = axis == 1 ? (int64_t)m.my * m.zh : 0;
+    const int64_t es = axis == 1 ? m.zh : nlines;
+    dim3 grid((unsigned)((nlines + W - 1) / W), (unsigned)batch);
+    const size_t lds = sizeof(float2) * (N + W * (N + 1));
+    const float2 *tw = (const float2 *)(axis == 1 ? plan.tw_y.ptr : plan.tw_x.ptr);
+    if (sgn < 0.f)
+        fft_strided_ct_kernel<N, W, -1><<<grid, kBlock, lds, s>>>(data, m.gc, nlines, inner, outer, es, tw);
+    else
+        fft_strided_ct_kernel<N, W, 1><<<grid, kBlock, lds, s>>>(data, m.gc, nlines, inner, outer, es, tw);
+    const hipError_t e = hipGetLastError();
+    *err = HYMD_OK;
+    if (e != hipSuccess) {
+        set_error("fft_strided_ct_kernel<%d, %d> launch failed: %s", N, W, hipGetErrorString(e));
+        *err = HYMD_ERR_HIP;
+    }
+    return true;
+}
+
 static int launch_strided(const FftPlan &plan, float2 *data, int batch, int axis, float sgn, hipStream_t s) {
     const Mesh &m = plan.mesh;
-    {
-        // 256-point lines, 8 per workgroup: the compile-time kernel
-        const FftAxis &ax = axis == 1 ? plan.ay : plan.ax;
-        const int w = axis == 1 ? plan.wy : plan.wx;
-        if (ax.n == 256 && w == 8 && same_plan(ax, CtPlan<256>::v)) {
-            const int nlines = axis == 1 ? m.mx * m.zh : m.my * m.zh;
-            const int inner = axis == 1 ? m.zh : nlines;
-            const int64_t outer = axis == 1 ? (int64_t)m.my * m.zh : 0;
-            const int64_t es = axis == 1 ? m.zh : nlines;
-            dim3 grid((unsigned)((nlines + 7) / 8), (unsigned)batch);
-            const size_t lds = sizeof(float2) * (256 + 8 * 257);
-            const float2 *tw = (const float2 *)(axis == 1 ? plan.tw_y.ptr : plan.tw_x.ptr);
-            if (sgn < 0.f)
-                fft_strided_ct_kernel<256, 8, -1><<<grid, kBlock, lds, s>>>(data, m.gc, nlines, inner, outer, es, tw);
-            else
-                fft_strided_ct_kernel<256, 8, 1><<<grid, kBlock, lds, s>>>(data, m.gc, nlines, inner, outer, es, tw);
-            HYMD_HIP(hipGetLastError());
-            return HYMD_OK;
-        }
-    }
+    int err = HYMD_OK;
+    if (strided_ct<256, 8>(plan, data, batch, axis, sgn, s, &err) ||
+        strided_ct<160, 16>(plan, data, batch, axis, sgn, s, &err) ||
+        strided_ct<128, 16>(plan, data, batch, axis, sgn, s, &err) ||
+        strided_ct<64, 16>(plan, data, batch, axis, sgn, s, &err) ||
+        strided_ct<48, 16>(plan, data, batch, axis, sgn, s, &err))
+        return err;
     if (axis == 1) {  // y: lines (x, kz), stride zh
         const int nlines = m.mx * m.zh;
         dim3 grid((unsigned)((nlines + plan.wy - 1) / plan.wy), (unsigned)batch);
@@ -906,9 +921,13 @@ int fft_c2r_yz(const FftPlan &plan, float2 *in, float *out, int batch, float sca
     HYMD_TRY(launch_strided(plan, in, batch, 1, 1.f, s));
     const int64_t rows = (int64_t)m.mx * m.my;
     dim3 grid((unsigned)((rows + plan.wz - 1) / plan.wz), (unsigned)batch);
-    c2r_z_kernel<<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, m.mz, m.zh, plan.azh,
-                                                   (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr,
-                                                   scale, plan.wz, 0.f);
+    if (m.mz == 256 && plan.wz == 16 && plan.azh.ld == 129 && same_plan(plan.azh, CtPlan<128>::v))
+        c2r_z_ct_kernel<128, 16><<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, (const float2 *)plan.tw_zh.ptr,
+                                                                  (const float2 *)plan.post.ptr, scale, 0.f);
+    else
+        c2r_z_kernel<<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, m.mz, m.zh, plan.azh,
+                                                       (const float2 *)plan.tw_zh.ptr, (const float2 *)plan.post.ptr,
+                                                       scale, plan.wz, 0.f);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }
