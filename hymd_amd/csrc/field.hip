@@ -569,11 +569,19 @@ struct XlineCt {
     }
 };
 
-template <int MAXT, int NT, class G>
+struct PmeK {
+    double sigma2;
+    double inv_g;
+    double four_pi_elec;  // 4 pi k_e / eps_r
+};
+
+// PME (T = 1): the k-space pass is pme_kspace_kernel's (psi = 4 pi elec H
+// rho~ / |k|^2 with the same FP64 arithmetic); outputs -i k_x psi and psi.
+template <int MAXT, int NT, class G, bool PME = false>
 __global__ void __launch_bounds__(NT) field_xline_grad_kernel(const float2 *__restrict__ spec,
                                                               float2 *__restrict__ out, KGeom g, FieldK p, G geo,
                                                               const float2 *__restrict__ tw_glob,
-                                                              double *__restrict__ partials) {
+                                                              double *__restrict__ partials, PmeK pk) {
     constexpr int PMAX = 16 / MAXT;  // W n <= NT PMAX (checked on the host)
     extern __shared__ float2 smem[];
     const int n = geo.n;  // Mx
@@ -633,6 +641,19 @@ __global__ void __launch_bounds__(NT) field_xline_grad_kernel(const float2 *__re
         const int64_t kk = (int64_t)e * plane + line0 + l;
         double k[3], kd[3], wgt;
         kvec(g, kk, k, kd, wgt);
+        if constexpr (PME) {
+            double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+            const double H = exp(-0.5 * pk.sigma2 * k2);
+            if (kk == 0) k2 = 1.0;
+            const float2 sp = D[l * ld + e];
+            const double rx = (double)sp.x * pk.inv_g, ry = (double)sp.y * pk.inv_g;
+            const double f = pk.four_pi_elec * H / k2;
+            const double px = f * rx, py = f * ry;
+            e_acc[0] += 0.5 * wgt * (rx * px + ry * py);
+            hold[j][0] = make_float2((float)px, (float)py);
+            D[l * ld + e] = make_float2((float)(kd[0] * py), (float)(-kd[0] * px));
+            continue;
+        }
         const double k2 = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
         const float H = (float)exp(-0.5 * p.sigma2 * k2);
         const float hn = (float)((double)H * p.inv_g);
@@ -891,11 +912,6 @@ __global__ void __launch_bounds__(kBlock) field_pressure_kernel(const float2 *__
     block_reduce_store<kPressNv>(acc, partials + (int64_t)blockIdx.x * kPressNv);
 }
 
-struct PmeK {
-    double sigma2;
-    double inv_g;
-    double four_pi_elec;  // 4 pi k_e / eps_r
-};
 
 // PME (field.py:363-397): rho~ = H rho/G; psi = 4 pi elec rho~/|k|^2 (|k|^2 := 1
 // at k = 0); E_d = -i k_d psi.  Energy partials: 1/2 wgt Re(rho^* psi).
@@ -1111,7 +1127,7 @@ static XlineShape xline_grad_shape(const FftPlan &f, int T, int want = 0) {
 
 template <int MAXT, int NT, class G>
 static int launch_xline_grad_g(hymd_pm *pm, const float2 *spec, float2 *out, const KGeom &g, const FieldK &p, int W,
-                               const G &geo, double *partials, int *nblocks) {
+                               const G &geo, double *partials, int *nblocks, const PmeK *pme) {
     const FftPlan &f = pm->fft;
     const int plane = g.my * g.zh;
     const int nb = (plane + W - 1) / W;
@@ -1121,10 +1137,16 @@ static int launch_xline_grad_g(hymd_pm *pm, const float2 *spec, float2 *out, con
     if (!attr_set && lds > 64 * 1024) {
         HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&field_xline_grad_kernel<MAXT, NT, G>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim));
+        HYMD_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&field_xline_grad_kernel<MAXT, NT, G, true>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim));
         attr_set = true;
     }
-    field_xline_grad_kernel<MAXT, NT, G><<<nb, NT, lds, pm->stream>>>(spec, out, g, p, geo,
-                                                                      (const float2 *)f.tw_x.ptr, partials);
+    if (pme)
+        field_xline_grad_kernel<MAXT, NT, G, true><<<nb, NT, lds, pm->stream>>>(
+            spec, out, g, p, geo, (const float2 *)f.tw_x.ptr, partials, *pme);
+    else
+        field_xline_grad_kernel<MAXT, NT, G><<<nb, NT, lds, pm->stream>>>(spec, out, g, p, geo,
+                                                                          (const float2 *)f.tw_x.ptr, partials, PmeK{});
     HYMD_HIP(hipGetLastError());
     *nblocks = nb;
     return HYMD_OK;
@@ -1132,12 +1154,15 @@ static int launch_xline_grad_g(hymd_pm *pm, const float2 *spec, float2 *out, con
 
 template <int MAXT, class G>
 static int launch_xline_grad_nt(hymd_pm *pm, const float2 *spec, float2 *out, const KGeom &g, const FieldK &p,
-                                const XlineShape &sh, const G &geo, double *partials, int *nblocks) {
+                                const XlineShape &sh, const G &geo, double *partials, int *nblocks,
+                                const PmeK *pme) {
     if (sh.NT == 4 * kXlineGradThreads)
-        return launch_xline_grad_g<MAXT, 4 * kXlineGradThreads>(pm, spec, out, g, p, sh.W, geo, partials, nblocks);
+        return launch_xline_grad_g<MAXT, 4 * kXlineGradThreads>(pm, spec, out, g, p, sh.W, geo, partials, nblocks,
+                                                                pme);
     if (sh.NT == 2 * kXlineGradThreads)
-        return launch_xline_grad_g<MAXT, 2 * kXlineGradThreads>(pm, spec, out, g, p, sh.W, geo, partials, nblocks);
-    return launch_xline_grad_g<MAXT, kXlineGradThreads>(pm, spec, out, g, p, sh.W, geo, partials, nblocks);
+        return launch_xline_grad_g<MAXT, 2 * kXlineGradThreads>(pm, spec, out, g, p, sh.W, geo, partials, nblocks,
+                                                                pme);
+    return launch_xline_grad_g<MAXT, kXlineGradThreads>(pm, spec, out, g, p, sh.W, geo, partials, nblocks, pme);
 }
 
 template <int N>
@@ -1154,13 +1179,13 @@ static bool xline_ct_plan(const FftPlan &f) {
 // for Mx = 256 with six types at 8 lines of 1024 threads or 4 of 512, the
 // runtime one otherwise.
 static int launch_xline_grad(hymd_pm *pm, const float2 *spec, float2 *out, const KGeom &g, const FieldK &p,
-                             const XlineShape &sh, double *partials, int *nblocks) {
+                             const XlineShape &sh, double *partials, int *nblocks, const PmeK *pme = nullptr) {
     const FftPlan &f = pm->fft;
     const int T = p.T, W = sh.W;
 #define HYMD_XLINE_CT(NX, TT, WW, MT)                                                                      \
     if (sh.NT == kXlineGradThreads && f.ax.n == NX && T == TT && W == WW && xline_ct_plan<NX>(f))          \
         return launch_xline_grad_g<MT, kXlineGradThreads>(pm, spec, out, g, p, W, XlineCt<TT, NX, WW>{}, partials, \
-                                                          nblocks);
+                                                          nblocks, pme);
     HYMD_XLINE_CT(128, 1, 16, 2)
     HYMD_XLINE_CT(128, 2, 16, 2)
     HYMD_XLINE_CT(128, 3, 8, 4)
@@ -1169,23 +1194,24 @@ static int launch_xline_grad(hymd_pm *pm, const float2 *spec, float2 *out, const
     HYMD_XLINE_CT(160, 2, 8, 2)
     HYMD_XLINE_CT(160, 3, 4, 4)
     HYMD_XLINE_CT(160, 4, 4, 4)
+    HYMD_XLINE_CT(256, 1, 8, 2)
 #undef HYMD_XLINE_CT
     // 256-point lines with six types (the 256^3 workloads): 8 lines of 1024
     // threads or 4 lines of 512
     if (f.ax.n == 256 && T == 6 && xline_ct_plan<256>(f)) {
         if (sh.NT == 4 * kXlineGradThreads && W == 8)
             return launch_xline_grad_g<8, 4 * kXlineGradThreads>(pm, spec, out, g, p, W, XlineCt<6, 256, 8>{},
-                                                                 partials, nblocks);
+                                                                 partials, nblocks, pme);
         if (sh.NT == 2 * kXlineGradThreads && W == 4)
             return launch_xline_grad_g<8, 2 * kXlineGradThreads>(pm, spec, out, g, p, W, XlineCt<6, 256, 4>{},
-                                                                 partials, nblocks);
+                                                                 partials, nblocks, pme);
     }
     int wshift = 0;
     while ((1 << wshift) < W) ++wshift;
     const XlineRt<kXlineGradElems> geo{f.ax.n, W, wshift, T, f.ax.ld, f.ax.ndiv, make_fastdiv((uint32_t)(T * W)), f.ax};
-    if (T <= 2) return launch_xline_grad_nt<2>(pm, spec, out, g, p, sh, geo, partials, nblocks);
-    if (T <= 4) return launch_xline_grad_nt<4>(pm, spec, out, g, p, sh, geo, partials, nblocks);
-    return launch_xline_grad_nt<8>(pm, spec, out, g, p, sh, geo, partials, nblocks);
+    if (T <= 2) return launch_xline_grad_nt<2>(pm, spec, out, g, p, sh, geo, partials, nblocks, pme);
+    if (T <= 4) return launch_xline_grad_nt<4>(pm, spec, out, g, p, sh, geo, partials, nblocks, pme);
+    return launch_xline_grad_nt<8>(pm, spec, out, g, p, sh, geo, partials, nblocks, pme);
 }
 
 // LDS budget of one row-paint workgroup (int64 accumulators of tc types x Mz)
@@ -1927,23 +1953,35 @@ int hymd::update_field_force_q_impl(hymd_pm *pm, const float *x, const float *ch
     float *rho = phi_q ? phi_q : (float *)pm->paint.ptr;
     HYMD_TRY(sort_particles(pm, x, nullptr, charges, n));
     HYMD_TRY(paint_sorted(pm, 1, nullptr, rho, nullptr));
-    HYMD_TRY(fft_r2c_real(pm->fft, rho, spec, 1, s));
     PmeK p;
     p.sigma2 = sigma * sigma;
     p.inv_g = 1.0 / (double)m.g;
     p.four_pi_elec = 4.0 * M_PI * elec_factor;
     KGeom g = kgeom(pm);
     double *partials = energy_q ? (double *)pm->partials.ptr : nullptr;
-    pme_kspace_kernel<<<pm->kblocks, kBlock, 0, s>>>(spec, out, psi ? 1 : 0, 1, g, p, partials);
-    HYMD_HIP(hipGetLastError());
+    int nparts = pm->kblocks;
+    const XlineShape sh = xline_grad_shape(pm->fft, 1, pm->xline_w);
+    if (sh.W > 0) {
+        // z + y forward passes, then one x-line pass: the forward x
+        // transform, the k-space pass and the inverse x transforms of
+        // -i k_x psi and psi
+        HYMD_TRY(fft_r2c_zy(pm->fft, rho, spec, 1, s));
+        FieldK fk{};
+        fk.T = 1;
+        HYMD_TRY(launch_xline_grad(pm, spec, out, g, fk, sh, partials, &nparts, &p));
+    } else {
+        HYMD_TRY(fft_r2c_real(pm->fft, rho, spec, 1, s));
+        pme_kspace_kernel<<<pm->kblocks, kBlock, 0, s>>>(spec, out, psi ? 1 : 0, 1, g, p, partials);
+        HYMD_HIP(hipGetLastError());
+        // x-inverse of -i k_x psi and psi
+        HYMD_TRY(fft_inverse_x(pm->fft, out, 2, s));
+    }
     float *field = elec_field;
     if (!field) {
         HYMD_TRY(pm->rscratch.reserve(sizeof(float) * 3 * m.g));
         field = (float *)pm->rscratch.ptr;
     }
-    // x-inverse of -i k_x psi and psi; the plane pass (or the y and z line
-    // passes) forms E_y, E_z from the latter
-    HYMD_TRY(fft_inverse_x(pm->fft, out, 2, s));
+    // the plane pass (or the y and z line passes) forms E_y, E_z from psi
     if (pm->fft.lds_plane) {
         HYMD_TRY(fft_c2r_yz_grad(pm->fft, out, field, 1, g.kfac[1], g.kfac[2], 1.f, s));
     } else {
@@ -1955,7 +1993,7 @@ int hymd::update_field_force_q_impl(hymd_pm *pm, const float *x, const float *ch
     HYMD_TRY(readout_sorted(pm, 3, 1, field, elec_forces, true, accumulate));
     if (energy_q) {
         double e = 0.0;
-        HYMD_TRY(read_result(pm, pm->kblocks, 1, &e));
+        HYMD_TRY(read_result(pm, nparts, 1, &e));
         *energy_q = e * (m.box[0] * m.box[1] * m.box[2]) - self_energy;
     }
     return HYMD_OK;
