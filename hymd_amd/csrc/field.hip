@@ -309,24 +309,38 @@ __global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict
         const CicAxis cy = cic_axis((double)r.y, m.inv_h[1], m.my);
         const CicAxis cz = cic_axis((double)r.z, m.inv_h[2], m.mz);
         const float *gt = grids + (int64_t)t * NCOMP * m.g;
-        float v[8][NCOMP];
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int a = c >> 2, b = (c >> 1) & 1, d = c & 1;
-            const int64_t idx =
-                ((int64_t)(a ? cx.i1 : cx.i0) * m.my + (b ? cy.i1 : cy.i0)) * m.mz + (d ? cz.i1 : cz.i0);
-#pragma unroll
-            for (int q = 0; q < NCOMP; ++q) v[c][q] = gt[q * m.g + idx];
-        }
         double acc[NCOMP];
 #pragma unroll
         for (int q = 0; q < NCOMP; ++q) acc[q] = 0.0;
+        // two halves of four corners (a = 0, then a = 1), the sums in corner
+        // order; the empty asm makes the second half's x index depend on the
+        // first half's sums, so its loads follow them and a thread holds 12
+        // loaded values, not 24
+        int ix1 = cx.i1;
 #pragma unroll
-        for (int c = 0; c < 8; ++c) {
-            const int a = c >> 2, b = (c >> 1) & 1, d = c & 1;
-            const double w = ((a ? cx.w1 : cx.w0) * (b ? cy.w1 : cy.w0)) * (d ? cz.w1 : cz.w0);
+        for (int h = 0; h < 2; ++h) {
+            float v[4][NCOMP];
+            if (h == 1) {
+                if constexpr (NCOMP == 3)
+                    asm volatile("" : "+v"(ix1) : "v"(acc[0]), "v"(acc[1]), "v"(acc[2]));
+                else
+                    asm volatile("" : "+v"(ix1) : "v"(acc[0]));
+            }
 #pragma unroll
-            for (int q = 0; q < NCOMP; ++q) acc[q] += w * (double)v[c][q];
+            for (int c4 = 0; c4 < 4; ++c4) {
+                const int b = c4 >> 1, d = c4 & 1;
+                const int64_t idx =
+                    ((int64_t)(h ? ix1 : cx.i0) * m.my + (b ? cy.i1 : cy.i0)) * m.mz + (d ? cz.i1 : cz.i0);
+#pragma unroll
+                for (int q = 0; q < NCOMP; ++q) v[c4][q] = gt[q * m.g + idx];
+            }
+#pragma unroll
+            for (int c4 = 0; c4 < 4; ++c4) {
+                const int b = c4 >> 1, d = c4 & 1;
+                const double w = ((h ? cx.w1 : cx.w0) * (b ? cy.w1 : cy.w0)) * (d ? cz.w1 : cz.w0);
+#pragma unroll
+                for (int q = 0; q < NCOMP; ++q) acc[q] += w * (double)v[c4][q];
+            }
         }
         const double s = scale_per_particle ? (double)scale_per_particle[i] : 1.0;
 #pragma unroll
