@@ -402,6 +402,39 @@ __device__ __forceinline__ void fft_first_ct(float2 *buf, Load ld) {
     __syncthreads();
 }
 
+// Stage 0 of fft_inplace_ct with its inputs formed by ld(line, element)
+// from buf itself (a pass that would otherwise write buf first, such as the
+// real-FFT split): every input is read before the barrier, every output
+// written after it.
+template <int N, int NLINES, int SE, int SL, int NT, int SGN, bool I_FAST, class Load>
+__device__ __forceinline__ void fft_first_ct_inplace(float2 *buf, Load ld) {
+    constexpr StagePlan sp = CtPlan<N>::v;
+    static_assert(sp.ns >= 1, "no stage plan");
+    constexpr int R = sp.r[0], NR = N / R, TOTAL = NLINES * NR, KB = (TOTAL + NT - 1) / NT;
+    constexpr float sgn = (float)SGN;
+    float2 u[KB][8];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+        const int b = threadIdx.x + kb * NT;
+        if (TOTAL % NT != 0 && b >= TOTAL) continue;
+        const int i = I_FAST ? b % NR : b / NLINES, l = I_FAST ? b / NR : b - i * NLINES;
+#pragma unroll
+        for (int j = 0; j < R; ++j) u[kb][j] = ld(l, i + j * NR);
+        small_dft<R>(u[kb], sgn);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+        const int b = threadIdx.x + kb * NT;
+        if (TOTAL % NT != 0 && b >= TOTAL) continue;
+        const int i = I_FAST ? b % NR : b / NLINES, l = I_FAST ? b / NR : b - i * NLINES;
+        float2 *dst = buf + l * SL + i * R * SE;
+#pragma unroll
+        for (int j = 0; j < R; ++j) dst[j * SE] = u[kb][j];
+    }
+    __syncthreads();
+}
+
 template <int N, int NLINES, int SE, int SL, int NT, int SGN, int S = 0, int SEND = CtPlan<N>::v.ns>
 __device__ __forceinline__ void fft_inplace_ct(float2 *buf, const float2 *__restrict__ tw) {
     constexpr StagePlan sp = CtPlan<N>::v;

@@ -549,6 +549,15 @@ struct PlaneCt {
         fft_inplace_ct<nh, MY, 1, zh, NT, SGN, 0, CtPlan<nh>::v.ns - 1>(D, tw);
         fft_last_ct<nh, MY, 1, zh, NT, SGN, true>(D, tw, st);
     }
+    // the same with the first stage's inputs formed in place by ld(y, k)
+    // (the c2r pre-process) and the last stage's outputs to st(y, k, value)
+    template <int SGN, class Load, class Store>
+    __device__ void fft_z_from_to(float2 *D, const float2 *tw, Load ld, Store st) const {
+        static_assert(CtPlan<nh>::v.ns >= 2, "first and last stages must differ");
+        fft_first_ct_inplace<nh, MY, 1, zh, NT, SGN, true>(D, ld);
+        fft_inplace_ct<nh, MY, 1, zh, NT, SGN, 1, CtPlan<nh>::v.ns - 1>(D, tw);
+        fft_last_ct<nh, MY, 1, zh, NT, SGN, true>(D, tw, st);
+    }
     // columns: lines of My, element stride zh, line stride 1
     template <int SGN>
     __device__ void fft_y(float2 *D, const float2 *tw) const {
@@ -564,6 +573,15 @@ struct PlaneCt {
     template <int SGN, class Store>
     __device__ void fft_y_to(float2 *D, const float2 *tw, Store st) const {
         fft_inplace_ct<MY, zh, zh, 1, NT, SGN, 0, CtPlan<MY>::v.ns - 1>(D, tw);
+        fft_last_ct<MY, zh, zh, 1, NT, SGN, false>(D, tw, st);
+    }
+    // the same with the first stage's inputs formed in place by ld(kz, y)
+    // (the r2c split)
+    template <int SGN, class Load, class Store>
+    __device__ void fft_y_from_to(float2 *D, const float2 *tw, Load ld, Store st) const {
+        static_assert(CtPlan<MY>::v.ns >= 2, "first and last stages must differ");
+        fft_first_ct_inplace<MY, zh, zh, 1, NT, SGN, false>(D, ld);
+        fft_inplace_ct<MY, zh, zh, 1, NT, SGN, 1, CtPlan<MY>::v.ns - 1>(D, tw);
         fft_last_ct<MY, zh, zh, 1, NT, SGN, false>(D, tw, st);
     }
     static constexpr bool kCt = true;
@@ -614,6 +632,23 @@ __global__ void __launch_bounds__(G::nt, G::wpe) r2c_zy_plane_kernel(const float
         __syncthreads();
         g.template fft_z<-1>(D, twz);
     }
+    float2 *dst = out + plane * my * zh;
+    if constexpr (G::kCt) {
+        // the split X[m] = (Z[m] + Z*[nh-m])/2 + post[m] (Z[m] - Z*[nh-m])/(2i)
+        // (Z indices mod nh) formed element by element in the y pass's first
+        // stage, the last stage writing the plane to global memory
+        g.template fft_y_from_to<-1>(
+            D, twy,
+            [&](int kz, int y) -> float2 {
+                const float2 *row = D + y * zh;
+                const float2 zm = row[kz == nh ? 0 : kz], zc = row[kz == 0 ? 0 : nh - kz];
+                const float2 a = cscale(cadd(zm, cconj(zc)), 0.5f);
+                const float2 b = cmul_i(csub(zm, cconj(zc)), -0.5f);
+                return cadd(a, cmul(post[kz], b));
+            },
+            [&](int kz, int y, float2 v) { dst[y * zh + kz] = v; });
+        return;
+    }
     // split in place: the thread of pair (k, nh - k) writes X[k] and X[nh - k]
     // (k = 0 writes X[0] and X[nh]; k = nh/2 is its own pair)
     const int npairs = g.npairs;
@@ -634,14 +669,8 @@ __global__ void __launch_bounds__(G::nt, G::wpe) r2c_zy_plane_kernel(const float
         if (km != k) row[km] = xm;
     }
     __syncthreads();
-    float2 *dst = out + plane * my * zh;
-    if constexpr (G::kCt) {
-        // the y pass's last stage writes the plane to global memory
-        g.template fft_y_to<-1>(D, twy, [&](int kz, int y, float2 v) { dst[y * zh + kz] = v; });
-    } else {
-        g.template fft_y<-1>(D, twy);
-        for (int idx = threadIdx.x; idx < my * zh; idx += blockDim.x) dst[idx] = D[idx];
-    }
+    g.template fft_y<-1>(D, twy);
+    for (int idx = threadIdx.x; idx < my * zh; idx += blockDim.x) dst[idx] = D[idx];
 }
 
 // inverse: FFT along y of the Mz/2+1 columns -> per row Z'[k] = (X[k] + X*[nh-k])
@@ -723,6 +752,28 @@ __global__ void __launch_bounds__(G::nt, G::wpe) c2r_yz_plane_kernel(const float
         __syncthreads();
         g.template fft_y<1>(D, twy);
     }
+    float2 *dst = reinterpret_cast<float2 *>(out) + out_plane * my * nh;
+    if constexpr (G::kCt) {
+        // the pre-process Z[m] = (X[m] + X*[nh-m]) + i conj(post[m]) (X[m] - X*[nh-m])
+        // (m = 0 reads X[nh]; with -i k_z for d = 2) formed element by element in
+        // the z pass's first stage, the last stage writing the rows to global memory
+        g.template fft_z_from_to<1>(
+            D, twz,
+            [&](int y, int m) -> float2 {
+                const float2 *row = D + y * zh;
+                const int p = nh - m;
+                float2 xm = row[m], xp = row[p];
+                if (GRAD && d == 2) {  // column kz: k_z, zero on the Nyquist column
+                    xm = mul_minus_i(xm, 2 * m == mz ? 0.f : (float)(gk.kfz * m));
+                    xp = mul_minus_i(xp, 2 * p == mz ? 0.f : (float)(gk.kfz * p));
+                }
+                const float2 sm = cadd(xm, cconj(xp));
+                const float2 df = cmul(cconj(post[m]), csub(xm, cconj(xp)));
+                return cadd(sm, cmul_i(df, 1.f));
+            },
+            [&](int y, int k, float2 v) { dst[y * nh + k] = make_float2(v.x * scale, v.y * scale); });
+        return;
+    }
     // pre-process in place, pairs (k, nh - k) per thread; k = 0 reads X[nh]
     const int npairs = g.npairs;
     for (int idx = threadIdx.x; idx < my * npairs; idx += blockDim.x) {
@@ -745,19 +796,11 @@ __global__ void __launch_bounds__(G::nt, G::wpe) c2r_yz_plane_kernel(const float
         row[k] = zk;
     }
     __syncthreads();
-    float2 *dst = reinterpret_cast<float2 *>(out) + out_plane * my * nh;
-    if constexpr (G::kCt) {
-        // the z pass's last stage writes the rows to global memory
-        g.template fft_z_to<1>(D, twz, [&](int y, int k, float2 v) {
-            dst[y * nh + k] = make_float2(v.x * scale, v.y * scale);
-        });
-    } else {
-        g.template fft_z<1>(D, twz);
-        for (int idx = threadIdx.x; idx < my * nh; idx += blockDim.x) {
-            const int y = g.div_nh(idx);
-            const float2 v = D[y * zh + (idx - y * nh)];
-            dst[idx] = make_float2(v.x * scale, v.y * scale);
-        }
+    g.template fft_z<1>(D, twz);
+    for (int idx = threadIdx.x; idx < my * nh; idx += blockDim.x) {
+        const int y = g.div_nh(idx);
+        const float2 v = D[y * zh + (idx - y * nh)];
+        dst[idx] = make_float2(v.x * scale, v.y * scale);
     }
 }
 
