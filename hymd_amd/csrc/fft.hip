@@ -267,8 +267,7 @@ __global__ void __launch_bounds__(kBlock) c2r_z_ct_kernel(const float2 *__restri
     extern __shared__ float2 smem[];
     float2 *tw = smem;
     float2 *post = tw + NH;
-    float2 *A = post + ZH;
-    float2 *B = A + W * ZH;
+    float2 *A = post + ZH;  // W rows at stride ZH
     for (int i = threadIdx.x; i < NH - 1; i += blockDim.x) tw[i] = tw_glob[i];
     for (int i = threadIdx.x; i < ZH; i += blockDim.x) post[i] = post_glob[i];
     const unsigned flat = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
@@ -300,22 +299,27 @@ __global__ void __launch_bounds__(kBlock) c2r_z_ct_kernel(const float2 *__restri
         }
     }
     __syncthreads();
-    for (int idx = threadIdx.x; idx < W * NH; idx += blockDim.x) {
-        const int l = idx / NH, k = idx - l * NH;
+    // the pre-process formed element by element in the first stage, in place
+    static_assert(NS >= 2, "first and last stages must differ");
+    fft_first_ct_inplace<NH, W, 1, ZH, kBlock, 1, true>(A, [&](int l, int k) -> float2 {
         const float2 *xr = A + l * ZH;
-        float2 xk = xr[k];
-        float2 xc = cconj(xr[NH - k]);
-        float2 sm = cadd(xk, xc);
-        float2 df = cmul(cconj(post[k]), csub(xk, xc));
-        B[l * ZH + k] = cadd(sm, cmul_i(df, 1.f));
-    }
-    __syncthreads();
+        const float2 xk = xr[k];
+        const float2 xc = cconj(xr[NH - k]);
+        const float2 sm = cadd(xk, xc);
+        const float2 df = cmul(cconj(post[k]), csub(xk, xc));
+        return cadd(sm, cmul_i(df, 1.f));
+    });
     float2 *dst = reinterpret_cast<float2 *>(out) + (bofs + row0) * NH;
-    fft_inplace_ct<NH, W, 1, ZH, kBlock, 1, 0, NS - 1>(B, tw);
-    fft_last_ct<NH, W, 1, ZH, kBlock, 1, true>(B, tw, [&](int l, int e, float2 v) {
+    fft_inplace_ct<NH, W, 1, ZH, kBlock, 1, 1, NS - 1>(A, tw);
+    fft_last_ct<NH, W, 1, ZH, kBlock, 1, true>(A, tw, [&](int l, int e, float2 v) {
         if (l < wv) dst[l * NH + e] = make_float2(v.x * scale, v.y * scale);
     });
 }
+
+// dynamic LDS bytes of r2c_z_ct_kernel and c2r_z_ct_kernel: twiddles, post
+// factors and W rows
+template <int NH, int W>
+constexpr size_t z_ct_lds() { return sizeof(float2) * (size_t)(NH + (NH + 1) + W * (NH + 1)); }
 
 // In-place complex pass along a strided axis.  Line l -> base (l / inner) *
 // outer + (l % inner), element e at base + e * es.  2^wshift consecutive
@@ -932,7 +936,7 @@ int fft_r2c_zy(const FftPlan &plan, const float *in, float2 *out, int batch, hip
     const int64_t rows = (int64_t)m.mx * m.my;
     dim3 grid((unsigned)((rows + plan.wz - 1) / plan.wz), (unsigned)batch);
     if (m.mz == 256 && plan.wz == 16 && plan.azh.ld == 129 && same_plan(plan.azh, CtPlan<128>::v))
-        r2c_z_ct_kernel<128, 16><<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, (const float2 *)plan.tw_zh.ptr,
+        r2c_z_ct_kernel<128, 16><<<grid, kBlock, z_ct_lds<128, 16>(), s>>>(in, out, rows, (const float2 *)plan.tw_zh.ptr,
                                                                   (const float2 *)plan.post.ptr);
     else
         r2c_z_kernel<<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, m.mz, m.zh, plan.azh,
@@ -965,7 +969,7 @@ int fft_c2r_yz(const FftPlan &plan, float2 *in, float *out, int batch, float sca
     const int64_t rows = (int64_t)m.mx * m.my;
     dim3 grid((unsigned)((rows + plan.wz - 1) / plan.wz), (unsigned)batch);
     if (m.mz == 256 && plan.wz == 16 && plan.azh.ld == 129 && same_plan(plan.azh, CtPlan<128>::v))
-        c2r_z_ct_kernel<128, 16><<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, (const float2 *)plan.tw_zh.ptr,
+        c2r_z_ct_kernel<128, 16><<<grid, kBlock, z_ct_lds<128, 16>(), s>>>(in, out, rows, (const float2 *)plan.tw_zh.ptr,
                                                                   (const float2 *)plan.post.ptr, scale, 0.f);
     else
         c2r_z_kernel<<<grid, kBlock, plan.lds_z, s>>>(in, out, rows, m.mz, m.zh, plan.azh,
@@ -1011,7 +1015,7 @@ int fft_c2r_lines_grad(const FftPlan &plan, const float2 *in, float2 *tmp, float
     const int64_t rows = (int64_t)m.mx * m.my;
     dim3 gz((unsigned)((rows + plan.wz - 1) / plan.wz), (unsigned)(3 * T));
     if (m.mz == 256 && plan.wz == 16 && plan.azh.ld == 129 && same_plan(plan.azh, CtPlan<128>::v))
-        c2r_z_ct_kernel<128, 16><<<gz, kBlock, plan.lds_z, s>>>(tmp, out, rows, (const float2 *)plan.tw_zh.ptr,
+        c2r_z_ct_kernel<128, 16><<<gz, kBlock, z_ct_lds<128, 16>(), s>>>(tmp, out, rows, (const float2 *)plan.tw_zh.ptr,
                                                                 (const float2 *)plan.post.ptr, scale, (float)kfz);
     else
         c2r_z_kernel<<<gz, kBlock, plan.lds_z, s>>>(tmp, out, rows, m.mz, m.zh, plan.azh,
