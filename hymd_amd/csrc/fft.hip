@@ -822,13 +822,15 @@ static bool same_plan(const FftAxis &ax, const StagePlan &sp) {
 }
 
 // The compile-time strided pass when the axis has N points, W lines per
-// workgroup and the stage plan of CtPlan<N>; false if not.
+// workgroup and the stage plan of CtPlan<N>; false if not.  256-point lines
+// take the W of the caller (their kernels keep one LDS buffer, so 16 lines
+// fit); the others the plan's.
 template <int N, int W>
 static bool strided_ct(const FftPlan &plan, float2 *data, int batch, int axis, float sgn, hipStream_t s, int *err) {
     const Mesh &m = plan.mesh;
     const FftAxis &ax = axis == 1 ? plan.ay : plan.ax;
     const int w = axis == 1 ? plan.wy : plan.wx;
-    if (ax.n != N || w != W || !same_plan(ax, CtPlan<N>::v)) return false;
+    if (ax.n != N || (N != 256 && w != W) || !same_plan(ax, CtPlan<N>::v)) return false;
     const int nlines = axis == 1 ? m.mx * m.zh : m.my * m.zh;
     const int inner = axis == 1 ? m.zh : nlines;
     const int64_t outer = axis == 1 ? (int64_t)m.my * m.zh : 0;
@@ -852,7 +854,10 @@ static bool strided_ct(const FftPlan &plan, float2 *data, int batch, int axis, f
 static int launch_strided(const FftPlan &plan, float2 *data, int batch, int axis, float sgn, hipStream_t s) {
     const Mesh &m = plan.mesh;
     int err = HYMD_OK;
-    if (strided_ct<256, 8>(plan, data, batch, axis, sgn, s, &err) ||
+    // 256-point lines: 16 lines per workgroup (128-byte row segments, four
+    // workgroups per CU) for batches of 4 grids or more, else 8 (eight per CU)
+    if ((batch >= 4 ? strided_ct<256, 16>(plan, data, batch, axis, sgn, s, &err)
+                    : strided_ct<256, 8>(plan, data, batch, axis, sgn, s, &err)) ||
         strided_ct<160, 16>(plan, data, batch, axis, sgn, s, &err) ||
         strided_ct<128, 16>(plan, data, batch, axis, sgn, s, &err) ||
         strided_ct<64, 16>(plan, data, batch, axis, sgn, s, &err) ||
