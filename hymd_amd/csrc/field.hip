@@ -269,16 +269,29 @@ __global__ void __launch_bounds__(kBlock) paint_rows_wave_kernel(const float4 *_
 // which read the same grid rows, share one L2.
 // Particles come as binned records (x, y, z, type bits) or, with rec == NULL,
 // as plain xyz rows of type 0.
-template <int NCOMP>
+// SORTED (binned records, no device count): each block first orders its
+// particles by (type, z line of the grid) with a counting sort in LDS, so the
+// lanes of a wavefront gather from few grids and few cache lines; thread k
+// then reads out the block's k-th particle in that order.  Every particle's
+// arithmetic and output row are those of the unsorted form.
+constexpr int kReadoutBuckets = 256;  // (type, z line) buckets per block, + 1 for foreign types
+
+template <int NCOMP, bool SORTED>
 __global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict__ grids,
                                                           const float4 *__restrict__ rec,
                                                           const float *__restrict__ xyz,
                                                           const float *__restrict__ scale_per_particle,
                                                           int64_t n, int n_types, Mesh m,
                                                           float *__restrict__ out, const int *__restrict__ perm,
-                                                          const int *__restrict__ n_dev, int accumulate) {
+                                                          const int *__restrict__ n_dev, int accumulate, int zshift,
+                                                          int nzl) {
+    __shared__ int cnt[SORTED ? kReadoutBuckets + 1 : 1];
+    __shared__ int order[SORTED ? kBlock : 1];
     int64_t first, stride, end = n;
-    if (n_dev && gridDim.x >= kXcds) {
+    if (SORTED) {  // block-uniform chunks of blockDim.x particles
+        first = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
+        stride = (int64_t)gridDim.x * blockDim.x;
+    } else if (n_dev && gridDim.x >= kXcds) {
         // a device count of kept particles (a weighted binning): XCD x takes
         // the x-th eighth of the kept range with its own blocks (the hardware
         // deals blocks to the XCDs round robin), so each L2 sees neighbouring
@@ -294,7 +307,57 @@ __global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict
         first = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
         stride = (int64_t)gridDim.x * blockDim.x;
     }
-    for (int64_t i = first; i < end; i += stride) {
+    for (int64_t it = first; it < end; it += stride) {
+        int64_t i = it;
+        if constexpr (SORTED) {
+            const int nbk = n_types * nzl + 1;  // the last bucket: foreign types
+            __syncthreads();                    // the previous chunk's reads of order[] are done
+            for (int k = threadIdx.x; k < nbk; k += blockDim.x) cnt[k] = 0;
+            __syncthreads();
+            const int64_t ip = it + threadIdx.x;
+            int bk = 0, rank = 0;
+            if (ip < n) {
+                const float4 rp = rec[ip];
+                const int tp = __float_as_int(rp.w);
+                if ((unsigned)tp >= (unsigned)n_types) {
+                    bk = nbk - 1;
+                } else {
+                    const int zl = min(max((int)(rp.z * (float)m.inv_h[2]), 0), m.mz - 1) >> zshift;
+                    bk = tp * nzl + min(zl, nzl - 1);
+                }
+                rank = atomicAdd(&cnt[bk], 1);
+            }
+            __syncthreads();
+            if (threadIdx.x < 64) {  // exclusive scan of the counts, one wavefront
+                const int lane = threadIdx.x;
+                constexpr int kPer = (kReadoutBuckets + 1 + 63) / 64;
+                int v[kPer], sum = 0;
+#pragma unroll
+                for (int u = 0; u < kPer; ++u) {
+                    const int k = lane * kPer + u;
+                    v[u] = k < nbk ? cnt[k] : 0;
+                    sum += v[u];
+                }
+                int incl = sum;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int y = __shfl_up(incl, o, 64);
+                    if (lane >= o) incl += y;
+                }
+                int run = incl - sum;
+#pragma unroll
+                for (int u = 0; u < kPer; ++u) {
+                    const int k = lane * kPer + u;
+                    if (k < nbk) cnt[k] = run;
+                    run += v[u];
+                }
+            }
+            __syncthreads();
+            if (ip < n) order[cnt[bk] + rank] = threadIdx.x;
+            __syncthreads();
+            if (ip >= n) continue;
+            i = it + order[threadIdx.x];
+        }
         const float4 r = rec ? rec[i] : make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.f);
         const int t = __float_as_int(r.w);
         const int64_t o = perm ? (int64_t)perm[i] : i;
@@ -1469,12 +1532,27 @@ static int readout_sorted(hymd_pm *pm, int ncomp, int n_types, const float *grid
         // a weighted binning keeps a count known on the device only: a grid
         // of at most 4096 blocks loops over it
         const int nb = kept ? grid_for(n, bs, 4096) : grid_for(n, bs);
-        if (ncomp == 3)
-            readout_kernel<3><<<nb, bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh, out, pp, kept,
-                                                         accumulate ? 1 : 0);
-        else
-            readout_kernel<1><<<nb, bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh, out, pp, kept,
-                                                         accumulate ? 1 : 0);
+        const int acc = accumulate ? 1 : 0;
+        if (kept) {
+            if (ncomp == 3)
+                readout_kernel<3, false><<<nb, bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh,
+                                                                    out, pp, kept, acc, 0, 0);
+            else
+                readout_kernel<1, false><<<nb, bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh,
+                                                                    out, pp, kept, acc, 0, 0);
+        } else {
+            // z lines of 32 floats (one 128-byte cache line), coarser when
+            // the (type, line) buckets would exceed kReadoutBuckets
+            int zshift = 5;
+            while (n_types * (((pm->mesh.mz - 1) >> zshift) + 1) > kReadoutBuckets) ++zshift;
+            const int nzl = ((pm->mesh.mz - 1) >> zshift) + 1;
+            if (ncomp == 3)
+                readout_kernel<3, true><<<nb, bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh, out,
+                                                                   pp, nullptr, acc, zshift, nzl);
+            else
+                readout_kernel<1, true><<<nb, bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh, out,
+                                                                   pp, nullptr, acc, zshift, nzl);
+        }
     }
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
@@ -1541,8 +1619,8 @@ extern "C" int hymd_fft_c2r(hymd_pm *pm, const void *in, float *out, int32_t bat
 extern "C" int hymd_readout(hymd_pm *pm, const float *grid, const float *x, int64_t n, float *out) {
     HYMD_REQUIRE(pm && grid && out && (x || n == 0) && n >= 0, "bad argument");
     if (n > 0)
-        readout_kernel<1><<<grid_for(n), kBlock, 0, pm->stream>>>(grid, nullptr, x, nullptr, n, 1, pm->mesh, out,
-                                                                    nullptr, nullptr, 0);
+        readout_kernel<1, false><<<grid_for(n), kBlock, 0, pm->stream>>>(grid, nullptr, x, nullptr, n, 1, pm->mesh,
+                                                                           out, nullptr, nullptr, 0, 0, 0);
     HYMD_HIP(hipGetLastError());
     return HYMD_OK;
 }

@@ -80,6 +80,24 @@ def test_fft_vs_numpy(hip, mesh):
     np.testing.assert_allclose(y / G, x, atol=2e-5 * np.abs(x).max())
 
 
+# 256-point strided passes take 16 lines per workgroup for batches of 4
+# grids or more and 8 below: y lines (8 x 256 x 256, line passes) and x lines
+# (256 x 16 x 16, after the plane kernels), both directions, both widths
+@pytest.mark.parametrize("mesh", [(8, 256, 256), (256, 16, 16)])
+@pytest.mark.parametrize("B", [2, 6])
+def test_fft_256_lines_batched_vs_numpy(hip, mesh, B):
+    rng = np.random.default_rng(B * 1000 + mesh[0])
+    pm = _pm(mesh, (1.0, 1.0, 1.0), 1)
+    x = rng.normal(size=(B,) + mesh).astype(np.float32)
+    X = pm.r2c(_dev(x, torch.float32)).cpu().numpy()
+    ref = np.fft.rfftn(x.astype(np.float64), axes=(1, 2, 3))
+    for b in range(B):
+        assert np.sqrt(np.mean(np.abs(X[b] - ref[b]) ** 2)) / np.sqrt(np.mean(np.abs(ref[b]) ** 2)) < 2e-6, b
+    y = pm.c2r(_dev(ref.astype(np.complex64), torch.complex64)).cpu().numpy()
+    G = np.prod(mesh)
+    np.testing.assert_allclose(y / G, x, atol=2e-5 * np.abs(x).max())
+
+
 def _diblock(n_chains=600, box=8.0):
     from hymd_amd import synth
     return synth.diblock_melt(n_chains=n_chains, box=box, mesh=32, seed=1000)
