@@ -274,8 +274,6 @@ __global__ void __launch_bounds__(kBlock) paint_rows_wave_kernel(const float4 *_
 // lanes of a wavefront gather from few grids and few cache lines; thread k
 // then reads out the block's k-th particle in that order.  Every particle's
 // arithmetic and output row are those of the unsorted form.
-constexpr int kReadoutBuckets = 256;  // (type, z line) buckets per block, + 1 for foreign types
-
 template <int NCOMP, bool SORTED>
 __global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict__ grids,
                                                           const float4 *__restrict__ rec,
@@ -285,7 +283,7 @@ __global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict
                                                           float *__restrict__ out, const int *__restrict__ perm,
                                                           const int *__restrict__ n_dev, int accumulate, int zshift,
                                                           int nzl) {
-    __shared__ int cnt[SORTED ? kReadoutBuckets + 1 : 1];
+    __shared__ int cnt[SORTED ? kSortBuckets + 1 : 1];
     __shared__ int order[SORTED ? kBlock : 1];
     int64_t first, stride, end = n;
     if (SORTED) {  // block-uniform chunks of blockDim.x particles
@@ -310,53 +308,18 @@ __global__ void __launch_bounds__(kBlock) readout_kernel(const float *__restrict
     for (int64_t it = first; it < end; it += stride) {
         int64_t i = it;
         if constexpr (SORTED) {
-            const int nbk = n_types * nzl + 1;  // the last bucket: foreign types
-            __syncthreads();                    // the previous chunk's reads of order[] are done
-            for (int k = threadIdx.x; k < nbk; k += blockDim.x) cnt[k] = 0;
-            __syncthreads();
             const int64_t ip = it + threadIdx.x;
-            int bk = 0, rank = 0;
+            int bk = 0;
             if (ip < n) {
                 const float4 rp = rec[ip];
                 const int tp = __float_as_int(rp.w);
-                if ((unsigned)tp >= (unsigned)n_types) {
-                    bk = nbk - 1;
-                } else {
-                    const int zl = min(max((int)(rp.z * (float)m.inv_h[2]), 0), m.mz - 1) >> zshift;
-                    bk = tp * nzl + min(zl, nzl - 1);
-                }
-                rank = atomicAdd(&cnt[bk], 1);
+                bk = (unsigned)tp >= (unsigned)n_types
+                         ? n_types * nzl  // the last bucket: foreign types
+                         : tp * nzl + min(z_line(rp.z, m.inv_h[2], m.mz, zshift), nzl - 1);
             }
-            __syncthreads();
-            if (threadIdx.x < 64) {  // exclusive scan of the counts, one wavefront
-                const int lane = threadIdx.x;
-                constexpr int kPer = (kReadoutBuckets + 1 + 63) / 64;
-                int v[kPer], sum = 0;
-#pragma unroll
-                for (int u = 0; u < kPer; ++u) {
-                    const int k = lane * kPer + u;
-                    v[u] = k < nbk ? cnt[k] : 0;
-                    sum += v[u];
-                }
-                int incl = sum;
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    const int y = __shfl_up(incl, o, 64);
-                    if (lane >= o) incl += y;
-                }
-                int run = incl - sum;
-#pragma unroll
-                for (int u = 0; u < kPer; ++u) {
-                    const int k = lane * kPer + u;
-                    if (k < nbk) cnt[k] = run;
-                    run += v[u];
-                }
-            }
-            __syncthreads();
-            if (ip < n) order[cnt[bk] + rank] = threadIdx.x;
-            __syncthreads();
-            if (ip >= n) continue;
-            i = it + order[threadIdx.x];
+            const int k = block_bucket_order(ip < n, bk, n_types * nzl + 1, cnt, order);
+            if (k < 0) continue;
+            i = it + k;
         }
         const float4 r = rec ? rec[i] : make_float4(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 0.f);
         const int t = __float_as_int(r.w);
@@ -1542,10 +1505,10 @@ static int readout_sorted(hymd_pm *pm, int ncomp, int n_types, const float *grid
                                                                     out, pp, kept, acc, 0, 0);
         } else {
             // z lines of 32 floats (one 128-byte cache line), coarser when
-            // the (type, line) buckets would exceed kReadoutBuckets
-            int zshift = 5;
-            while (n_types * (((pm->mesh.mz - 1) >> zshift) + 1) > kReadoutBuckets) ++zshift;
-            const int nzl = ((pm->mesh.mz - 1) >> zshift) + 1;
+            // the (type, line) buckets would exceed kSortBuckets
+            int zshift = 0;
+            while (n_types * (((pm->mesh.mz - 1) >> (5 + zshift)) + 1) > kSortBuckets) ++zshift;
+            const int nzl = ((pm->mesh.mz - 1) >> (5 + zshift)) + 1;
             if (ncomp == 3)
                 readout_kernel<3, true><<<nb, bs, 0, pm->stream>>>(grids, rec, nullptr, ws, n, n_types, pm->mesh, out,
                                                                    pp, nullptr, acc, zshift, nzl);

@@ -17,6 +17,55 @@ namespace hymd {
 int launch_key_scatter(const float *x, const int32_t *types, const float *w, int64_t n, const int2 *keys,
                        const int *start, int *perm, float4 *rec, float *ws, hipStream_t s);
 
+// Block-local counting sort for the field readout: thread k holds item k (valid
+// for a prefix of the block's threads) with bucket bk in [0, nbk), nbk <=
+// kSortBuckets + 1; returns the block-local index of the item thread k takes
+// in bucket order (-1 for the invalid threads).  The order within a bucket
+// is that of the LDS atomics.  Every thread of the block calls it (it holds
+// barriers); cnt has kSortBuckets + 1 ints and order blockDim.x ints of LDS.
+constexpr int kSortBuckets = 256;
+
+__device__ __forceinline__ int block_bucket_order(bool valid, int bk, int nbk, int *cnt, int *order) {
+    __syncthreads();  // the previous call's reads of cnt[] and order[] are done
+    for (int k = threadIdx.x; k < nbk; k += blockDim.x) cnt[k] = 0;
+    __syncthreads();
+    const int rank = valid ? atomicAdd(&cnt[bk], 1) : 0;
+    __syncthreads();
+    if (threadIdx.x < 64) {  // exclusive scan of the counts, one wavefront
+        const int lane = threadIdx.x;
+        constexpr int kPer = (kSortBuckets + 1 + 63) / 64;
+        int v[kPer], sum = 0;
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int k = lane * kPer + u;
+            v[u] = k < nbk ? cnt[k] : 0;
+            sum += v[u];
+        }
+        int incl = sum;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += y;
+        }
+        int run = incl - sum;
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int k = lane * kPer + u;
+            if (k < nbk) cnt[k] = run;
+            run += v[u];
+        }
+    }
+    __syncthreads();
+    if (valid) order[cnt[bk] + rank] = threadIdx.x;
+    __syncthreads();
+    return valid ? order[threadIdx.x] : -1;
+}
+
+// z line (32 floats, one 128-byte cache line, >> zshift) of a position
+__device__ __forceinline__ int z_line(float z, double inv_h, int mz, int zshift) {
+    return min(max((int)(z * (float)inv_h), 0), mz - 1) >> (5 + zshift);
+}
+
 struct PaintArgs {
     int n_types;
     double scale;  // 2^HYMD_FIXED_SHIFT
