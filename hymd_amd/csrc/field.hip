@@ -1215,26 +1215,28 @@ static bool xline_ct_plan(const FftPlan &f) {
 }
 
 // The x-line pass for T types in the shape sh: a compile-time geometry for
-// Mx = 128 or 160 at the 256-thread W that xline_grad_shape picks there and
-// for Mx = 256 with six types at 8 lines of 1024 threads or 4 of 512, the
-// runtime one otherwise.
+// Mx = 128 or 160 at the 256-thread W that xline_grad_shape picks there (run
+// at 512 threads for Mx = 128) and for Mx = 256 with six types at 8 lines of
+// 1024 threads or 4 of 512, the runtime one otherwise.
 static int launch_xline_grad(hymd_pm *pm, const float2 *spec, float2 *out, const KGeom &g, const FieldK &p,
                              const XlineShape &sh, double *partials, int *nblocks, const PmeK *pme = nullptr) {
     const FftPlan &f = pm->fft;
     const int T = p.T, W = sh.W;
-#define HYMD_XLINE_CT(NX, TT, WW, MT)                                                                      \
+    // NTH threads per workgroup for the 256-thread shape's lines and LDS: the
+    // Mx = 128 geometries run at 512, twice the waves in flight (the 128^3,
+    // T = 4 pass 33.6 -> 29.8 us); at 160 that made the peptide step slower
+#define HYMD_XLINE_CT(NX, TT, WW, MT, NTH)                                                                 \
     if (sh.NT == kXlineGradThreads && f.ax.n == NX && T == TT && W == WW && xline_ct_plan<NX>(f))          \
-        return launch_xline_grad_g<MT, kXlineGradThreads>(pm, spec, out, g, p, W, XlineCt<TT, NX, WW>{}, partials, \
-                                                          nblocks, pme);
-    HYMD_XLINE_CT(128, 1, 16, 2)
-    HYMD_XLINE_CT(128, 2, 16, 2)
-    HYMD_XLINE_CT(128, 3, 8, 4)
-    HYMD_XLINE_CT(128, 4, 8, 4)
-    HYMD_XLINE_CT(160, 1, 8, 2)
-    HYMD_XLINE_CT(160, 2, 8, 2)
-    HYMD_XLINE_CT(160, 3, 4, 4)
-    HYMD_XLINE_CT(160, 4, 4, 4)
-    HYMD_XLINE_CT(256, 1, 8, 2)
+        return launch_xline_grad_g<MT, NTH>(pm, spec, out, g, p, W, XlineCt<TT, NX, WW>{}, partials, nblocks, pme);
+    HYMD_XLINE_CT(128, 1, 16, 2, 2 * kXlineGradThreads)
+    HYMD_XLINE_CT(128, 2, 16, 2, 2 * kXlineGradThreads)
+    HYMD_XLINE_CT(128, 3, 8, 4, 2 * kXlineGradThreads)
+    HYMD_XLINE_CT(128, 4, 8, 4, 2 * kXlineGradThreads)
+    HYMD_XLINE_CT(160, 1, 8, 2, kXlineGradThreads)
+    HYMD_XLINE_CT(160, 2, 8, 2, kXlineGradThreads)
+    HYMD_XLINE_CT(160, 3, 4, 4, kXlineGradThreads)
+    HYMD_XLINE_CT(160, 4, 4, 4, kXlineGradThreads)
+    HYMD_XLINE_CT(256, 1, 8, 2, kXlineGradThreads)
 #undef HYMD_XLINE_CT
     // 256-point lines with six types (the 256^3 workloads): 8 lines of 1024
     // threads or 4 lines of 512
