@@ -249,7 +249,8 @@ def test_drop_in_field_surface(hip):
 @pytest.mark.parametrize("mesh,T", [((32, 48, 40), 3), ((24, 24, 24), 2), ((16, 20, 12), 5), ((48, 40, 64), 4),
                                     ((128, 96, 80), 6), ((64, 64, 64), 1), ((40, 30, 18), 8),
                                     # compile-time x-line (Mx 128, 160) and plane (128^2, 160^2) geometries
-                                    ((128, 40, 24), 4), ((128, 24, 20), 2), ((160, 20, 16), 3), ((160, 16, 24), 1),
+                                    ((128, 40, 24), 4), ((128, 24, 20), 2), ((128, 20, 16), 3), ((128, 16, 12), 1),
+                                    ((160, 20, 16), 3), ((160, 16, 24), 1),
                                     ((16, 128, 128), 4), ((8, 160, 160), 2),
                                     # long x lines with many types: 8 lines per workgroup of 1024 threads
                                     ((192, 16, 16), 6), ((256, 12, 10), 5), ((512, 8, 8), 4), ((160, 24, 20), 7)])
