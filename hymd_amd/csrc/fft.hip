@@ -369,8 +369,8 @@ __global__ void __launch_bounds__(kBlock) fft_strided_kernel(float2 *__restrict_
 // workgroup and sign SGN (the plan's own stage plan and twiddles): the first
 // stage reads the lines from global memory, the middle stages run in place
 // in LDS and the last stage writes the lines back.
-template <int N, int W, int SGN>
-__global__ void __launch_bounds__(kBlock) fft_strided_ct_kernel(float2 *__restrict__ data, int64_t batch_stride,
+template <int N, int W, int SGN, int NT>
+__global__ void __launch_bounds__(NT) fft_strided_ct_kernel(float2 *__restrict__ data, int64_t batch_stride,
                                                                  int nlines, int inner, int64_t outer, int64_t es,
                                                                  const float2 *__restrict__ tw_glob) {
     constexpr int NS = CtPlan<N>::v.ns;
@@ -381,17 +381,17 @@ __global__ void __launch_bounds__(kBlock) fft_strided_ct_kernel(float2 *__restri
     const unsigned bx = flat % gridDim.x, by = flat / gridDim.x;
     float2 *d = data + (int64_t)by * batch_stride;
     const int line0 = (int)bx * W;
-    for (int i = threadIdx.x; i < N - 1; i += kBlock) tw[i] = tw_glob[i];
+    for (int i = threadIdx.x; i < N - 1; i += NT) tw[i] = tw_glob[i];
     auto base = [&](int l) -> int64_t {
         const int line = line0 + l;
         return line < nlines ? (int64_t)(line / inner) * outer + (line % inner) : -1;
     };
-    fft_first_ct<N, W, 1, N + 1, kBlock, SGN>(A, [&](int l, int e) -> float2 {
+    fft_first_ct<N, W, 1, N + 1, NT, SGN>(A, [&](int l, int e) -> float2 {
         const int64_t b = base(l);
         return b >= 0 ? d[b + (int64_t)e * es] : make_float2(0.f, 0.f);
     });
-    fft_inplace_ct<N, W, 1, N + 1, kBlock, SGN, 1, NS - 1>(A, tw);
-    fft_last_ct<N, W, 1, N + 1, kBlock, SGN, false>(A, tw, [&](int l, int e, float2 v) {
+    fft_inplace_ct<N, W, 1, N + 1, NT, SGN, 1, NS - 1>(A, tw);
+    fft_last_ct<N, W, 1, N + 1, NT, SGN, false>(A, tw, [&](int l, int e, float2 v) {
         const int64_t b = base(l);
         if (b >= 0) d[b + (int64_t)e * es] = v;
     });
@@ -838,10 +838,13 @@ static bool strided_ct(const FftPlan &plan, float2 *data, int batch, int axis, f
     dim3 grid((unsigned)((nlines + W - 1) / W), (unsigned)batch);
     const size_t lds = sizeof(float2) * (N + W * (N + 1));
     const float2 *tw = (const float2 *)(axis == 1 ? plan.tw_y.ptr : plan.tw_x.ptr);
+    // 256-point lines at 16 per workgroup hold 35 KB of LDS, four workgroups
+    // per CU: 512 threads keep twice the waves in flight
+    constexpr int NT = (N == 256 && W == 16) ? 2 * kBlock : kBlock;
     if (sgn < 0.f)
-        fft_strided_ct_kernel<N, W, -1><<<grid, kBlock, lds, s>>>(data, m.gc, nlines, inner, outer, es, tw);
+        fft_strided_ct_kernel<N, W, -1, NT><<<grid, NT, lds, s>>>(data, m.gc, nlines, inner, outer, es, tw);
     else
-        fft_strided_ct_kernel<N, W, 1><<<grid, kBlock, lds, s>>>(data, m.gc, nlines, inner, outer, es, tw);
+        fft_strided_ct_kernel<N, W, 1, NT><<<grid, NT, lds, s>>>(data, m.gc, nlines, inner, outer, es, tw);
     const hipError_t e = hipGetLastError();
     *err = HYMD_OK;
     if (e != hipSuccess) {
